@@ -1,0 +1,251 @@
+/*
+ * wflow_hip.h - C ABI of the MI355X (gfx950) implementation of the Wflow.jl `sbm` time step.
+ *
+ * The replaceable unit in the reference is
+ *     update_model!(model::AbstractModel{<:SbmModel})        Wflow/src/sbm_model.jl:60-92
+ * called through the step hook
+ *     run_timestep!(model; update_func = update_model!)      Wflow/src/Wflow.jl:311-322
+ * One wfh_step() call performs exactly one update_model!: land hydrology (interception, snow,
+ * open-water runoff, SBM soil column), lateral subsurface flow (kinematic wave), the soil
+ * storage update, overland and river kinematic-wave routing and the total-storage update.
+ * storage_prev!/compute_mass_balance! (Wflow.jl:315-317) sit outside update_model! in the
+ * reference; here they are optional and fused into the same step (wfh_config.water_mass_balance).
+ *
+ * Conventions
+ *  - All quantities are in the reference's internal SI units (m, s, K, m3 s-1; units.jl:98-121).
+ *  - Land arrays have n entries in land-index order (active cells in column-major raster order,
+ *    Wflow/src/utils.jl:85-99); river arrays have nriv entries ordered by ascending land index
+ *    of the river cells (Wflow/src/network.jl:214-258).
+ *  - Layered arrays are layer-major: a[k * n + i] is layer k of cell i (the reference stores
+ *    Vector{SVector{N}}; the host shim converts).
+ *  - Every function returns WFH_OK (0) or a negative status; no exception crosses the ABI.
+ *    wfh_last_error() gives the message of the last failure on a handle.
+ *  - A handle owns one HIP stream and its device memory. Calls on one handle must come from
+ *    one host thread at a time. wfh_step() is asynchronous; wfh_get()/wfh_sync() synchronise.
+ */
+#ifndef WFLOW_HIP_H
+#define WFLOW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WFH_ABI_VERSION 1
+#define WFH_MAX_LAYERS 8
+
+enum wfh_status {
+    WFH_OK = 0,
+    WFH_ERR_ARG = -1,       /* invalid argument (null pointer, size, enum)            */
+    WFH_ERR_HIP = -2,       /* a HIP runtime call failed                               */
+    WFH_ERR_CYCLE = -3,     /* the drainage network has a cycle (routing/utils.jl:26) */
+    WFH_ERR_NONFINITE = -4, /* reserved: non-finite input rejected                     */
+    WFH_ERR_DEVICE = -5,    /* a kernel reported a failure (dependency wait timed out) */
+    WFH_ERR_STATE = -6      /* call out of order (e.g. wfh_step before wfh_set_forcing) */
+};
+
+/* WFH_MEM_HOST / WFH_MEM_DEVICE: arrays in land-index order in host / device memory.
+ * WFH_MEM_DEVICE_NODE_ORDER: device arrays already in the library's node order
+ * (wfh_get_node_order), bound without a copy. */
+enum wfh_memkind { WFH_MEM_HOST = 0, WFH_MEM_DEVICE = 1, WFH_MEM_DEVICE_NODE_ORDER = 2 };
+
+/* Model settings: the fields of the reference's `ModelSection` (config_structure.jl:62-115)
+ * that change the per-step arithmetic, plus the model time step. */
+typedef struct wfh_config {
+    int32_t abi_version;                 /* = WFH_ABI_VERSION                                  */
+    int32_t device;                      /* HIP device ordinal                                 */
+    double dt;                           /* model time step [s] (Clock.dt)                     */
+    double land_dt;                      /* land_kinematic_wave__time_step [s], default 3600    */
+    double river_dt;                     /* river_kinematic_wave__time_step [s], default 900    */
+    double subsurface_dt;                /* subsurface_kinematic_wave__time_step, default 86400 */
+    int32_t n_config_layers;             /* length of soil_layer__thickness (N - 1)            */
+    int32_t snow;                        /* snow__flag                                         */
+    double soil_layer_thickness_mm[WFH_MAX_LAYERS]; /* soil_layer__thickness [mm]            */
+    int32_t soil_infiltration_reduction; /* soil_infiltration_reduction__flag                  */
+    int32_t water_mass_balance;          /* water_mass_balance__flag                           */
+} wfh_config;
+
+/* Static maps as the reference reads them (standard names in brackets), already in SI.
+ * Pointers are host pointers and are only read during wfh_create(). */
+typedef struct wfh_static {
+    int64_t nx, ny;               /* raster size                                            */
+    int64_t n;                    /* number of active land cells                            */
+    int64_t nriv;                 /* number of river cells (= count of river_location)     */
+    const int64_t* cell_index;    /* [n] linear raster index x + y*nx, strictly ascending   */
+    const uint8_t* ldd;           /* [n] basin__local_drain_direction (PCRaster codes 1-9)  */
+    const double* x_length;       /* [n] cell length in x [m]                               */
+    const double* y_length;       /* [n] cell length in y [m]                               */
+    const double* land_slope;     /* [n] land_surface__slope (clamped to >= 1e-5)           */
+    const double* land_manning_n; /* [n] land_surface_water_flow__manning_n_parameter       */
+    const uint8_t* river_location;/* [n] river_location__mask                               */
+    const double* river_width_land;  /* [n] river__width on the land grid (river fraction)  */
+    const double* river_length_land; /* [n] river__length on the land grid                  */
+    const double* water_fraction; /* [n] land_water_covered__area_fraction                  */
+    const double* elevation;      /* [n] land_surface__elevation                            */
+    /* vegetation */
+    const double* canopy_gap_fraction;   /* [n] */
+    const double* max_canopy_storage;    /* [n] vegetation_water__storage_capacity [m]      */
+    const double* rooting_depth;         /* [n] vegetation_root__depth [m]                  */
+    const double* crop_coefficient;      /* [n] vegetation__crop_factor                     */
+    const double* evap_precip_ratio;     /* [n] Gash E/R ratio (used when dt >= 23 h)       */
+    /* snow (HBV) */
+    const double* cfmax;          /* [n] snowpack__degree_day_coefficient [m K-1 s-1]       */
+    const double* tt;             /* [n] snowfall temperature threshold [K]                 */
+    const double* tti;            /* [n] snowfall temperature interval [K]                  */
+    const double* ttm;            /* [n] melting temperature threshold [K]                  */
+    const double* whc;            /* [n] liquid water holding capacity [-]                  */
+    /* SBM soil */
+    const double* theta_s, *theta_r;
+    const double* theta_fc;       /* [n] or NULL: Brooks-Corey field capacity (soil_process.jl:138-160) */
+    const double* kv_0;           /* [n] surface vertical Ksat [m s-1]                      */
+    const double* f;              /* [n] Ksat scale parameter [m-1]                         */
+    const double* air_entry_pressure; /* [n] [m]                                            */
+    const double* h1, *h2, *h3_high, *h3_low, *h4, *alpha_h1; /* Feddes [m], [-]            */
+    const double* soil_thickness; /* [n] [m]                                                */
+    const double* infiltcap_compacted; /* [n] [m s-1]                                       */
+    const double* max_leakage;    /* [n] [m s-1]                                            */
+    const double* compacted_fraction; /* [n]                                                */
+    const double* rootdistpar;    /* [n] wet-root sigmoid parameter [m-1]                   */
+    const double* cap_hmax;       /* [n] [m]                                                */
+    const double* cap_n;          /* [n]                                                    */
+    const double* w_soil;         /* [n]                                                    */
+    const double* cf_soil;        /* [n]                                                    */
+    const double* brooks_corey_c; /* [N][n] layer-major, N = n_config_layers + 1            */
+    const double* kvfrac;         /* [N][n]                                                 */
+    const double* rootfraction;   /* [N][n] or NULL: derived from rooting_depth (soil.jl:526-548) */
+    /* lateral subsurface flow */
+    const double* khfrac;         /* [n] horizontal/vertical Ksat ratio                     */
+    /* river (nriv entries) */
+    const double* river_slope;    /* river__slope (clamped to >= 1e-5)                     */
+    const double* river_manning_n;
+    const double* river_bankfull_depth;
+    const double* river_length;
+    const double* river_width;
+} wfh_static;
+
+/* Readable / writable variables (BMI get_value / set_value equivalents, bmi.jl:203-262). */
+typedef enum wfh_var {
+    /* land states [n] */
+    WFH_CANOPY_STORAGE = 0,
+    WFH_SNOW_STORAGE,
+    WFH_SNOW_WATER,
+    WFH_SOIL_TEMPERATURE,
+    WFH_SATWATERDEPTH,
+    WFH_USTORELAYERDEPTH,        /* [N][n] */
+    /* land fluxes / diagnostics [n] */
+    WFH_WATER_TABLE_DEPTH,
+    WFH_USTOREDEPTH,
+    WFH_TRANSFER,
+    WFH_RECHARGE,
+    WFH_ACTUAL_INFILTRATION,
+    WFH_INFILTRATION_EXCESS,
+    WFH_SATURATION_EXCESS,
+    WFH_TRANSPIRATION,
+    WFH_SOIL_EVAPORATION,
+    WFH_ACTUAL_CAPILLARY_FLUX,
+    WFH_ACTUAL_LEAKAGE,
+    WFH_ACTUAL_EVAPOTRANSPIRATION,
+    WFH_INTERCEPTION_RATE,
+    WFH_NET_RUNOFF,
+    WFH_RUNOFF,
+    WFH_NET_RUNOFF_RIVER,
+    WFH_EXFILTRATION,
+    WFH_TOTAL_STORAGE,
+    WFH_SNOW_WATER_EQUIVALENT,
+    WFH_VWC,                     /* [N][n] volumetric water content               */
+    WFH_VWC_ROOT_ZONE,
+    WFH_ROOT_ZONE_STORAGE,
+    /* lateral subsurface flow [n] */
+    WFH_SSF_Q,
+    WFH_SSF_QIN_AVERAGE,
+    WFH_SSF_Q_AVERAGE,
+    WFH_SSF_TO_RIVER_AVERAGE,
+    WFH_SSF_STORAGE,
+    WFH_SSF_QMAX,
+    /* overland flow [n] */
+    WFH_LAND_Q,
+    WFH_LAND_QIN,
+    WFH_LAND_H,
+    WFH_LAND_STORAGE,
+    WFH_LAND_Q_AVERAGE,
+    WFH_LAND_QIN_AVERAGE,
+    WFH_LAND_TO_RIVER_AVERAGE,
+    WFH_LAND_INWATER,
+    /* river flow [nriv] */
+    WFH_RIVER_Q,
+    WFH_RIVER_QIN,
+    WFH_RIVER_H,
+    WFH_RIVER_STORAGE,
+    WFH_RIVER_Q_AVERAGE,
+    WFH_RIVER_QIN_AVERAGE,
+    WFH_RIVER_INWATER,
+    /* water balance (water_mass_balance = 1) */
+    WFH_BALANCE_LAND_ERROR,      /* [n]    */
+    WFH_BALANCE_LAND_RELERROR,   /* [n]    */
+    WFH_BALANCE_SSF_ERROR,       /* [n]    */
+    WFH_BALANCE_OVERLAND_ERROR,  /* [n]    */
+    WFH_BALANCE_RIVER_ERROR,     /* [nriv] */
+    /* derived parameters (read only) */
+    WFH_LAND_ALPHA,              /* [n]    */
+    WFH_RIVER_ALPHA,             /* [nriv] */
+    WFH_THETA_FC,                /* [n]    */
+    WFH_SURFACE_FLOW_WIDTH,      /* [n]    */
+    WFH_FLOW_FRACTION_TO_RIVER,  /* [n]    */
+    WFH_VAR_COUNT
+} wfh_var;
+
+typedef struct wfh_handle wfh_handle;
+
+/* Model(config) for the sbm concept: derives topology (flowgraph, topological order, upstream
+ * lists) and parameters, sets the cold-start states (sbm_model.jl:7-57, soil.jl:150-201,
+ * utils.jl:916-937) and uploads everything to the device.
+ * Replaces: Model(config, ::SbmModel)  Wflow/src/sbm_model.jl:7  */
+int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_handle** out);
+
+/* Forcing for the next step (precipitation [m s-1], potential evaporation [m s-1],
+ * temperature [K]); memkind WFH_MEM_HOST copies, WFH_MEM_DEVICE binds caller device buffers
+ * that must stay valid until the step has completed.
+ * Replaces: update_forcing!/load_dynamic_input!  Wflow/src/io.jl:108-178  */
+int32_t wfh_set_forcing(wfh_handle* h, const double* precipitation, const double* potential_evaporation,
+                        const double* temperature, int32_t memkind);
+
+/* One update_model! step.  Replaces: update_model!  Wflow/src/sbm_model.jl:60-92 */
+int32_t wfh_step(wfh_handle* h);
+
+/* BMI.get_value / set_value (Wflow/src/bmi.jl:203-262), land-index / river-index order.
+ * count = number of doubles the caller provides (n, nriv or N*n). */
+int32_t wfh_get(wfh_handle* h, int32_t var, double* dst, int64_t count, int32_t memkind);
+int32_t wfh_set(wfh_handle* h, int32_t var, const double* src, int64_t count, int32_t memkind);
+
+/* Topology as the reference builds it: topological order of the land and river graphs
+ * (Graphs.topological_sort_by_dfs, network.jl:99/244) as 0-based node ids, and the flow-graph
+ * edges (downstream node or -1 for pits).  Any pointer may be NULL.  Replaces: NetworkLand /
+ * NetworkRiver fields order, graph (network.jl:48-258). */
+int32_t wfh_get_topology(wfh_handle* h, int64_t* land_order, int64_t* land_down, int64_t* river_order,
+                         int64_t* river_down, int64_t* river_land_index, int32_t* max_level);
+
+/* The library's internal cell order (topological level, then land index): land_index_of_node[q]
+ * is the land index stored at node q. No counterpart in the reference; it lets a caller produce
+ * forcing directly in node order (WFH_MEM_DEVICE_NODE_ORDER). */
+int32_t wfh_get_node_order(wfh_handle* h, int64_t* land_index_of_node);
+
+/* Raw state checkpoint of all prognostic variables (states.jl:73-170 set).
+ * wfh_state_size() gives the byte count. */
+int64_t wfh_state_size(const wfh_handle* h);
+int32_t wfh_save_state(wfh_handle* h, void* buf, int64_t bytes);
+int32_t wfh_load_state(wfh_handle* h, const void* buf, int64_t bytes);
+
+int32_t wfh_sync(wfh_handle* h);
+/* Kernel timing of the last step (ms per kernel phase measured with HIP events on the handle
+ * stream); only filled when timing was enabled with wfh_set_timing(h, 1). */
+int32_t wfh_set_timing(wfh_handle* h, int32_t enable);
+int32_t wfh_get_timing(wfh_handle* h, double* vertical_ms, double* routing_ms);
+const char* wfh_last_error(const wfh_handle* h);
+void wfh_destroy(wfh_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WFLOW_HIP_H */

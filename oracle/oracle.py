@@ -1,0 +1,264 @@
+"""Python binding of the CPU oracle (oracle/wflow_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, never by the product package.
+"""
+
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(ROOT, "wflow.jl_amd"))
+
+from wflowhip import _abi  # noqa: E402
+
+LIB_PATH = os.path.join(HERE, "build", "libwflow_oracle.so")
+_lib = None
+
+_c = ctypes.c_double
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+class SoilCell(ctypes.Structure):
+    """ctypes mirror of wfo_soil (oracle/wflow_oracle.h)."""
+    N = 8
+    _fields_ = (
+        [("N", ctypes.c_int32), ("nl", ctypes.c_int32), ("n_unsat", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+        + [(k, _c) for k in ("theta_s theta_r theta_fc soil_water_capacity soil_thickness kv_0 f "
+                             "air_entry_pressure h1 h2 h3_high h3_low h4 alpha_h1 "
+                             "infiltcap_soil infiltcap_compacted max_leakage compacted_fraction rootdistpar "
+                             "cap_hmax cap_n w_soil cf_soil rooting_depth soil_fraction").split()]
+        + [(k, _c * 8) for k in ("c", "kvfrac", "rootfraction", "act_thick")]
+        + [("cum_depth", _c * 9)]
+        + [(k, _c * 8) for k in ("ustore", "ustorethick")]
+        + [(k, _c) for k in ("satwaterdepth zi drainable ustorecapacity ustoredepth total_soil_water_storage "
+                             "tsoil f_infred h3 water_flux_surface pot_transpiration pot_soilevaporation "
+                             "infiltration infiltration_excess transfer soil_evaporation soil_evap_sat "
+                             "ae_ustore ae_sat transpiration actual_infiltration saturation_excess "
+                             "act_inf_soil act_inf_compacted excess_soil excess_compacted "
+                             "capflux leakage recharge aet runoff net_runoff exfiltration").split()]
+        + [(k, _c * 8) for k in ("vwc", "rvwc")]
+        + [(k, _c) for k in ("root_zone_storage", "vwc_root", "rvwc_root")]
+    )
+
+    def set(self, **kw):
+        for k, v in kw.items():
+            cur = getattr(self, k)
+            if isinstance(cur, ctypes.Array):
+                vals = list(v) + [float("nan")] * (len(cur) - len(v))
+                for i, x in enumerate(vals):
+                    cur[i] = x
+            else:
+                setattr(self, k, v)
+        return self
+
+    def arr(self, k, m=None):
+        a = np.array(getattr(self, k)[:])
+        return a if m is None else a[:m]
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.wfo_create.restype = ctypes.c_void_p
+        L.wfo_create.argtypes = [ctypes.POINTER(_abi.WfhConfig), ctypes.POINTER(_abi.WfhStatic), ctypes.c_char_p,
+                                 ctypes.c_int64]
+        L.wfo_destroy.argtypes = [ctypes.c_void_p]
+        L.wfo_step.argtypes = [ctypes.c_void_p, _D, _D, _D]
+        L.wfo_get.argtypes = [ctypes.c_void_p, ctypes.c_int32, _D, ctypes.c_int64]
+        L.wfo_set.argtypes = [ctypes.c_void_p, ctypes.c_int32, _D, ctypes.c_int64]
+        for f in ("wfo_n", "wfo_nriv", "wfo_kinwave_iterations"):
+            getattr(L, f).restype = ctypes.c_int64
+            getattr(L, f).argtypes = [ctypes.c_void_p]
+        L.wfo_nlayers.argtypes = [ctypes.c_void_p]
+        L.wfo_topology.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 5
+        L.wfo_toposort_dfs.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        dbl_fns = {
+            "wfo_scurve": 4, "wfo_vwc_brooks_corey": 5, "wfo_head_brooks_corey": 5, "wfo_feddes_h3": 3,
+            "wfo_rwu_reduction_feddes": 6, "wfo_soil_temperature": 3, "wfo_kw_ssf_newton_raphson": 5,
+            "wfo_ssf_celerity": 6,
+        }
+        for name, k in dbl_fns.items():
+            getattr(L, name).restype = _c
+            getattr(L, name).argtypes = [_c] * k
+        L.wfo_infiltration_reduction_factor.restype = _c
+        L.wfo_infiltration_reduction_factor.argtypes = [_c, _c, ctypes.c_int, ctypes.c_int]
+        L.wfo_soil_evaporation_unsaturated_store.restype = _c
+        L.wfo_soil_evaporation_unsaturated_store.argtypes = [_c, _c, _c, ctypes.c_int, _c, _c]
+        L.wfo_soil_evaporation_saturated_store.restype = _c
+        L.wfo_soil_evaporation_saturated_store.argtypes = [_c, ctypes.c_int, _c, _c, _c, _c]
+        _lib = L
+    return _lib
+
+
+def _out(k):
+    return (ctypes.c_double * k)()
+
+
+# ---- process functions (one call = one reference function call) --------------------------
+
+def kinematic_wave(q_in, q_prev, q_lat, alpha, dt, dx):
+    o = _out(2)
+    lib().wfo_kinematic_wave(_c(q_in), _c(q_prev), _c(q_lat), _c(alpha), _c(dt), _c(dx), o)
+    return o[0], o[1]
+
+
+def ssf_celerity(zi, slope, sy, kh0, f, z_exp=float("nan")):
+    return lib().wfo_ssf_celerity(zi, slope, sy, kh0, f, z_exp)
+
+
+def kw_ssf_newton_raphson(q, C, cel, dt, dx):
+    return lib().wfo_kw_ssf_newton_raphson(q, C, cel, dt, dx)
+
+
+def kinematic_wave_ssf(q_in, q_prev, zi_prev, q_net, slope, sy, d, dt, dx, dw, qmax, kh0, f, soil, z_exp=float("nan")):
+    o = _out(4)
+    lib().wfo_kinematic_wave_ssf(*(_c(x) for x in (q_in, q_prev, zi_prev, q_net, slope, sy, d, dt, dx, dw, qmax,
+                                                   kh0, f, z_exp)), ctypes.byref(soil), o)
+    return tuple(o)
+
+
+def rainfall_interception_gash(smax, er, gap, precip, storage, max_evap, dt):
+    o = _out(4)
+    lib().wfo_rainfall_interception_gash(*(_c(x) for x in (smax, er, gap, precip, storage, max_evap, dt)), o)
+    return tuple(o)
+
+
+def rainfall_interception_modrut(precip, pot_evap, storage, gap, smax, dt):
+    o = _out(4)
+    lib().wfo_rainfall_interception_modrut(*(_c(x) for x in (precip, pot_evap, storage, gap, smax, dt)), o)
+    return tuple(o)
+
+
+def precipitation_hbv(precip, temp, tti, tt):
+    o = _out(2)
+    lib().wfo_precipitation_hbv(*(_c(x) for x in (precip, temp, tti, tt)), o)
+    return tuple(o)
+
+
+def snowpack_hbv(ss, sw, sp, lp, temp, ttm, cfmax, whc, dt):
+    o = _out(5)
+    lib().wfo_snowpack_hbv(*(_c(x) for x in (ss, sw, sp, lp, temp, ttm, cfmax, whc, dt)), o)
+    return tuple(o)
+
+
+def infiltration(pot, compacted, cap_soil, cap_compacted, ustorecap, fred, dt):
+    o = _out(2)
+    lib().wfo_infiltration(*(_c(x) for x in (pot, compacted, cap_soil, cap_compacted, ustorecap, fred, dt)), o)
+    return tuple(o)
+
+
+def unsatzone_flow_layer(usd, kv_z, l_sat, c, dt):
+    o = _out(2)
+    lib().wfo_unsatzone_flow_layer(*(_c(x) for x in (usd, kv_z, l_sat, c, dt)), o)
+    return tuple(o)
+
+
+def actual_infiltration_soil_path(pot, act, compacted, cap_soil, cap_compacted, fred):
+    o = _out(2)
+    lib().wfo_actual_infiltration_soil_path(*(_c(x) for x in (pot, act, compacted, cap_soil, cap_compacted, fred)), o)
+    return tuple(o)
+
+
+def water_table_change(soil, net_flux, sy, dt):
+    o = _out(2)
+    lib().wfo_water_table_change(ctypes.byref(soil), _c(net_flux), _c(sy), _c(dt), o)
+    return tuple(o)
+
+
+def compute_mass_balance_error(tin, tout, rate):
+    o = _out(2)
+    lib().wfo_compute_mass_balance_error(_c(tin), _c(tout), _c(rate), o)
+    return tuple(o)
+
+
+def soil_call(name, soil, *args):
+    """Run one soil-model update (e.g. 'soil_transpiration') on a single SoilCell."""
+    getattr(lib(), "wfo_" + name)(ctypes.byref(soil), *(_c(a) for a in args))
+    return soil
+
+
+def toposort_dfs(n, out_lists):
+    """Graphs.topological_sort_by_dfs on out-neighbour lists (0-based ids)."""
+    ptr = np.zeros(n + 1, np.int64)
+    idx = []
+    for v in range(n):
+        lst = sorted(out_lists[v])
+        idx.extend(lst)
+        ptr[v + 1] = len(idx)
+    idx = np.array(idx if idx else [0], np.int64)
+    order = np.zeros(n, np.int64)
+    rc = lib().wfo_toposort_dfs(n, ptr.ctypes.data, idx.ctypes.data, order.ctypes.data)
+    if rc != 0:
+        raise ValueError("cycle")
+    return order
+
+
+# ---- whole model ---------------------------------------------------------------------------
+
+class OracleModel:
+    """update_model! on the CPU for a synthetic Catchment (wflowhip.synth)."""
+
+    def __init__(self, catchment, water_mass_balance=True, **cfg_kw):
+        self.c = catchment
+        self.cfg = catchment.config(water_mass_balance=water_mass_balance, **cfg_kw)
+        self._static = catchment.static()
+        err = ctypes.create_string_buffer(256)
+        self.h = lib().wfo_create(ctypes.byref(self.cfg), ctypes.byref(self._static), err, 256)
+        if not self.h:
+            raise RuntimeError(err.value.decode())
+        self.n = lib().wfo_n(self.h)
+        self.nriv = lib().wfo_nriv(self.h)
+        self.nlayers = lib().wfo_nlayers(self.h)
+
+    def close(self):
+        if self.h:
+            lib().wfo_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def step(self, precip, pet, temp):
+        P = [np.ascontiguousarray(x, dtype=np.float64) for x in (precip, pet, temp)]
+        rc = lib().wfo_step(self.h, *(x.ctypes.data_as(_D) for x in P))
+        if rc != 0:
+            raise RuntimeError(f"wfo_step failed {rc}")
+
+    def get(self, var):
+        cnt = _abi.var_count(var, self.n, self.nriv, self.nlayers)
+        out = np.empty(cnt, np.float64)
+        rc = lib().wfo_get(self.h, int(var), out.ctypes.data_as(_D), cnt)
+        if rc != 0:
+            raise KeyError(var)
+        return out
+
+    def set(self, var, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        rc = lib().wfo_set(self.h, int(var), v.ctypes.data_as(_D), v.size)
+        if rc != 0:
+            raise KeyError(var)
+
+    def topology(self):
+        lo = np.empty(self.n, np.int64)
+        ld = np.empty(self.n, np.int64)
+        ro = np.empty(max(self.nriv, 1), np.int64)
+        rd = np.empty(max(self.nriv, 1), np.int64)
+        rl = np.empty(max(self.nriv, 1), np.int64)
+        lib().wfo_topology(self.h, lo.ctypes.data, ld.ctypes.data, ro.ctypes.data, rd.ctypes.data, rl.ctypes.data)
+        return lo, ld, ro[: self.nriv], rd[: self.nriv], rl[: self.nriv]
+
+    def kinwave_iterations(self):
+        return lib().wfo_kinwave_iterations(self.h)

@@ -1,0 +1,66 @@
+/*
+ * wflow_oracle.h - CPU FP64 oracle of the Wflow.jl `sbm` time step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This restatement of the reference algorithm is the checker for
+ * the HIP implementation: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it. The product never links or calls it.
+ */
+#ifndef WFLOW_ORACLE_H
+#define WFLOW_ORACLE_H
+
+#include <stdint.h>
+
+#include "wflow_hip.h"
+
+#define WFO_NMAX WFH_MAX_LAYERS
+
+/* One SBM soil column: the fields of SbmSoilParameters / SbmSoilVariables / SbmSoilBC
+ * (Wflow/src/soil/soil.jl:4-224) for a single cell; N-vectors hold the SVector{N} entries. */
+typedef struct wfo_soil {
+    int32_t N;       /* maximum_number_of_layers */
+    int32_t nl;      /* number_of_layers         */
+    int32_t n_unsat; /* n_unsatlayers            */
+    int32_t pad_;
+    /* parameters */
+    double theta_s, theta_r, theta_fc, soil_water_capacity, soil_thickness, kv_0, f;
+    double air_entry_pressure, h1, h2, h3_high, h3_low, h4, alpha_h1;
+    double infiltcap_soil, infiltcap_compacted, max_leakage, compacted_fraction, rootdistpar;
+    double cap_hmax, cap_n, w_soil, cf_soil, rooting_depth, soil_fraction;
+    double c[WFO_NMAX], kvfrac[WFO_NMAX], rootfraction[WFO_NMAX], act_thick[WFO_NMAX];
+    double cum_depth[WFO_NMAX + 1];
+    /* variables */
+    double ustore[WFO_NMAX], ustorethick[WFO_NMAX];
+    double satwaterdepth, zi, drainable, ustorecapacity, ustoredepth, total_soil_water_storage;
+    double tsoil, f_infred, h3;
+    /* boundary conditions */
+    double water_flux_surface, pot_transpiration, pot_soilevaporation;
+    /* fluxes */
+    double infiltration, infiltration_excess, transfer, soil_evaporation, soil_evap_sat;
+    double ae_ustore, ae_sat, transpiration, actual_infiltration, saturation_excess;
+    double act_inf_soil, act_inf_compacted, excess_soil, excess_compacted;
+    double capflux, leakage, recharge, aet, runoff, net_runoff, exfiltration;
+    double vwc[WFO_NMAX], rvwc[WFO_NMAX], root_zone_storage, vwc_root, rvwc_root;
+} wfo_soil;
+
+typedef struct wfo_model wfo_model;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+wfo_model* wfo_create(const wfh_config* cfg, const wfh_static* st, char* err, int64_t errlen);
+void wfo_destroy(wfo_model* m);
+int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const double* temp);
+int32_t wfo_get(const wfo_model* m, int32_t var, double* dst, int64_t count);
+int32_t wfo_set(wfo_model* m, int32_t var, const double* src, int64_t count);
+int64_t wfo_n(const wfo_model* m);
+int64_t wfo_nriv(const wfo_model* m);
+int32_t wfo_nlayers(const wfo_model* m);
+int32_t wfo_topology(const wfo_model* m, int64_t* land_order, int64_t* land_down, int64_t* river_order,
+                     int64_t* river_down, int64_t* river_land_index);
+int64_t wfo_kinwave_iterations(const wfo_model* m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

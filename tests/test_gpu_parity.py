@@ -1,0 +1,199 @@
+"""HIP path (through the C ABI) against the CPU oracle on the same seeded inputs.
+
+Bar: every state and flux within 1e-10 relative per cell per step (in practice bit-identical,
+both sides run the same IEEE-754 operation sequence and the same elementary functions);
+topology bit-exact; water-balance closure identical to the oracle's.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from wflowhip import Model, Var, synth
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+STATE_VARS = [Var.CANOPY_STORAGE, Var.SNOW_STORAGE, Var.SNOW_WATER, Var.SOIL_TEMPERATURE, Var.SATWATERDEPTH,
+              Var.USTORELAYERDEPTH, Var.SSF_Q, Var.LAND_Q, Var.LAND_H, Var.RIVER_Q, Var.RIVER_H]
+FLUX_VARS = [Var.WATER_TABLE_DEPTH, Var.USTOREDEPTH, Var.TRANSFER, Var.RECHARGE, Var.ACTUAL_INFILTRATION,
+             Var.INFILTRATION_EXCESS, Var.SATURATION_EXCESS, Var.TRANSPIRATION, Var.SOIL_EVAPORATION,
+             Var.ACTUAL_CAPILLARY_FLUX, Var.ACTUAL_LEAKAGE, Var.ACTUAL_EVAPOTRANSPIRATION, Var.INTERCEPTION_RATE,
+             Var.NET_RUNOFF, Var.RUNOFF, Var.NET_RUNOFF_RIVER, Var.EXFILTRATION, Var.TOTAL_STORAGE,
+             Var.VWC, Var.VWC_ROOT_ZONE, Var.ROOT_ZONE_STORAGE, Var.SSF_QIN_AVERAGE, Var.SSF_Q_AVERAGE,
+             Var.SSF_TO_RIVER_AVERAGE, Var.SSF_STORAGE, Var.LAND_QIN, Var.LAND_STORAGE, Var.LAND_Q_AVERAGE,
+             Var.LAND_QIN_AVERAGE, Var.LAND_TO_RIVER_AVERAGE, Var.LAND_INWATER, Var.RIVER_QIN, Var.RIVER_STORAGE,
+             Var.RIVER_Q_AVERAGE, Var.RIVER_QIN_AVERAGE, Var.RIVER_INWATER, Var.BALANCE_LAND_ERROR,
+             Var.BALANCE_SSF_ERROR, Var.BALANCE_OVERLAND_ERROR, Var.BALANCE_RIVER_ERROR]
+PARAM_VARS = [Var.LAND_ALPHA, Var.RIVER_ALPHA, Var.THETA_FC, Var.SURFACE_FLOW_WIDTH, Var.FLOW_FRACTION_TO_RIVER,
+              Var.SSF_QMAX]
+
+
+def rel_err(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    same = (a == b) | (np.isnan(a) & np.isnan(b))
+    d = np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+    d[same] = 0.0
+    return float(d.max()) if d.size else 0.0
+
+
+def compare(gpu, ref, vars_, where=""):
+    worst = {}
+    for v in vars_:
+        e = rel_err(gpu.get(v), ref.get(v))
+        worst[v.name] = e
+    bad = {k: v for k, v in worst.items() if not v <= RTOL}
+    assert not bad, f"{where}: {bad}"
+    return max(worst.values())
+
+
+def run_pair(c, steps, every=1, forcing=synth.forcing_numpy, **kw):
+    gpu = Model(c, **kw)
+    ref = O.OracleModel(c, **kw)
+    compare(gpu, ref, PARAM_VARS + STATE_VARS, "init")
+    for k in range(steps):
+        P, E, T = forcing(c, k)
+        gpu.run_timestep(P, E, T)
+        ref.step(P, E, T)
+        if (k + 1) % every == 0 or k == steps - 1:
+            compare(gpu, ref, STATE_VARS + FLUX_VARS, f"step {k}")
+    return gpu, ref
+
+
+def test_small_daily_every_step():
+    c = synth.make_catchment(64, cell=1000.0, dt=86400.0, river_threshold=100, seed=11)
+    gpu, ref = run_pair(c, 12)
+    gpu.close()
+    ref.close()
+
+
+def test_small_hourly_every_step():
+    c = synth.make_catchment(96, cell=250.0, dt=3600.0, river_threshold=200, seed=12)
+    gpu, ref = run_pair(c, 30)
+    gpu.close()
+    ref.close()
+
+
+def test_no_snow_no_balance():
+    c = synth.make_catchment(64, cell=1000.0, dt=86400.0, river_threshold=100, seed=13, snow=False,
+                             infiltration_reduction=False)
+    gpu = Model(c, water_mass_balance=False)
+    ref = O.OracleModel(c, water_mass_balance=False)
+    for k in range(6):
+        P, E, T = synth.forcing_numpy(c, k)
+        gpu.run_timestep(P, E, T)
+        ref.step(P, E, T)
+    compare(gpu, ref, STATE_VARS + [Var.TOTAL_STORAGE, Var.RIVER_Q_AVERAGE, Var.SSF_Q_AVERAGE], "no-snow")
+
+
+def test_substep_remainder():
+    """dt not a multiple of the kinematic-wave step: last sub-step shortened (timestepping.jl:11-17)."""
+    c = synth.make_catchment(48, cell=500.0, dt=7200.0, river_threshold=60, seed=14)
+    gpu = Model(c, land_dt=5000.0, river_dt=1700.0, subsurface_dt=3000.0)
+    ref = O.OracleModel(c, land_dt=5000.0, river_dt=1700.0, subsurface_dt=3000.0)
+    for k in range(8):
+        P, E, T = synth.forcing_numpy(c, k)
+        gpu.run_timestep(P, E, T)
+        ref.step(P, E, T)
+        compare(gpu, ref, STATE_VARS + FLUX_VARS, f"step {k}")
+
+
+def test_topology_bit_exact():
+    c = synth.make_catchment(128, cell=1000.0, dt=86400.0, river_threshold=100, seed=15)
+    gpu = Model(c)
+    ref = O.OracleModel(c)
+    lo, ld, ro, rd, rl, maxlev = gpu.topology()
+    rlo, rld, rro, rrd, rrl = ref.topology()
+    assert np.array_equal(lo, rlo) and np.array_equal(ld, rld)
+    assert np.array_equal(ro, rro) and np.array_equal(rd, rrd) and np.array_equal(rl, rrl)
+    assert maxlev == c.max_depth
+    # node order is a level order: every upstream cell sits before its downstream cell
+    order = gpu.node_order()
+    pos = np.empty_like(order)
+    pos[order] = np.arange(order.size)
+    has = ld >= 0
+    assert np.all(pos[np.nonzero(has)[0]] < pos[ld[has]])
+
+
+def test_c0_config_365_daily_steps():
+    """256x256, dx 1 km, daily, 365 steps; checked every 5 steps and at the end."""
+    c = synth.make_catchment(256, cell=1000.0, dt=86400.0, river_threshold=100, seed=0)
+    gpu, ref = run_pair(c, 365, every=5)
+    # identical water-balance closure, and the reference's closure bound where it applies
+    for v in (Var.BALANCE_LAND_ERROR, Var.BALANCE_OVERLAND_ERROR, Var.BALANCE_RIVER_ERROR, Var.BALANCE_SSF_ERROR):
+        assert np.array_equal(gpu.get(v), ref.get(v)), v
+    assert np.abs(gpu.get(Var.BALANCE_LAND_ERROR)).max() < 1e-9
+    gpu.close()
+    ref.close()
+
+
+def test_restart_equals_continuous():
+    """save_state after k steps, reload into a fresh model: identical to the continuous run
+    (Wflow/test/run.jl:70-72)."""
+    c = synth.make_catchment(64, cell=1000.0, dt=86400.0, river_threshold=100, seed=16)
+    a = Model(c)
+    for k in range(5):
+        a.run_timestep(*synth.forcing_numpy(c, k))
+    state = a.save_state()
+    for k in range(5, 9):
+        a.run_timestep(*synth.forcing_numpy(c, k))
+    b = Model(c)
+    b.load_state(state)
+    for k in range(5, 9):
+        b.run_timestep(*synth.forcing_numpy(c, k))
+    for v in STATE_VARS + [Var.RIVER_Q_AVERAGE, Var.TOTAL_STORAGE]:
+        assert np.array_equal(a.get(v), b.get(v)), v
+
+
+def test_repeatable_bitwise():
+    """The dataflow schedule is timing dependent; results must not be."""
+    c = synth.make_catchment(128, cell=250.0, dt=3600.0, river_threshold=150, seed=17)
+    outs = []
+    for _ in range(2):
+        m = Model(c)
+        for k in range(6):
+            m.run_timestep(*synth.forcing_numpy(c, k))
+        outs.append([m.get(v) for v in STATE_VARS])
+        m.close()
+    for x, y in zip(*outs):
+        assert np.array_equal(x, y)
+
+
+def test_forcing_memkinds_agree():
+    import torch
+    from wflowhip import _abi
+    c = synth.make_catchment(64, cell=1000.0, dt=86400.0, river_threshold=100, seed=18)
+    ms = [Model(c) for _ in range(3)]
+    order = torch.as_tensor(ms[2].node_order(), device="cuda")
+    for k in range(4):
+        P, E, T = synth.forcing_numpy(c, k)
+        ms[0].run_timestep(P, E, T)
+        dev = [torch.as_tensor(x, device="cuda") for x in (P, E, T)]
+        ms[1].set_forcing(*dev)
+        ms[1].update_model()
+        node = [x[order].contiguous() for x in dev]
+        ms[2].set_forcing(*node, memkind=_abi.WFH_MEM_DEVICE_NODE_ORDER)
+        ms[2].update_model()
+        torch.cuda.synchronize()
+    for v in STATE_VARS:
+        a = ms[0].get(v)
+        assert np.array_equal(a, ms[1].get(v)) and np.array_equal(a, ms[2].get(v)), v
+
+
+def test_step_without_forcing_is_an_error():
+    from wflowhip import _abi
+    from wflowhip.model import WflowError
+    c = synth.make_catchment(32, cell=1000.0, dt=86400.0, river_threshold=40, seed=19)
+    m = Model(c)
+    with pytest.raises(WflowError) as e:
+        m.update_model()
+    assert e.value.code == _abi.WFH_ERR_STATE
+
+
+def test_c1_hourly_1024_first_24_steps():
+    """configs[1] generator (250 m, hourly, river threshold 2000 cells) at 1024^2, 24 steps."""
+    c = synth.make_catchment(1024, cell=250.0, dt=3600.0, river_threshold=2000, seed=1)
+    gpu, ref = run_pair(c, 24, every=6)
+    gpu.close()
+    ref.close()

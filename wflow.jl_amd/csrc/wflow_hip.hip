@@ -1,0 +1,1810 @@
+// wflow_hip.hip - MI355X (gfx950) implementation of the Wflow.jl `sbm` time step behind the C ABI
+// declared in include/wflow_hip.h.
+//
+// Device layout
+//   * Land cells are renumbered in topological-level order (level = longest upstream path in
+//     cells, headwaters first; ascending land index inside a level). Every per-cell array is a
+//     structure-of-arrays FP64 buffer in that order, so both kernels read and write coalesced.
+//   * Layered soil variables are layer-major: a[k * n + p].
+//   * Parameter maps whose values are all equal are stored once (mask 0) and broadcast.
+//   * Upstream lists are int32 CSR in node order; inside a list the upstream cells keep
+//     ascending land index, the order in which the reference folds `sum_at`
+//     (Wflow/src/utils.jl:472-477).
+//
+// Kernels per step
+//   K1 k_vertical  one thread per cell: interception, snow, open-water runoff, SBM soil column
+//                  (update_land_hydrology_model!, Wflow/src/sbm.jl:82-132).
+//   K2 k_route     persistent dataflow kernel over all routing of the step: lateral subsurface
+//                  flow (kinwave_subsurface_update!), update_soil_water_storage!, overland and
+//                  river kinematic waves for every fixed sub-step, total storage and the water
+//                  balance. One lane owns one cell for the whole step. A lane waits only on the
+//                  upstream cells' outflow of the same sub-step, published into single-consumer
+//                  slots: a slot holds a sentinel NaN until the upstream lane stores its q
+//                  (relaxed agent-scope atomic store); the downstream lane polls it, folds it into
+//                  its inflow sum and resets it to the sentinel. Wave tickets are handed out in
+//                  level order, so a lane only ever waits on lanes that already hold a ticket; the
+//                  kernel therefore drains without any grid-wide barrier. Waits are bounded: a
+//                  lane that spins past the limit raises an abort flag that every wave polls, and
+//                  the host reports WFH_ERR_DEVICE.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "wflow_hip.h"
+#include "wfh_math.h"
+#include "wfh_physics.h"
+
+using namespace wfh;
+
+namespace {
+
+constexpr uint64_t SLOT_EMPTY = 0xFFF40000DEADBEEFULL;  // signalling NaN, never produced by arithmetic
+constexpr int WAVE = 64;
+constexpr int ROUTE_BLOCK = 256;
+constexpr long SPIN_LIMIT = 20000000;  // polls (each with a sleep) before a wait is declared stuck
+
+struct Prm {  // parameter map, broadcast when uniform
+    const double* p;
+    uint32_t mask;
+};
+struct PrmL {  // layered parameter map [N][n]
+    const double* p;
+    uint32_t mask;
+    uint32_t kstride;
+};
+__device__ __forceinline__ double ld(const Prm& a, int i) { return a.p[i & a.mask]; }
+__device__ __forceinline__ double ld(const PrmL& a, int k, int i) { return a.p[k * a.kstride + (i & a.mask)]; }
+
+struct Dev {
+    int n, nriv;
+    // forcing (node order)
+    const double *P, *E, *T;
+    // ---- static maps (node order) ----
+    Prm gap, smax, kc, er, cfmax, tt, tti, ttm, whc, rf, wf, sfrac;
+    Prm ts, tr, tfc, kv0, f, hb, h1, h2, h3h, h3l, h4, ah1, d, icap_s, icap_c, maxleak, compf, rdp, caph, capn, wsoil,
+        cfsoil, rd;
+    PrmL c, kvfrac, rootfrac;
+    Prm area, fl, fw, slope, sy, kh0, qmax, alpha, sfw, elev;
+    // topology
+    const int* up_ptr;   // [n+1]
+    const int* up_idx;   // [E] node ids of upstream cells
+    const double* up_f;  // [E] flow_fraction_to_river of the upstream cell
+    const int* river_of; // [n] river id or -1
+    const uint8_t* flags;  // bit0: has a land downstream, bit1: river cell, bit2: has a river downstream
+    const int* rup_ptr;  // [nriv+1]
+    const int* rup_idx;  // [Er] river ids
+    // river static (river order)
+    const double *r_alpha, *r_fw, *r_fl, *r_area;
+    // ---- land states ----
+    double *canopy, *snow_s, *snow_w, *tsoil, *swd, *ustore, *tsws;
+    double *ssf_q, *ol_q, *ol_h, *ol_storage;
+    double *r_q, *r_h, *r_storage;
+    // ---- K1 outputs ----
+    double *zi, *interception, *swe, *transfer, *recharge, *act_inf, *inf_excess, *sat_excess, *transpiration,
+        *soil_evap, *capflux, *leakage, *aet, *net_runoff_river, *runoff_land, *aeow_l, *store_prev;
+    // ---- K2 outputs ----
+    double *ustoredepth, *runoff, *net_runoff, *exfilt, *vwc, *vwc_root, *rzs;
+    double *ssf_qin_avg, *ssf_q_avg, *ssf_tor_avg, *ssf_storage;
+    double *ol_qin, *ol_q_avg, *ol_qin_avg, *ol_tor_avg, *ol_inwater;
+    double *r_qin, *r_q_avg, *r_qin_avg, *r_inwater;
+    double *total_storage, *bal_land, *bal_land_rel, *bal_ssf, *bal_ol, *bal_r;
+    // ---- scalars ----
+    double cfg_thick[WFH_MAX_LAYERS], cfg_cum[WFH_MAX_LAYERS + 1];
+    double dt;
+    int gash, snow, infred, balance;
+};
+
+struct RouteArgs {
+    uint64_t* slot_ssf;  // [S_s][n]
+    uint64_t* slot_ol;   // [S_l][n]
+    uint64_t* slot_r;    // [S_r][nriv]
+    unsigned long long* counters;  // [2] wave tickets, alternating per launch
+    int* abort_flag;
+    int parity;
+    int nchunks;
+    int S_s, S_l, S_r;
+    double dts_s[2], dts_l[2], dts_r[2];  // sub-step lengths: first (S-1) steps and the last one
+};
+
+// ============================================================================================
+// K1: vertical column update (update_land_hydrology_model!, Wflow/src/sbm.jl:82-132)
+// ============================================================================================
+template <int N>
+__global__ void __launch_bounds__(256) k_vertical(Dev d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.n) return;
+    const int n = d.n;
+    const double dt = d.dt;
+    const double P = d.P[i], E = d.E[i], T = d.T[i];
+
+    // storage_prev! (Wflow/src/mass_balance.jl:96-115,170-183) from last step's states
+    double canopy = d.canopy[i];
+    double ss = d.snow_s[i], sw = d.snow_w[i];
+    if (d.balance) {
+        const double snow = d.snow ? ss + sw : 0.0 + 0.0;
+        d.store_prev[i] = d.tsws[i] + canopy + snow + 0.0 * 0.0 + 0.0;
+    }
+
+    // ---- interception: canopy.jl:54-98 (Gash, dt >= 23 h) / :115-143 (modified Rutter) ----
+    const double gap = ld(d.gap, i), smax = ld(d.smax, i);
+    const double cpe = ld(d.kc, i) * E * (1.0 - gap);
+    double tf, ir, sf;
+    if (d.gash) {
+        interception_gash(smax, ld(d.er, i), gap, P, cpe, dt, tf, ir, sf);
+    } else {
+        interception_modrut(P, cpe, gap, smax, dt, canopy, tf, ir, sf);
+    }
+
+    // ---- snow: snow.jl:123-177 ----
+    double wfs;
+    if (d.snow) {
+        double swe, sro;
+        snow_hbv(tf + sf, T, ld(d.tti, i), ld(d.tt, i), ld(d.ttm, i), ld(d.cfmax, i), ld(d.whc, i), dt, ss, sw, swe,
+                 sro);
+        d.swe[i] = swe;
+        wfs = sro + 0.0 * 0.0;  // get_water_flux_surface! with snow (runoff.jl:49-58), no glacier
+    } else {
+        wfs = tf + sf;  // runoff.jl:37-46
+    }
+
+    // ---- open water runoff: runoff.jl:61-111 ----
+    const int r = d.river_of[i];
+    const double wd_land = d.ol_h[i];
+    const double wd_river = r >= 0 ? d.r_h[r] : 0.0;
+    const double rf = ld(d.rf, i), wf = ld(d.wf, i);
+    const double rr = wfh_min(1.0, rf) * wfs;
+    const double rl = wfh_min(1.0, wf) * wfs;
+    const double aor = rf * wfh_min(wd_river / dt, E);
+    const double aol = wf * wfh_min(wd_land / dt, E);
+    const double nrr = rr - aor;
+
+    // ---- soil boundary conditions: soil.jl:643-682 ----
+    const double pot_trans = wfh_max(0.0, cpe - ir);
+    double pot_sevap = ld(d.sfrac, i) * E;
+    pot_sevap = pot_sevap - 0.0;
+    const double wfs_soil = wfh_max(((wfs + 0.0) - rr) - rl + 0.0 / dt, 0.0);
+
+    // ---- update_soil_water_flow!  soil.jl:1150-1211 ----
+    const double ts = ld(d.ts, i), tr = ld(d.tr, i), tfc = ld(d.tfc, i), dsoil = ld(d.d, i);
+    const double kv0 = ld(d.kv0, i), fk = ld(d.f, i);
+    double act[N], cum[N + 1];
+    const int nl = soil_layers<N>(dsoil, d.cfg_cum, d.cfg_thick, act, cum);
+    double us[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) us[k] = d.ustore[k * n + i];
+    const double swd = d.swd[i];
+    const double swc = dsoil * (ts - tr);
+    // update_diagnostic_vars!  soil.jl:1400-1436
+    double usd = 0.0;
+    {
+        bool first = true;
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            if (k < nl) {
+                usd = first ? us[k] : usd + us[k];
+                first = false;
+            }
+    }
+    const double zi = wfh_max(0.0, dsoil - swd / (ts - tr));
+    const double thd = drainable_porosity(ts, tfc);
+    double drainable = (dsoil - zi) * thd;
+    double ucap = swc - swd - usd;
+    double th[N];
+    const int nu = set_layerthickness<N>(zi, cum, act, th);
+    // soil_temperature!, infiltration_reduction_factor!  soil.jl:685-730
+    double tsoil = d.tsoil[i];
+    if (d.snow) tsoil = tsoil + ld(d.wsoil, i) * (T - tsoil);
+    double fred = 1.0;
+    if (d.snow && d.infred) {
+        const double cf = ld(d.cfsoil, i);
+        const double bb = 1.0 / (1.0 - cf);
+        fred = scurve(tsoil, 0.0 + 273.15, bb, 8.0) + cf;
+    }
+    // infiltration  soil_process.jl:16-41
+    const double compf = ld(d.compf, i), icap_s = ld(d.icap_s, i), icap_c = ld(d.icap_c, i);
+    double infilt, inf_exc;
+    {
+        const double soilinf = wfs_soil * (1.0 - compf);
+        const double pathinf = wfs_soil * compf;
+        const double ms = wfh_min(icap_s * fred, soilinf);
+        const double mp = wfh_min(icap_c * fred, pathinf);
+        infilt = wfh_min(mp + ms, wfh_max(0.0, ucap / dt));
+        inf_exc = (soilinf - ms) + (pathinf - mp);
+    }
+    // unsaturated_zone_flow!  soil.jl:764-804
+    double transfer = 0.0;
+    if (nu > 0) {
+        double z = 0.0, flow = 0.0;
+#pragma unroll
+        for (int m = 0; m < N; ++m) {
+            if (m >= nu) continue;
+            z = (m == 0) ? th[0] : z + th[m];
+            const double l_sat = th[m] * (ts - tr);
+            const double kvz = ld(d.kvfrac, m, i) * kv0 * wfh_exp(-fk * z);
+            double u = (m == 0) ? us[m] + infilt * dt : us[m] + flow * dt;
+            unsatzone_flow_layer(u, kvz, l_sat, ld(d.c, m, i), dt, flow);
+            us[m] = u;
+        }
+        transfer = flow;
+    }
+    // soil_evaporation!  soil.jl:814-856
+    double sevap, ses;
+    {
+        double pot = pot_sevap;
+        double seu;
+        if (nu == 0)
+            seu = 0.0;
+        else if (nu == 1)
+            seu = pot * wfh_min(1.0, us[0] / (zi * (ts - tr)));
+        else
+            seu = pot * wfh_min(1.0, us[0] / (th[0] * (ts - tr)));
+        seu = wfh_min(seu, us[0] / dt);
+        pot -= seu;
+        us[0] = us[0] - seu * dt;
+        if (nu == 0 || nu == 1) {
+            const double e = pot * wfh_min(1.0, (act[0] - zi) / act[0]);
+            ses = wfh_min(e, (act[0] - zi) * thd / dt);
+        } else {
+            ses = 0.0;
+        }
+        sevap = seu + ses;
+        drainable -= ses * dt;
+    }
+    // transpiration!  soil.jl:865-975
+    const double rd = ld(d.rd, i);
+    double ae_u = 0.0, ae_s, transp;
+    {
+        const double h3 = feddes_h3(ld(d.h3h, i), ld(d.h3l, i), pot_trans);
+        const double h1 = ld(d.h1, i), h2 = ld(d.h2, i), h4 = ld(d.h4, i), ah1 = ld(d.ah1, i);
+        const double hb = ld(d.hb, i);
+        double sum_rf = 0.0, rf_low = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            if (k >= nu) continue;
+            double rfu;
+            if (k == nu - 1 && zi < rd) {
+                const double rootlength = wfh_min(act[k], rd - cum[k]);
+                rfu = ld(d.rootfrac, k, i) * (th[k] / rootlength);
+            } else {
+                rfu = ld(d.rootfrac, k, i);
+            }
+            sum_rf += rfu;
+            rf_low = rfu;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            if (k >= nu) continue;
+            const double rfu = (k < nu - 1) ? ld(d.rootfrac, k, i) : rf_low;
+            const double rfs = rd > 0.0 ? wfh_max((1.0 / sum_rf), 1.0) * rfu : 0.0;
+            const double vwc = wfh_max(us[k] / th[k], 1e-7);
+            const double head = head_brooks_corey(vwc, ts, tr, ld(d.c, k, i), hb);
+            const double alpha = rwu_reduction_feddes(head, h1, h2, h3, h4, ah1);
+            const double availcap = wfh_min(1.0, wfh_max(0.0, (rd - cum[k]) / th[k]));
+            const double maxextr = us[k] * availcap / dt;
+            const double layer = wfh_min(alpha * rfs * pot_trans, maxextr);
+            us[k] = us[k] - layer * dt;
+            ae_u += layer;
+        }
+        const double wetroots = scurve(zi, rd, 1.0, ld(d.rdp, i));
+        const double alpha0 = rwu_reduction_feddes(0.0, h1, h2, h3, h4, ah1);
+        const double rest = pot_trans - ae_u;
+        ae_s = wfh_min(rest * wetroots * alpha0, drainable / dt);
+        drainable -= ae_s * dt;
+        transp = ae_u + ae_s;
+    }
+    // actual_infiltration!  soil.jl:987-1017
+    double act_inf;
+    {
+        double excess = 0.0;
+#pragma unroll
+        for (int k = N - 1; k >= 0; --k) {
+            if (k >= nu) continue;
+            excess = wfh_max(0.0, us[k] - th[k] * (ts - tr));
+            us[k] = us[k] - excess;
+            if (k > 0) us[k - 1] = us[k - 1] + excess;
+        }
+        act_inf = infilt - excess / dt;
+    }
+    const double sat_exc = (wfs_soil - act_inf) - inf_exc;
+    // unsaturated_store_depth! + capacity  soil.jl:1194-1196
+    {
+        bool first = true;
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            if (k < nl) {
+                usd = first ? us[k] : usd + us[k];
+                first = false;
+            }
+        if (nl <= 0) usd = 0.0;
+    }
+    ucap = swc - swd - usd;
+    // capillary_flux!  soil.jl:1050-1111
+    double capflux = 0.0;
+    if (nu > 0) {
+        const double ksat = ld(d.kvfrac, nu - 1, i) * kv0 * wfh_exp(-fk * zi);
+        const double mx = wfh_max(0.0, wfh_min(wfh_min(wfh_min(ksat, ae_u), ucap / dt), drainable / dt));
+        double cap;
+        const double caph = ld(d.caph, i);
+        if (zi > rd)
+            cap = mx * wfh_pow(1.0 - wfh_min(zi, caph) / (caph), ld(d.capn, i));
+        else
+            cap = 0.0;
+        double net = cap;
+#pragma unroll
+        for (int k = N - 1; k >= 0; --k) {
+            if (k >= nu) continue;
+            const double toadd = wfh_min(net, wfh_max((th[k] * (ts - tr) - us[k]) / dt, 0.0));
+            us[k] = us[k] + toadd * dt;
+            net -= toadd;
+            capflux += toadd;
+        }
+    }
+    // leakage!  soil.jl:1118-1136
+    double leak;
+    {
+        const double deepksat = ld(d.kvfrac, nl - 1, i) * kv0 * wfh_exp(-fk * dsoil);
+        const double deeptransfer = wfh_min(drainable / dt, deepksat);
+        leak = wfh_max(0.0, wfh_min(ld(d.maxleak, i), deeptransfer));
+    }
+    const double recharge = (transfer - capflux - leak - ae_s - ses);
+    double aet = sevap + transp + aor + aol + 0.0;
+    aet = aet + ir;  // sbm.jl:130
+
+    // ---- write back ----
+    d.canopy[i] = canopy;
+    if (d.snow) {
+        d.snow_s[i] = ss;
+        d.snow_w[i] = sw;
+    }
+    d.tsoil[i] = tsoil;
+#pragma unroll
+    for (int k = 0; k < N; ++k) d.ustore[k * n + i] = us[k];
+    d.zi[i] = zi;
+    d.interception[i] = ir;
+    d.transfer[i] = transfer;
+    d.recharge[i] = recharge;
+    d.act_inf[i] = act_inf;
+    d.inf_excess[i] = inf_exc;
+    d.sat_excess[i] = sat_exc;
+    d.transpiration[i] = transp;
+    d.soil_evap[i] = sevap;
+    d.capflux[i] = capflux;
+    d.leakage[i] = leak;
+    d.aet[i] = aet;
+    d.net_runoff_river[i] = nrr;
+    d.runoff_land[i] = rl;
+    d.aeow_l[i] = aol;
+}
+
+// ============================================================================================
+// K2: routing dataflow kernel
+// ============================================================================================
+__device__ __forceinline__ void slot_put(uint64_t* s, double v) {
+    __hip_atomic_store(s, wfh_bits(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool slot_take(uint64_t* s, double& v) {
+    const uint64_t b = __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (b == SLOT_EMPTY) return false;
+    __hip_atomic_store(s, SLOT_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v = wfh_from_bits(b);
+    return true;
+}
+
+enum Phase : int { PH_SSF = 0, PH_STORE = 1, PH_OL = 2, PH_RIVER = 3, PH_FINAL = 4, PH_DONE = 5 };
+
+template <int N>
+struct Lane {
+    int p, r, phase, sub, e, e_end;
+    double acc1, acc2;  // running upstream folds (kept in the reference's left-fold order)
+    double qnb;         // q_net_bnds of the current subsurface sub-step
+    // subsurface
+    double sq, szi, s_qin_cum, s_q_cum, s_exf_cum, s_qnet_cum, s_tor_cum, rech_cum;
+    Column<N> col;
+    // overland
+    double oq, oh, oqlat, o_q_cum, o_qin_cum, o_tor_cum, oqin, o_inwater, o_store_prev;
+    // river
+    double rq, rh, rqlat, r_q_cum, r_qin_cum, rqin, r_inwater, r_store_prev, r_storage;
+    double o_storage;
+};
+
+template <int N>
+__device__ void lane_init(const Dev& d, Lane<N>& L) {
+    const int p = L.p, n = d.n;
+    L.r = d.river_of[p];
+    L.phase = PH_SSF;
+    L.sub = 0;
+    L.e = d.up_ptr[p];
+    L.e_end = d.up_ptr[p + 1];
+    L.acc1 = 0.0;
+    L.acc2 = 0.0;
+    const double dsoil = ld(d.d, p);
+    const double ts = ld(d.ts, p), tr = ld(d.tr, p), tfc = ld(d.tfc, p);
+    soil_layers<N>(dsoil, d.cfg_cum, d.cfg_thick, L.col.act, L.col.cum);
+#pragma unroll
+    for (int k = 0; k < N; ++k) L.col.ustore[k] = d.ustore[k * n + p];
+    L.col.zi = d.zi[p];
+    L.col.n_unsat = set_layerthickness<N>(L.col.zi, L.col.cum, L.col.act, L.col.thick);
+    L.col.theta_e = ts - tr;
+    L.col.theta_fc_r = tfc - tr;
+    L.sq = d.ssf_q[p];
+    L.szi = L.col.zi;  // routing.subsurface_flow.variables.water_table_depth .= soil's (sbm_model.jl:80)
+    L.s_qin_cum = L.s_q_cum = L.s_exf_cum = L.s_qnet_cum = L.s_tor_cum = L.rech_cum = 0.0;
+    L.oq = d.ol_q[p];
+    L.oh = d.ol_h[p];
+    L.o_store_prev = d.ol_storage[p];
+    if (L.r >= 0) {
+        L.rq = d.r_q[L.r];
+        L.rh = d.r_h[L.r];
+        L.r_store_prev = d.r_storage[L.r];
+    }
+}
+
+// recharge boundary flux for one subsurface sub-step: flux!(::RechargeModel)
+// Wflow/src/routing/subsurface/boundary_conditions.jl:219-236, check_flux :12-21
+template <int N>
+__device__ __forceinline__ void ssf_begin(const Dev& d, Lane<N>& L, double dts) {
+    double fl = d.recharge[L.p] * ld(d.area, L.p);
+    if (L.szi >= ld(d.d, L.p)) fl = wfh_max(0.0, fl);
+    L.rech_cum += fl * dts;
+    L.qnb = 0.0 + fl;
+}
+
+// Try to advance one lane through as many phases as its inputs allow. Returns true on progress.
+template <int N>
+__device__ bool lane_advance(const Dev& d, const RouteArgs& a, Lane<N>& L) {
+    const int p = L.p, n = d.n;
+    const double dt = d.dt;
+    bool progressed = false;
+    for (;;) {
+        if (L.phase == PH_SSF) {
+            // ---- kinwave_subsurface_update!  lateral_subsurface_flow.jl:198-273 ----
+            const int k = L.sub;
+            uint64_t* slots = a.slot_ssf + (size_t)k * n;
+            while (L.e < L.e_end) {
+                double qu;
+                if (!slot_take(slots + d.up_idx[L.e], qu)) return progressed;
+                const double fu = d.up_f[L.e];
+                L.acc1 = L.acc1 + qu * (1.0 - fu);
+                L.acc2 = L.acc2 + qu * fu;
+                ++L.e;
+            }
+            const double dts = (k == a.S_s - 1) ? a.dts_s[1] : a.dts_s[0];
+            const double qin = L.acc1;
+            L.s_tor_cum += L.acc2 * dts;
+            double qn, zin, exf, net;
+            kinematic_wave_ssf<N>(qin, L.sq, L.szi, L.qnb, ld(d.slope, p), ld(d.sy, p), ld(d.d, p), dts, ld(d.fl, p),
+                                  ld(d.fw, p), ld(d.qmax, p), ld(d.kh0, p), ld(d.f, p), L.col, qn, zin, exf, net);
+            L.sq = qn;
+            L.szi = zin;
+            L.s_qin_cum += qin * dts;
+            L.s_q_cum += qn * dts;
+            L.s_exf_cum += exf * dts;
+            L.s_qnet_cum += net * ld(d.area, p) * dts;
+            if (d.flags[p] & 1) slot_put(slots + p, qn);
+            L.e = d.up_ptr[p];
+            L.acc1 = 0.0;
+            L.acc2 = 0.0;
+            progressed = true;
+            if (k + 1 < a.S_s) {
+                L.sub = k + 1;
+                ssf_begin<N>(d, L, (k + 1 == a.S_s - 1) ? a.dts_s[1] : a.dts_s[0]);
+            } else {
+                L.phase = PH_STORE;
+            }
+        } else if (L.phase == PH_STORE) {
+            // ---- average_flux_vars!, flux_to_river!  groundwater.jl:630-636, lateral_subsurface_flow.jl:182-196
+            const double s_exf_avg = L.s_exf_cum / dt;
+            const double s_qin_avg = L.s_qin_cum / dt;
+            const double s_q_avg = L.s_q_cum / dt;
+            const double s_qnet_avg = L.s_qnet_cum / dt;
+            const double rech_avg = L.rech_cum / dt;
+            const double s_tor_avg = L.s_tor_cum / dt;
+            d.ssf_q[p] = L.sq;
+            d.ssf_qin_avg[p] = s_qin_avg;
+            d.ssf_q_avg[p] = s_q_avg;
+            d.ssf_tor_avg[p] = s_tor_avg;
+            const double area = ld(d.area, p);
+            const double sy = ld(d.sy, p), dsoil = ld(d.d, p);
+            d.ssf_storage[p] = sy * (dsoil - L.szi) * area;
+            // ---- update_soil_water_storage!  soil.jl:1294-1392 ----
+            const double ts = ld(d.ts, p), tr = ld(d.tr, p), tfc = ld(d.tfc, p), rd = ld(d.rd, p);
+            const Column<N>& c = L.col;
+            double usd = 0.0;
+            {
+                bool first = true;
+#pragma unroll
+                for (int k = 0; k < N; ++k)
+                    if (k < c.n_unsat) {
+                        usd = first ? c.ustore[k] : usd + c.ustore[k];
+                        first = false;
+                    }
+            }
+            const double sbm_runoff =
+                wfh_max(0.0, s_exf_avg + d.sat_excess[p] + d.runoff_land[p] + d.inf_excess[p]);
+            const int nl = [&] {
+                int k2 = 0;
+#pragma unroll
+                for (int k = 0; k < N; ++k) k2 += !wfh_isnan(c.act[k]);
+                return k2;
+            }();
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                double v = NaN_;
+                if (k < nl) {
+                    if (k < c.n_unsat)
+                        v = (c.ustore[k] + (c.act[k] - c.thick[k]) * (ts - tr)) / c.act[k] + tr;
+                    else
+                        v = ts;
+                }
+                d.vwc[k * n + p] = v;
+            }
+            double rs_unsat = 0.0;
+#pragma unroll
+            for (int k = 0; k < N; ++k)
+                if (k < c.n_unsat)
+                    rs_unsat += wfh_min(1.0, (wfh_max(0.0, rd - c.cum[k]) / c.thick[k])) * c.ustore[k];
+            const double rs_sat = wfh_max(0.0, rd - c.zi) * (ts - tr);
+            const double rzs = rs_sat + rs_unsat;
+            const double swd = (dsoil - c.zi) * (ts - tr);
+            d.ustoredepth[p] = usd;
+            d.swd[p] = swd;
+            d.tsws[p] = swd + usd;
+            d.exfilt[p] = s_exf_avg;
+            d.runoff[p] = sbm_runoff;
+            d.rzs[p] = rzs;
+            d.vwc_root[p] = rzs / rd + tr;
+            const double net_runoff = sbm_runoff - d.aeow_l[p];
+            d.net_runoff[p] = net_runoff;
+            d.zi[p] = c.zi;
+#pragma unroll
+            for (int k = 0; k < N; ++k) d.ustore[k * n + p] = c.ustore[k];
+            (void)tfc;
+            if (d.balance) {
+                // subsurface flow balance  mass_balance.jl:470-490 (recharge is a boundary flux)
+                double fin = 0.0, fout = 0.0;
+                if (rech_avg > 0.0)
+                    fin += rech_avg;
+                else
+                    fout -= rech_avg;
+                const double tin = s_qin_avg + fin, tout = s_q_avg + fout;
+                d.bal_ssf[p] = tin - tout - s_qnet_avg;
+                // land balance  mass_balance.jl:215-253
+                const double t_in = s_qin_avg / area + (d.P[p] + 0.0) + 0.0 + 0.0 / area;
+                const double vout = net_runoff + d.aet[p] + d.net_runoff_river[p] + d.leakage[p] + 0.0;
+                const double t_out = s_q_avg / area + vout + 0.0 + 0.0 / area;
+                const double snow = d.snow ? d.snow_s[p] + d.snow_w[p] : 0.0 + 0.0;
+                const double storage = (swd + usd) + d.canopy[p] + snow + 0.0 * 0.0 + 0.0;
+                const double rate = (storage - d.store_prev[p]) / dt;
+                const double err = t_in - t_out - rate;
+                const double avg = (t_in + t_out) / 2.0;
+                d.bal_land[p] = err;
+                d.bal_land_rel[p] = avg == 0.0 ? 0.0 : err / avg;
+            }
+            // ---- overland lateral inflow  surface_kinwave.jl:740-766, 366-371 ----
+            L.o_inwater = (net_runoff + 0.0) * area + 0.0;
+            d.ol_inwater[p] = L.o_inwater;
+            L.oqlat = L.o_inwater / ld(d.fl, p);
+            L.o_q_cum = L.o_qin_cum = L.o_tor_cum = 0.0;
+            L.phase = PH_OL;
+            L.sub = 0;
+            progressed = true;
+        } else if (L.phase == PH_OL) {
+            // ---- kinwave_land_update!  surface_kinwave.jl:293-341 ----
+            const int k = L.sub;
+            uint64_t* slots = a.slot_ol + (size_t)k * n;
+            while (L.e < L.e_end) {
+                double qu;
+                if (!slot_take(slots + d.up_idx[L.e], qu)) return progressed;
+                const double fu = d.up_f[L.e];
+                L.acc1 = L.acc1 + qu * fu;
+                L.acc2 = L.acc2 + qu * (1.0 - fu);
+                ++L.e;
+            }
+            const double dts = (k == a.S_l - 1) ? a.dts_l[1] : a.dts_l[0];
+            L.o_tor_cum += L.acc1 * dts;
+            const double sfw = ld(d.sfw, p);
+            double qin = 0.0;
+            if (sfw > 0.0) qin = L.acc2;
+            double qn, ar;
+            const double flen = ld(d.fl, p);
+            kinematic_wave(qin, L.oq, L.oqlat, ld(d.alpha, p), dts, flen, qn, ar);
+            L.oq = qn;
+            if (sfw > 0.0) L.oh = ar / sfw;
+            L.o_storage = flen * sfw * L.oh;
+            L.o_q_cum += qn * dts;
+            L.o_qin_cum += qin * dts;
+            L.oqin = qin;
+            if (d.flags[p] & 1) slot_put(slots + p, qn);
+            L.e = d.up_ptr[p];
+            L.acc1 = 0.0;
+            L.acc2 = 0.0;
+            progressed = true;
+            if (k + 1 < a.S_l) {
+                L.sub = k + 1;
+            } else {
+                const double q_avg = L.o_q_cum / dt;
+                const double tor_avg = L.o_tor_cum / dt;
+                const double qin_avg = L.o_qin_cum / dt;
+                d.ol_q[p] = L.oq;
+                d.ol_h[p] = L.oh;
+                d.ol_storage[p] = L.o_storage;
+                d.ol_qin[p] = L.oqin;
+                d.ol_q_avg[p] = q_avg;
+                d.ol_tor_avg[p] = tor_avg;
+                d.ol_qin_avg[p] = qin_avg;
+                if (d.balance) {  // overland balance  mass_balance.jl:382-399
+                    const double tin = L.o_inwater + qin_avg, tout = q_avg;
+                    d.bal_ol[p] = tin - tout - (L.o_storage - L.o_store_prev) / dt;
+                }
+                if (L.r >= 0) {
+                    // ---- river lateral inflow  surface_kinwave.jl:710-734, 632 ----
+                    const int r = L.r;
+                    L.r_inwater = (d.ssf_tor_avg[p] + tor_avg + d.net_runoff_river[p] * ld(d.area, p)) +
+                                  0.0 * d.r_area[r];
+                    d.r_inwater[r] = L.r_inwater;
+                    L.rqlat = L.r_inwater / d.r_fl[r];
+                    L.r_q_cum = L.r_qin_cum = 0.0;
+                    L.e = d.rup_ptr[r];
+                    L.e_end = d.rup_ptr[r + 1];
+                    L.phase = PH_RIVER;
+                    L.sub = 0;
+                } else {
+                    L.phase = PH_FINAL;
+                }
+            }
+        } else if (L.phase == PH_RIVER) {
+            // ---- kinwave_river_update!  surface_kinwave.jl:492-566 ----
+            const int k = L.sub, r = L.r;
+            uint64_t* slots = a.slot_r + (size_t)k * d.nriv;
+            while (L.e < L.e_end) {
+                double qu;
+                if (!slot_take(slots + d.rup_idx[L.e], qu)) return progressed;
+                L.acc1 = L.acc1 + qu;
+                ++L.e;
+            }
+            const double dts = (k == a.S_r - 1) ? a.dts_r[1] : a.dts_r[0];
+            const double rfl = d.r_fl[r];
+            const double qin = 0.0 + L.acc1;
+            double inflow = 0.0 / rfl;
+            inflow -= 0.0 / rfl;
+            double qn, ar;
+            kinematic_wave(qin, L.rq, L.rqlat + inflow, d.r_alpha[r], dts, rfl, qn, ar);
+            L.rq = qn;
+            L.rh = ar / d.r_fw[r];
+            L.r_storage = rfl * ar;
+            L.r_q_cum += qn * dts;
+            L.r_qin_cum += qin * dts;
+            L.rqin = qin;
+            if (d.flags[p] & 4) slot_put(slots + r, qn);
+            L.e = d.rup_ptr[r];
+            L.acc1 = 0.0;
+            progressed = true;
+            if (k + 1 < a.S_r) {
+                L.sub = k + 1;
+            } else {
+                const double q_avg = L.r_q_cum / dt, qin_avg = L.r_qin_cum / dt;
+                d.r_q[r] = L.rq;
+                d.r_h[r] = L.rh;
+                d.r_storage[r] = L.r_storage;
+                d.r_qin[r] = L.rqin;
+                d.r_q_avg[r] = q_avg;
+                d.r_qin_avg[r] = qin_avg;
+                if (d.balance) {  // river balance  mass_balance.jl:290-313
+                    const double tin = L.r_inwater + qin_avg + wfh_max(0.0, 0.0);
+                    const double tout = q_avg + 0.0 + 0.0;
+                    d.bal_r[r] = tin - tout - (L.r_storage - L.r_store_prev) / dt;
+                }
+                L.phase = PH_FINAL;
+            }
+        } else if (L.phase == PH_FINAL) {
+            // ---- update_total_water_storage!  sbm.jl:143-182 ----
+            const double area = ld(d.area, p);
+            double tot = 0.0;
+            if (L.r >= 0) tot = (L.rh * d.r_fw[L.r] * d.r_fl[L.r]) / (area);
+            const double ss = d.snow ? d.snow_s[p] : 0.0, sw = d.snow ? d.snow_w[p] : 0.0;
+            tot = tot + ((((ss + sw) + 0.0 * 0.0) + d.canopy[p]) + 0.0);
+            const double sub = d.ustoredepth[p] + d.swd[p];
+            const double lat = L.oh * (1.0 - ld(d.rf, p));
+            tot += sub + lat;
+            d.total_storage[p] = tot;
+            L.phase = PH_DONE;
+            return true;
+        } else {
+            return progressed;
+        }
+    }
+}
+
+template <int N>
+__global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[a.parity ^ 1] = 0ull;  // next launch's counter
+    for (;;) {
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(&a.counters[a.parity], 1ull);
+        t = __shfl(t, 0);
+        if (t >= (unsigned long long)a.nchunks) break;
+        if (__hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        Lane<N> L;
+        L.p = (int)t * WAVE + lane;
+        if (L.p < d.n) {
+            lane_init<N>(d, L);
+            ssf_begin<N>(d, L, (a.S_s == 1) ? a.dts_s[1] : a.dts_s[0]);
+        } else {
+            L.phase = PH_DONE;
+        }
+        long spin = 0;
+        for (;;) {
+            const bool active = L.phase != PH_DONE;
+            if (!__any(active)) break;
+            bool prog = false;
+            if (active) prog = lane_advance<N>(d, a, L);
+            if (!__any(prog)) {
+                __builtin_amdgcn_s_sleep(2);
+                ++spin;
+                if ((spin & 255) == 0 &&
+                    __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    return;
+                if (spin > SPIN_LIMIT) {
+                    __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    return;
+                }
+            } else {
+                spin = 0;
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------------------------------------
+__global__ void k_fill_u64(uint64_t* p, uint64_t v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_gather(double* out, const double* in, const int* idx, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[idx[i]];
+}
+__global__ void k_scatter(double* out, const double* in, const int* idx, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[idx[i]] = in[i];
+}
+
+}  // namespace
+
+// ============================================================================================
+// host side
+// ============================================================================================
+struct wfh_handle {
+    wfh_config cfg{};
+    int64_t n = 0, nriv = 0;
+    int N = 0;
+    int S_s = 0, S_l = 0, S_r = 0;
+    double dts_s[2]{}, dts_l[2]{}, dts_r[2]{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[3]{};
+    bool timing = false, events_recorded = false;
+    bool forcing_set = false, broken = false;
+    int parity = 0;
+    int route_grid = 0;
+    int nchunks = 0;
+    int max_level = 0;
+    std::string err;
+    // host topology (land index order unless stated)
+    std::vector<int64_t> land_order, land_down, river_order, river_down, river_land;
+    std::vector<int> node_of;   // land index -> node
+    std::vector<int> land_of;   // node -> land index
+    // device allocations
+    std::vector<void*> allocs;
+    Dev d{};
+    RouteArgs ra{};
+    double *fP = nullptr, *fE = nullptr, *fT = nullptr;  // node-order forcing buffers
+    int* d_land_of = nullptr;
+    std::vector<double> staging;
+};
+
+namespace {
+
+const int PCR_DX[10] = {0, -1, 0, 1, -1, 0, 1, -1, 0, 1};
+const int PCR_DY[10] = {0, -1, -1, -1, 0, 0, 0, 1, 1, 1};
+
+int32_t fail(wfh_handle* h, int32_t code, const std::string& msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+#define HIPCHK(h, call)                                                                                  \
+    do {                                                                                                 \
+        hipError_t e_ = (call);                                                                          \
+        if (e_ != hipSuccess) return fail((h), WFH_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+T* dalloc(wfh_handle* h, size_t count, hipError_t& e) {
+    void* p = nullptr;
+    e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) h->allocs.push_back(p);
+    return (T*)p;
+}
+
+// flowgraph  Wflow/src/routing/utils.jl:6-33 over sorted raster indices; -1 = pit
+void flowgraph(const int64_t* idx, int64_t n, const uint8_t* ldd, int64_t nx, int64_t ny, std::vector<int64_t>& down,
+               std::vector<uint8_t>& ldd_out) {
+    down.assign(n, -1);
+    ldd_out.assign(n, 5);
+    for (int64_t i = 0; i < n; ++i) {
+        const int l = ldd[i];
+        ldd_out[i] = (uint8_t)l;
+        if (l == 5) continue;
+        if (l < 1 || l > 9) {
+            ldd_out[i] = 5;
+            continue;
+        }
+        const int64_t x = idx[i] % nx + PCR_DX[l], y = idx[i] / nx + PCR_DY[l];
+        if (x < 0 || y < 0 || x >= nx || y >= ny) {
+            ldd_out[i] = 5;
+            continue;
+        }
+        const int64_t t = x + y * nx;
+        const int64_t* it = std::lower_bound(idx, idx + n, t);
+        if (it == idx + n || *it != t) {
+            ldd_out[i] = 5;
+            continue;
+        }
+        down[i] = it - idx;
+    }
+}
+
+// Graphs.topological_sort_by_dfs for a graph with at most one out-neighbour per vertex.
+// Returns false on a cycle (is_cyclic check of flowgraph).
+bool toposort_dfs(const std::vector<int64_t>& down, std::vector<int64_t>& order) {
+    const int64_t n = (int64_t)down.size();
+    std::vector<uint8_t> color(n, 0);
+    std::vector<int64_t> post;
+    post.reserve(n);
+    std::vector<int64_t> stack;
+    for (int64_t v = 0; v < n; ++v) {
+        if (color[v]) continue;
+        stack.clear();
+        stack.push_back(v);
+        color[v] = 1;
+        while (!stack.empty()) {
+            const int64_t u = stack.back();
+            const int64_t w = down[u];
+            if (w >= 0 && color[w] == 1) return false;
+            if (w >= 0 && color[w] == 0) {
+                color[w] = 1;
+                stack.push_back(w);
+            } else {
+                color[u] = 2;
+                post.push_back(u);
+                stack.pop_back();
+            }
+        }
+    }
+    order.assign(post.rbegin(), post.rend());
+    return true;
+}
+
+double flow_length_of(int ldd, double xl, double yl) {  // get_flow_length  utils.jl:418-430
+    if (ldd == 2 || ldd == 8) return yl;
+    if (ldd == 4 || ldd == 6) return xl;
+    return std::hypot(xl, yl);
+}
+double flow_width_of(int ldd, double xl, double yl) {  // get_flow_width  utils.jl:439-450
+    if (ldd == 2 || ldd == 8) return xl;
+    if (ldd == 4 || ldd == 6) return yl;
+    return (xl * yl) / std::hypot(xl, yl);
+}
+
+// upload a parameter map (land order in) in node order, broadcast when uniform
+int32_t put_prm(wfh_handle* h, const std::vector<double>& v, Prm& out) {
+    const size_t n = v.size();
+    bool uniform = n > 0;
+    for (size_t i = 1; i < n && uniform; ++i) uniform = wfh_bits(v[i]) == wfh_bits(v[0]);
+    hipError_t e;
+    if (uniform) {
+        double* p = dalloc<double>(h, 1, e);
+        if (e != hipSuccess) return fail(h, WFH_ERR_HIP, "hipMalloc");
+        HIPCHK(h, hipMemcpy(p, v.data(), sizeof(double), hipMemcpyHostToDevice));
+        out = Prm{p, 0u};
+    } else {
+        std::vector<double> tmp(n);
+        for (size_t q = 0; q < n; ++q) tmp[q] = v[h->land_of[q]];
+        double* p = dalloc<double>(h, n, e);
+        if (e != hipSuccess) return fail(h, WFH_ERR_HIP, "hipMalloc");
+        HIPCHK(h, hipMemcpy(p, tmp.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        out = Prm{p, 0xFFFFFFFFu};
+    }
+    return WFH_OK;
+}
+
+// layered map [N][n] in land order
+int32_t put_prml(wfh_handle* h, const std::vector<double>& v, int N, PrmL& out) {
+    const size_t n = (size_t)h->n;
+    bool uniform = true;
+    for (int k = 0; k < N && uniform; ++k)
+        for (size_t i = 1; i < n && uniform; ++i) uniform = wfh_bits(v[k * n + i]) == wfh_bits(v[k * n]);
+    hipError_t e;
+    if (uniform) {
+        std::vector<double> tmp(N);
+        for (int k = 0; k < N; ++k) tmp[k] = v[k * n];
+        double* p = dalloc<double>(h, N, e);
+        if (e != hipSuccess) return fail(h, WFH_ERR_HIP, "hipMalloc");
+        HIPCHK(h, hipMemcpy(p, tmp.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        out = PrmL{p, 0u, 1u};
+    } else {
+        std::vector<double> tmp((size_t)N * n);
+        for (int k = 0; k < N; ++k)
+            for (size_t q = 0; q < n; ++q) tmp[k * n + q] = v[k * n + h->land_of[q]];
+        double* p = dalloc<double>(h, (size_t)N * n, e);
+        if (e != hipSuccess) return fail(h, WFH_ERR_HIP, "hipMalloc");
+        HIPCHK(h, hipMemcpy(p, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+        out = PrmL{p, 0xFFFFFFFFu, (uint32_t)n};
+    }
+    return WFH_OK;
+}
+
+// device array in node order from land-order host values
+int32_t put_state(wfh_handle* h, const std::vector<double>& v, double*& out, int layers = 1) {
+    const size_t n = (size_t)h->n;
+    hipError_t e;
+    out = dalloc<double>(h, n * layers, e);
+    if (e != hipSuccess) return fail(h, WFH_ERR_HIP, "hipMalloc");
+    if (!v.empty()) {
+        std::vector<double> tmp(n * layers);
+        for (int k = 0; k < layers; ++k)
+            for (size_t q = 0; q < n; ++q) tmp[k * n + q] = v[k * n + h->land_of[q]];
+        HIPCHK(h, hipMemcpy(out, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(h, hipMemset(out, 0, n * layers * sizeof(double)));
+    }
+    return WFH_OK;
+}
+
+int32_t put_river(wfh_handle* h, const std::vector<double>& v, double*& out) {
+    hipError_t e;
+    out = dalloc<double>(h, (size_t)h->nriv, e);
+    if (e != hipSuccess) return fail(h, WFH_ERR_HIP, "hipMalloc");
+    if (h->nriv > 0) {
+        if (!v.empty())
+            HIPCHK(h, hipMemcpy(out, v.data(), h->nriv * sizeof(double), hipMemcpyHostToDevice));
+        else
+            HIPCHK(h, hipMemset(out, 0, h->nriv * sizeof(double)));
+    }
+    return WFH_OK;
+}
+
+template <class T>
+int32_t put_raw(wfh_handle* h, const std::vector<T>& v, const T*& out) {
+    hipError_t e;
+    T* p = dalloc<T>(h, v.size(), e);
+    if (e != hipSuccess) return fail(h, WFH_ERR_HIP, "hipMalloc");
+    if (!v.empty()) HIPCHK(h, hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    out = p;
+    return WFH_OK;
+}
+
+int32_t launch_fill(wfh_handle* h, uint64_t* p, uint64_t v, size_t n) {
+    if (n == 0) return WFH_OK;
+    const int grid = (int)std::min<size_t>((n + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid), dim3(256), 0, h->stream, p, v, n);
+    HIPCHK(h, hipGetLastError());
+    return WFH_OK;
+}
+
+int sub_steps(double dt, double dt_fixed, double dts[2]) {
+    // while t < dt: dt_s = check_timestepsize(dt_fixed, t, dt) (routing/timestepping.jl:11-17)
+    double t = 0.0;
+    int k = 0;
+    dts[0] = dts[1] = 0.0;
+    while (t < dt) {
+        double s = dt_fixed;
+        if (t + s > dt) s = dt - t;
+        if (k == 0) dts[0] = s;
+        dts[1] = s;
+        t += s;
+        ++k;
+    }
+    return k;
+}
+
+struct ProgState {
+    double** p;
+    int layers;
+    bool river;
+};
+
+}  // namespace
+
+#define REQF(field)                                                                \
+    do {                                                                           \
+        if (!st->field) {                                                          \
+            h->err = "wfh_static." #field " is NULL";                              \
+            int32_t rc_ = WFH_ERR_ARG;                                             \
+            wfh_destroy_keep_error(h, out);                                        \
+            return rc_;                                                            \
+        }                                                                          \
+    } while (0)
+
+static thread_local std::string g_create_error;
+
+static void wfh_destroy_keep_error(wfh_handle* h, wfh_handle** out) {
+    g_create_error = h->err;
+    wfh_destroy(h);
+    *out = nullptr;
+}
+
+template <int N>
+static void launch_kernels(wfh_handle* h);
+
+extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_handle** out) {
+    if (!out) return WFH_ERR_ARG;
+    *out = nullptr;
+    if (!cfg || !st) {
+        g_create_error = "null argument";
+        return WFH_ERR_ARG;
+    }
+    if (cfg->abi_version != WFH_ABI_VERSION) {
+        g_create_error = "ABI version mismatch";
+        return WFH_ERR_ARG;
+    }
+    wfh_handle* h = new wfh_handle();
+    *out = h;
+    h->cfg = *cfg;
+    const int64_t n = st->n;
+    const int N = cfg->n_config_layers + 1;
+    if (n < 1 || n > (int64_t)INT32_MAX / 2 || N < 2 || N > 6 || cfg->dt <= 0.0 || cfg->land_dt <= 0.0 ||
+        cfg->river_dt <= 0.0 || cfg->subsurface_dt <= 0.0) {
+        h->err = "unsupported sizes (1 <= n < 2^30, 1 <= soil_layer__thickness count <= 5) or non-positive time step";
+        wfh_destroy_keep_error(h, out);
+        return WFH_ERR_ARG;
+    }
+    REQF(cell_index); REQF(ldd); REQF(x_length); REQF(y_length); REQF(land_slope); REQF(land_manning_n);
+    REQF(river_location); REQF(river_width_land); REQF(river_length_land); REQF(water_fraction); REQF(elevation);
+    REQF(canopy_gap_fraction); REQF(max_canopy_storage); REQF(rooting_depth); REQF(crop_coefficient);
+    REQF(theta_s); REQF(theta_r); REQF(kv_0); REQF(f); REQF(air_entry_pressure); REQF(h1); REQF(h2);
+    REQF(h3_high); REQF(h3_low); REQF(h4); REQF(alpha_h1); REQF(soil_thickness); REQF(infiltcap_compacted);
+    REQF(max_leakage); REQF(compacted_fraction); REQF(rootdistpar); REQF(cap_hmax); REQF(cap_n); REQF(w_soil);
+    REQF(cf_soil); REQF(brooks_corey_c); REQF(kvfrac); REQF(khfrac);
+    const bool gash = cfg->dt >= 23.0 * 3600.0;
+    if (gash) REQF(evap_precip_ratio);
+    if (cfg->snow) { REQF(cfmax); REQF(tt); REQF(tti); REQF(ttm); REQF(whc); }
+    for (int64_t i = 1; i < n; ++i)
+        if (st->cell_index[i] <= st->cell_index[i - 1]) {
+            h->err = "cell_index must be strictly ascending";
+            wfh_destroy_keep_error(h, out);
+            return WFH_ERR_ARG;
+        }
+    int64_t nriv = 0;
+    for (int64_t i = 0; i < n; ++i) nriv += st->river_location[i] != 0;
+    if (nriv != st->nriv) {
+        h->err = "nriv does not match river_location";
+        wfh_destroy_keep_error(h, out);
+        return WFH_ERR_ARG;
+    }
+    if (nriv > 0 && (!st->river_slope || !st->river_manning_n || !st->river_bankfull_depth || !st->river_length ||
+                     !st->river_width)) {
+        h->err = "river arrays are NULL";
+        wfh_destroy_keep_error(h, out);
+        return WFH_ERR_ARG;
+    }
+    h->n = n;
+    h->nriv = nriv;
+    h->N = N;
+    int32_t rc;
+
+    // ---------------- topology (network.jl:48-258) ----------------
+    std::vector<uint8_t> ldd2;
+    flowgraph(st->cell_index, n, st->ldd, st->nx, st->ny, h->land_down, ldd2);
+    if (!toposort_dfs(h->land_down, h->land_order)) {
+        h->err = "One or more cycles detected in flow graph";
+        wfh_destroy_keep_error(h, out);
+        return WFH_ERR_CYCLE;
+    }
+    std::vector<int64_t> land_river(n, -1);
+    for (int64_t i = 0; i < n; ++i)
+        if (st->river_location[i]) {
+            land_river[i] = (int64_t)h->river_land.size();
+            h->river_land.push_back(i);
+        }
+    {
+        std::vector<int64_t> ridx(nriv);
+        std::vector<uint8_t> rldd(nriv), rldd2;
+        for (int64_t r = 0; r < nriv; ++r) {
+            ridx[r] = st->cell_index[h->river_land[r]];
+            rldd[r] = ldd2[h->river_land[r]];
+        }
+        flowgraph(ridx.data(), nriv, rldd.data(), st->nx, st->ny, h->river_down, rldd2);
+        if (!toposort_dfs(h->river_down, h->river_order)) {
+            h->err = "One or more cycles detected in river flow graph";
+            wfh_destroy_keep_error(h, out);
+            return WFH_ERR_CYCLE;
+        }
+    }
+    // levels (longest upstream path) in DFS topological order, then the level-major node order
+    std::vector<int32_t> level(n, 0);
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t v = h->land_order[k], w = h->land_down[v];
+        if (w >= 0) level[w] = std::max(level[w], level[v] + 1);
+    }
+    h->max_level = 0;
+    for (int64_t i = 0; i < n; ++i) h->max_level = std::max(h->max_level, level[i]);
+    {
+        std::vector<int64_t> cnt(h->max_level + 2, 0);
+        for (int64_t i = 0; i < n; ++i) cnt[level[i] + 1]++;
+        for (int l = 0; l <= h->max_level; ++l) cnt[l + 1] += cnt[l];
+        h->land_of.assign(n, 0);
+        h->node_of.assign(n, 0);
+        for (int64_t i = 0; i < n; ++i) {  // ascending land index within a level
+            const int64_t q = cnt[level[i]]++;
+            h->land_of[q] = (int)i;
+            h->node_of[i] = (int)q;
+        }
+    }
+    // upstream CSR in node order, lists in ascending land index
+    std::vector<int> up_ptr(n + 1, 0), up_idx;
+    std::vector<int> river_of(n, -1);
+    std::vector<uint8_t> flags(n, 0);
+    {
+        std::vector<std::vector<int>> ups(n);
+        for (int64_t i = 0; i < n; ++i)
+            if (h->land_down[i] >= 0) ups[h->land_down[i]].push_back((int)i);  // ascending i
+        up_idx.reserve(n);
+        for (int64_t q = 0; q < n; ++q) {
+            const int v = h->land_of[q];
+            for (int u : ups[v]) up_idx.push_back(h->node_of[u]);
+            up_ptr[q + 1] = (int)up_idx.size();
+            if (h->land_down[v] >= 0) flags[q] |= 1;
+            if (land_river[v] >= 0) {
+                flags[q] |= 2;
+                river_of[q] = (int)land_river[v];
+                if (h->river_down[land_river[v]] >= 0) flags[q] |= 4;
+            }
+        }
+    }
+    std::vector<int> rup_ptr(nriv + 1, 0), rup_idx;
+    {
+        std::vector<std::vector<int>> ups(nriv);
+        for (int64_t r = 0; r < nriv; ++r)
+            if (h->river_down[r] >= 0) ups[h->river_down[r]].push_back((int)r);
+        for (int64_t r = 0; r < nriv; ++r) {
+            for (int u : ups[r]) rup_idx.push_back(u);
+            rup_ptr[r + 1] = (int)rup_idx.size();
+        }
+    }
+
+    // ---------------- land parameters (domain.jl:218-366) ----------------
+    std::vector<double> area(n), fw(n), fl(n), slope(n), rf(n), wf(n), sfw(n), fftr(n, 0.0), alpha(n);
+    for (int64_t i = 0; i < n; ++i) {
+        const double xl = st->x_length[i], yl = st->y_length[i];
+        area[i] = xl * yl;
+        fw[i] = flow_width_of(ldd2[i], xl, yl);
+        fl[i] = flow_length_of(ldd2[i], xl, yl);
+        slope[i] = wfh_clamp(st->land_slope[i], 0.00001, INFINITY);
+        rf[i] = st->river_location[i] ? wfh_min((st->river_length_land[i] * st->river_width_land[i]) / (area[i]), 1.0)
+                                      : 0.0;
+        wf[i] = wfh_max(st->water_fraction[i] - rf[i], 0.0);
+        const double land_area = (1.0 - rf[i]) * area[i];
+        sfw[i] = st->river_location[i] ? land_area / fl[i] : fw[i];
+    }
+    {  // get_flow_fraction_to_river  utils.jl:493-510 (every upstream j of a river cell i)
+        for (int64_t j = 0; j < n; ++j) {
+            const int64_t i = h->land_down[j];
+            if (i >= 0 && st->river_location[i] && ldd2[j] != ldd2[i]) fftr[j] = slope[j] / (slope[i] + slope[j]);
+        }
+    }
+    const double apow = (2.0 / 3.0) * 0.6;
+    for (int64_t i = 0; i < n; ++i) {  // ManningFlowParameters  surface_kinwave.jl:32-44
+        const double at = wfh_pow(st->land_manning_n[i] / std::sqrt(slope[i]), 0.6);
+        alpha[i] = at * wfh_pow(sfw[i], apow);
+    }
+    std::vector<double> r_alpha(nriv), r_fw(nriv), r_fl(nriv), r_area(nriv);
+    for (int64_t r = 0; r < nriv; ++r) {
+        const double s = wfh_clamp(st->river_slope[r], 0.00001, INFINITY);
+        r_fw[r] = st->river_width[r];
+        r_fl[r] = st->river_length[r];
+        const double wp = r_fw[r] + st->river_bankfull_depth[r];
+        const double at = wfh_pow(st->river_manning_n[r] / std::sqrt(s), 0.6);
+        r_alpha[r] = at * wfh_pow(wp, apow);
+        r_area[r] = area[h->river_land[r]];
+    }
+    // ---------------- soil (soil.jl:333-615) ----------------
+    double cfg_thick[WFH_MAX_LAYERS], cfg_cum[WFH_MAX_LAYERS + 1];
+    for (int k = 0; k < N - 1; ++k) cfg_thick[k] = cfg->soil_layer_thickness_mm[k] * 1e-3;
+    cfg_thick[N - 1] = NaN_;
+    cfg_cum[0] = 0.0;
+    cfg_cum[1] = cfg_thick[0];
+    for (int k = 1; k < N; ++k) cfg_cum[k + 1] = cfg_cum[k] + cfg_thick[k];
+    std::vector<double> tfc(n), rootfrac((size_t)N * n), rd(n), sy(n), kh0(n), qmax(n), q0(n), swd0(n), zi0(n),
+        sfrac(n), icap_s(n), storage0(n);
+    for (int64_t i = 0; i < n; ++i) {
+        double act[WFH_MAX_LAYERS], cum[WFH_MAX_LAYERS + 1];
+        int nl = 0;
+        {
+            for (int k = 0; k < N; ++k) act[k] = cfg_thick[k] * NaN_;
+            for (int k = 0; k < N; ++k) {
+                if (st->soil_thickness[i] > cfg_cum[k + 1])
+                    act[k] = cfg_thick[k];
+                else if (st->soil_thickness[i] - cfg_cum[k] > 0.0)
+                    act[k] = st->soil_thickness[i] - cfg_cum[k];
+                nl += !wfh_isnan(act[k]);
+            }
+            cum[0] = 0.0;
+            cum[1] = act[0];
+            for (int k = 1; k < N; ++k) cum[k + 1] = cum[k] + act[k];
+        }
+        const double ts = st->theta_s[i], tr = st->theta_r[i];
+        if (st->theta_fc) {
+            tfc[i] = st->theta_fc[i];
+        } else {  // field_capacity  soil_process.jl:138-160 with vwc_brooks_corey :99-106 at h = -1 m
+            double fc = 0.0, total = 0.0;
+            for (int k = 0; k < nl; ++k) {
+                const double hb = st->air_entry_pressure[i];
+                const double c = st->brooks_corey_c[(size_t)k * n + i];
+                double v;
+                if (-1.0 < hb)
+                    v = (ts - tr) * wfh_pow(hb / -1.0, 2.0 / (c - 3.0)) + tr;
+                else
+                    v = ts;
+                fc += v * act[k];
+                total += act[k];
+            }
+            tfc[i] = fc / total;
+        }
+        const double rdu = st->rooting_depth[i];
+        for (int k = 0; k < N; ++k) {  // default root fraction (unclipped rooting depth)  soil.jl:526-548
+            double v;
+            if (st->rootfraction)
+                v = st->rootfraction[(size_t)k * n + i];
+            else if (rdu > 0.0)
+                v = ((rdu - cum[k]) >= act[k]) ? act[k] / rdu : wfh_max((rdu - cum[k]) / rdu, 0.0);
+            else
+                v = 0.0;
+            rootfrac[(size_t)k * n + i] = v;
+        }
+        const double d = st->soil_thickness[i];
+        const double swc = d * (ts - tr);
+        rd[i] = wfh_min(d * 0.99, rdu);  // sbm.jl:57-58
+        swd0[i] = 0.85 * swc;             // soil.jl:164
+        zi0[i] = wfh_max(0.0, d - swd0[i] / (ts - tr));
+        sfrac[i] = wfh_max(st->canopy_gap_fraction[i] - wf[i] - rf[i] - 0.0, 0.0);  // soil_fraction!
+        icap_s[i] = st->kv_0[i] * st->kvfrac[i];
+        // lateral subsurface flow (lateral_subsurface_flow.jl:81-171, utils.jl:916-937)
+        sy[i] = wfh_max(ts - tfc[i], 0.02);
+        kh0[i] = st->khfrac[i] * st->kv_0[i];
+        const double fk = st->f[i];
+        qmax[i] = ((kh0[i] * slope[i]) / fk) * (1.0 - wfh_exp(-fk * d));
+        q0[i] = ((kh0[i] * slope[i]) / fk) * (wfh_exp(-fk * zi0[i]) - wfh_exp(-fk * d)) * fw[i];
+    }
+
+    // ---------------- node order, uploads ----------------
+    if (hipSetDevice(cfg->device) != hipSuccess) {
+        h->err = "hipSetDevice failed";
+        wfh_destroy_keep_error(h, out);
+        return WFH_ERR_HIP;
+    }
+    auto V = [&](const double* p) { return std::vector<double>(p, p + n); };
+    Dev& D = h->d;
+    D.n = (int)n;
+    D.nriv = (int)nriv;
+    D.dt = cfg->dt;
+    D.gash = gash;
+    D.snow = cfg->snow != 0;
+    D.infred = cfg->soil_infiltration_reduction != 0;
+    D.balance = cfg->water_mass_balance != 0;
+    for (int k = 0; k < WFH_MAX_LAYERS; ++k) D.cfg_thick[k] = k < N ? cfg_thick[k] : NaN_;
+    for (int k = 0; k <= WFH_MAX_LAYERS; ++k) D.cfg_cum[k] = k <= N ? cfg_cum[k] : NaN_;
+    std::vector<double> zero(n, 0.0);
+    if ((rc = put_prm(h, V(st->canopy_gap_fraction), D.gap))) goto done;
+    if ((rc = put_prm(h, V(st->max_canopy_storage), D.smax))) goto done;
+    if ((rc = put_prm(h, V(st->crop_coefficient), D.kc))) goto done;
+    if ((rc = put_prm(h, gash ? V(st->evap_precip_ratio) : zero, D.er))) goto done;
+    if ((rc = put_prm(h, D.snow ? V(st->cfmax) : zero, D.cfmax))) goto done;
+    if ((rc = put_prm(h, D.snow ? V(st->tt) : zero, D.tt))) goto done;
+    if ((rc = put_prm(h, D.snow ? V(st->tti) : zero, D.tti))) goto done;
+    if ((rc = put_prm(h, D.snow ? V(st->ttm) : zero, D.ttm))) goto done;
+    if ((rc = put_prm(h, D.snow ? V(st->whc) : zero, D.whc))) goto done;
+    if ((rc = put_prm(h, rf, D.rf))) goto done;
+    if ((rc = put_prm(h, wf, D.wf))) goto done;
+    if ((rc = put_prm(h, sfrac, D.sfrac))) goto done;
+    if ((rc = put_prm(h, V(st->theta_s), D.ts))) goto done;
+    if ((rc = put_prm(h, V(st->theta_r), D.tr))) goto done;
+    if ((rc = put_prm(h, tfc, D.tfc))) goto done;
+    if ((rc = put_prm(h, V(st->kv_0), D.kv0))) goto done;
+    if ((rc = put_prm(h, V(st->f), D.f))) goto done;
+    if ((rc = put_prm(h, V(st->air_entry_pressure), D.hb))) goto done;
+    if ((rc = put_prm(h, V(st->h1), D.h1))) goto done;
+    if ((rc = put_prm(h, V(st->h2), D.h2))) goto done;
+    if ((rc = put_prm(h, V(st->h3_high), D.h3h))) goto done;
+    if ((rc = put_prm(h, V(st->h3_low), D.h3l))) goto done;
+    if ((rc = put_prm(h, V(st->h4), D.h4))) goto done;
+    if ((rc = put_prm(h, V(st->alpha_h1), D.ah1))) goto done;
+    if ((rc = put_prm(h, V(st->soil_thickness), D.d))) goto done;
+    if ((rc = put_prm(h, icap_s, D.icap_s))) goto done;
+    if ((rc = put_prm(h, V(st->infiltcap_compacted), D.icap_c))) goto done;
+    if ((rc = put_prm(h, V(st->max_leakage), D.maxleak))) goto done;
+    if ((rc = put_prm(h, V(st->compacted_fraction), D.compf))) goto done;
+    if ((rc = put_prm(h, V(st->rootdistpar), D.rdp))) goto done;
+    if ((rc = put_prm(h, V(st->cap_hmax), D.caph))) goto done;
+    if ((rc = put_prm(h, V(st->cap_n), D.capn))) goto done;
+    if ((rc = put_prm(h, V(st->w_soil), D.wsoil))) goto done;
+    if ((rc = put_prm(h, V(st->cf_soil), D.cfsoil))) goto done;
+    if ((rc = put_prm(h, rd, D.rd))) goto done;
+    if ((rc = put_prml(h, std::vector<double>(st->brooks_corey_c, st->brooks_corey_c + (size_t)N * n), N, D.c)))
+        goto done;
+    if ((rc = put_prml(h, std::vector<double>(st->kvfrac, st->kvfrac + (size_t)N * n), N, D.kvfrac))) goto done;
+    if ((rc = put_prml(h, rootfrac, N, D.rootfrac))) goto done;
+    if ((rc = put_prm(h, area, D.area))) goto done;
+    if ((rc = put_prm(h, fl, D.fl))) goto done;
+    if ((rc = put_prm(h, fw, D.fw))) goto done;
+    if ((rc = put_prm(h, slope, D.slope))) goto done;
+    if ((rc = put_prm(h, sy, D.sy))) goto done;
+    if ((rc = put_prm(h, kh0, D.kh0))) goto done;
+    if ((rc = put_prm(h, qmax, D.qmax))) goto done;
+    if ((rc = put_prm(h, alpha, D.alpha))) goto done;
+    if ((rc = put_prm(h, sfw, D.sfw))) goto done;
+    if ((rc = put_prm(h, V(st->elevation), D.elev))) goto done;
+    {
+        // topology
+        std::vector<double> up_f(up_idx.size());
+        for (size_t e = 0; e < up_idx.size(); ++e) up_f[e] = fftr[h->land_of[up_idx[e]]];
+        if ((rc = put_raw<int>(h, up_ptr, D.up_ptr))) goto done;
+        if ((rc = put_raw<int>(h, up_idx, D.up_idx))) goto done;
+        if ((rc = put_raw<double>(h, up_f, D.up_f))) goto done;
+        if ((rc = put_raw<int>(h, river_of, D.river_of))) goto done;
+        if ((rc = put_raw<uint8_t>(h, flags, D.flags))) goto done;
+        if ((rc = put_raw<int>(h, rup_ptr, D.rup_ptr))) goto done;
+        if ((rc = put_raw<int>(h, rup_idx, D.rup_idx))) goto done;
+        if ((rc = put_raw<double>(h, r_alpha, D.r_alpha))) goto done;
+        if ((rc = put_raw<double>(h, r_fw, D.r_fw))) goto done;
+        if ((rc = put_raw<double>(h, r_fl, D.r_fl))) goto done;
+        if ((rc = put_raw<double>(h, r_area, D.r_area))) goto done;
+        const int* lo = nullptr;
+        if ((rc = put_raw<int>(h, h->land_of, lo))) goto done;
+        h->d_land_of = const_cast<int*>(lo);
+    }
+    {
+        // cold-start states (soil.jl:150-201, canopy/snow zeros, routing zeros)
+        std::vector<double> tsoil0(n, 10.0 + 273.15), ssf_storage0(n);
+        for (int64_t i = 0; i < n; ++i) ssf_storage0[i] = sy[i] * (st->soil_thickness[i] - zi0[i]) * area[i];
+        if ((rc = put_state(h, {}, D.canopy))) goto done;
+        if ((rc = put_state(h, {}, D.snow_s))) goto done;
+        if ((rc = put_state(h, {}, D.snow_w))) goto done;
+        if ((rc = put_state(h, tsoil0, D.tsoil))) goto done;
+        if ((rc = put_state(h, swd0, D.swd))) goto done;
+        if ((rc = put_state(h, {}, D.ustore, N))) goto done;
+        if ((rc = put_state(h, swd0, D.tsws))) goto done;
+        if ((rc = put_state(h, q0, D.ssf_q))) goto done;
+        if ((rc = put_state(h, {}, D.ol_q))) goto done;
+        if ((rc = put_state(h, {}, D.ol_h))) goto done;
+        if ((rc = put_state(h, {}, D.ol_storage))) goto done;
+        if ((rc = put_river(h, {}, D.r_q))) goto done;
+        if ((rc = put_river(h, {}, D.r_h))) goto done;
+        if ((rc = put_river(h, {}, D.r_storage))) goto done;
+        double** outs[] = {&D.zi, &D.interception, &D.swe, &D.transfer, &D.recharge, &D.act_inf, &D.inf_excess,
+                           &D.sat_excess, &D.transpiration, &D.soil_evap, &D.capflux, &D.leakage, &D.aet,
+                           &D.net_runoff_river, &D.runoff_land, &D.aeow_l, &D.store_prev, &D.ustoredepth, &D.runoff,
+                           &D.net_runoff, &D.exfilt, &D.vwc_root, &D.rzs, &D.ssf_qin_avg, &D.ssf_q_avg,
+                           &D.ssf_tor_avg, &D.ssf_storage, &D.ol_qin, &D.ol_q_avg, &D.ol_qin_avg, &D.ol_tor_avg,
+                           &D.ol_inwater, &D.total_storage, &D.bal_land, &D.bal_land_rel, &D.bal_ssf, &D.bal_ol};
+        for (double** o : outs)
+            if ((rc = put_state(h, {}, *o))) goto done;
+        if ((rc = put_state(h, zi0, D.zi))) goto done;
+        if ((rc = put_state(h, ssf_storage0, D.ssf_storage))) goto done;
+        if ((rc = put_state(h, {}, D.vwc, N))) goto done;
+        double** routs[] = {&D.r_qin, &D.r_q_avg, &D.r_qin_avg, &D.r_inwater, &D.bal_r};
+        for (double** o : routs)
+            if ((rc = put_river(h, {}, *o))) goto done;
+        hipError_t e;
+        h->fP = dalloc<double>(h, n, e);
+        h->fE = dalloc<double>(h, n, e);
+        h->fT = dalloc<double>(h, n, e);
+        if (e != hipSuccess) {
+            rc = fail(h, WFH_ERR_HIP, "hipMalloc forcing");
+            goto done;
+        }
+    }
+    {
+        // routing sub-steps (fixed kinematic-wave time steps, config_structure.jl:84-88)
+        h->S_s = sub_steps(cfg->dt, cfg->subsurface_dt, h->dts_s);
+        h->S_l = sub_steps(cfg->dt, cfg->land_dt, h->dts_l);
+        h->S_r = sub_steps(cfg->dt, cfg->river_dt, h->dts_r);
+        hipError_t e;
+        RouteArgs& A = h->ra;
+        A.slot_ssf = dalloc<uint64_t>(h, (size_t)h->S_s * n, e);
+        if (e != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipMalloc slots"); goto done; }
+        A.slot_ol = dalloc<uint64_t>(h, (size_t)h->S_l * n, e);
+        if (e != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipMalloc slots"); goto done; }
+        A.slot_r = dalloc<uint64_t>(h, (size_t)h->S_r * std::max<int64_t>(nriv, 1), e);
+        if (e != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipMalloc slots"); goto done; }
+        A.counters = dalloc<unsigned long long>(h, 2, e);
+        A.abort_flag = dalloc<int>(h, 1, e);
+        if (e != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipMalloc counters"); goto done; }
+        A.S_s = h->S_s;
+        A.S_l = h->S_l;
+        A.S_r = h->S_r;
+        for (int k = 0; k < 2; ++k) {
+            A.dts_s[k] = h->dts_s[k];
+            A.dts_l[k] = h->dts_l[k];
+            A.dts_r[k] = h->dts_r[k];
+        }
+        h->nchunks = (int)((n + WAVE - 1) / WAVE);
+        A.nchunks = h->nchunks;
+        if (hipStreamCreate(&h->stream) != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipStreamCreate"); goto done; }
+        for (auto& ev : h->ev)
+            if (hipEventCreate(&ev) != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipEventCreate"); goto done; }
+        if ((rc = launch_fill(h, A.slot_ssf, SLOT_EMPTY, (size_t)h->S_s * n))) goto done;
+        if ((rc = launch_fill(h, A.slot_ol, SLOT_EMPTY, (size_t)h->S_l * n))) goto done;
+        if ((rc = launch_fill(h, A.slot_r, SLOT_EMPTY, (size_t)h->S_r * std::max<int64_t>(nriv, 1)))) goto done;
+        if ((rc = launch_fill(h, (uint64_t*)A.counters, 0, 2))) goto done;
+        if (hipMemsetAsync(A.abort_flag, 0, sizeof(int), h->stream) != hipSuccess) {
+            rc = fail(h, WFH_ERR_HIP, "hipMemsetAsync");
+            goto done;
+        }
+        // persistent grid: as many resident blocks as the device holds, at most one per 4 chunks
+        int dev = cfg->device, cus = 0, per_cu = 0;
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        hipError_t oe = hipSuccess;
+        switch (N) {
+            case 2: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_route<2>, ROUTE_BLOCK, 0); break;
+            case 3: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_route<3>, ROUTE_BLOCK, 0); break;
+            case 4: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_route<4>, ROUTE_BLOCK, 0); break;
+            case 5: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_route<5>, ROUTE_BLOCK, 0); break;
+            default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_route<6>, ROUTE_BLOCK, 0); break;
+        }
+        if (oe != hipSuccess || per_cu < 1) per_cu = 1;
+        if (cus < 1) cus = 1;
+        const int want = (h->nchunks + (ROUTE_BLOCK / WAVE) - 1) / (ROUTE_BLOCK / WAVE);
+        h->route_grid = std::max(1, std::min(want, per_cu * cus));
+        if (hipStreamSynchronize(h->stream) != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "init sync"); goto done; }
+    }
+    D.P = h->fP;
+    D.E = h->fE;
+    D.T = h->fT;
+    return WFH_OK;
+done:
+    wfh_destroy_keep_error(h, out);
+    return rc;
+}
+
+template <int N>
+static void launch_kernels(wfh_handle* h) {
+    const int n = (int)h->n;
+    if (h->timing) hipEventRecord(h->ev[0], h->stream);
+    hipLaunchKernelGGL(k_vertical<N>, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d);
+    if (h->timing) hipEventRecord(h->ev[1], h->stream);
+    RouteArgs a = h->ra;
+    a.parity = h->parity;
+    h->parity ^= 1;
+    hipLaunchKernelGGL(k_route<N>, dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, a);
+    if (h->timing) {
+        hipEventRecord(h->ev[2], h->stream);
+        h->events_recorded = true;
+    }
+}
+
+extern "C" int32_t wfh_set_forcing(wfh_handle* h, const double* P, const double* E, const double* T,
+                                   int32_t memkind) {
+    if (!h) return WFH_ERR_ARG;
+    if (!P || !E || !T) return fail(h, WFH_ERR_ARG, "null forcing pointer");
+    const int n = (int)h->n;
+    if (memkind == WFH_MEM_HOST) {
+        h->staging.resize((size_t)3 * n);
+        for (int q = 0; q < n; ++q) {
+            const int i = h->land_of[q];
+            h->staging[q] = P[i];
+            h->staging[n + q] = E[i];
+            h->staging[2 * n + q] = T[i];
+        }
+        HIPCHK(h, hipMemcpyAsync(h->fP, h->staging.data(), n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->fE, h->staging.data() + n, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->fT, h->staging.data() + 2 * n, n * sizeof(double), hipMemcpyHostToDevice,
+                                 h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));  // staging buffer reuse
+        h->d.P = h->fP;
+        h->d.E = h->fE;
+        h->d.T = h->fT;
+    } else if (memkind == WFH_MEM_DEVICE) {
+        const int g = (n + 255) / 256;
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(256), 0, h->stream, h->fP, P, h->d_land_of, n);
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(256), 0, h->stream, h->fE, E, h->d_land_of, n);
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(256), 0, h->stream, h->fT, T, h->d_land_of, n);
+        HIPCHK(h, hipGetLastError());
+        h->d.P = h->fP;
+        h->d.E = h->fE;
+        h->d.T = h->fT;
+    } else if (memkind == WFH_MEM_DEVICE_NODE_ORDER) {
+        h->d.P = P;
+        h->d.E = E;
+        h->d.T = T;
+    } else {
+        return fail(h, WFH_ERR_ARG, "bad memkind");
+    }
+    h->forcing_set = true;
+    return WFH_OK;
+}
+
+extern "C" int32_t wfh_step(wfh_handle* h) {
+    if (!h) return WFH_ERR_ARG;
+    if (h->broken) return fail(h, WFH_ERR_STATE, "handle is in an error state (a previous step failed)");
+    if (!h->forcing_set) return fail(h, WFH_ERR_STATE, "wfh_set_forcing must be called before wfh_step");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    switch (h->N) {
+        case 2: launch_kernels<2>(h); break;
+        case 3: launch_kernels<3>(h); break;
+        case 4: launch_kernels<4>(h); break;
+        case 5: launch_kernels<5>(h); break;
+        default: launch_kernels<6>(h); break;
+    }
+    HIPCHK(h, hipGetLastError());
+    return WFH_OK;
+}
+
+extern "C" int32_t wfh_sync(wfh_handle* h) {
+    if (!h) return WFH_ERR_ARG;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    int flag = 0;
+    HIPCHK(h, hipMemcpy(&flag, h->ra.abort_flag, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) {
+        h->broken = true;
+        return fail(h, WFH_ERR_DEVICE, "routing kernel aborted: a dependency wait exceeded its spin limit");
+    }
+    return WFH_OK;
+}
+
+namespace {
+struct VarInfo {
+    double* p;
+    int layers;
+    bool river;
+    bool settable;
+};
+
+VarInfo var_info(wfh_handle* h, int32_t v) {
+    Dev& D = h->d;
+    const int N = h->N;
+    switch (v) {
+        case WFH_CANOPY_STORAGE: return {D.canopy, 1, false, true};
+        case WFH_SNOW_STORAGE: return {D.snow_s, 1, false, true};
+        case WFH_SNOW_WATER: return {D.snow_w, 1, false, true};
+        case WFH_SOIL_TEMPERATURE: return {D.tsoil, 1, false, true};
+        case WFH_SATWATERDEPTH: return {D.swd, 1, false, true};
+        case WFH_USTORELAYERDEPTH: return {D.ustore, N, false, true};
+        case WFH_WATER_TABLE_DEPTH: return {D.zi, 1, false, false};
+        case WFH_USTOREDEPTH: return {D.ustoredepth, 1, false, false};
+        case WFH_TRANSFER: return {D.transfer, 1, false, false};
+        case WFH_RECHARGE: return {D.recharge, 1, false, false};
+        case WFH_ACTUAL_INFILTRATION: return {D.act_inf, 1, false, false};
+        case WFH_INFILTRATION_EXCESS: return {D.inf_excess, 1, false, false};
+        case WFH_SATURATION_EXCESS: return {D.sat_excess, 1, false, false};
+        case WFH_TRANSPIRATION: return {D.transpiration, 1, false, false};
+        case WFH_SOIL_EVAPORATION: return {D.soil_evap, 1, false, false};
+        case WFH_ACTUAL_CAPILLARY_FLUX: return {D.capflux, 1, false, false};
+        case WFH_ACTUAL_LEAKAGE: return {D.leakage, 1, false, false};
+        case WFH_ACTUAL_EVAPOTRANSPIRATION: return {D.aet, 1, false, false};
+        case WFH_INTERCEPTION_RATE: return {D.interception, 1, false, false};
+        case WFH_NET_RUNOFF: return {D.net_runoff, 1, false, false};
+        case WFH_RUNOFF: return {D.runoff, 1, false, false};
+        case WFH_NET_RUNOFF_RIVER: return {D.net_runoff_river, 1, false, false};
+        case WFH_EXFILTRATION: return {D.exfilt, 1, false, false};
+        case WFH_TOTAL_STORAGE: return {D.total_storage, 1, false, false};
+        case WFH_SNOW_WATER_EQUIVALENT: return {D.swe, 1, false, false};
+        case WFH_VWC: return {D.vwc, N, false, false};
+        case WFH_VWC_ROOT_ZONE: return {D.vwc_root, 1, false, false};
+        case WFH_ROOT_ZONE_STORAGE: return {D.rzs, 1, false, false};
+        case WFH_SSF_Q: return {D.ssf_q, 1, false, true};
+        case WFH_SSF_QIN_AVERAGE: return {D.ssf_qin_avg, 1, false, false};
+        case WFH_SSF_Q_AVERAGE: return {D.ssf_q_avg, 1, false, false};
+        case WFH_SSF_TO_RIVER_AVERAGE: return {D.ssf_tor_avg, 1, false, false};
+        case WFH_SSF_STORAGE: return {D.ssf_storage, 1, false, false};
+        case WFH_LAND_Q: return {D.ol_q, 1, false, true};
+        case WFH_LAND_QIN: return {D.ol_qin, 1, false, false};
+        case WFH_LAND_H: return {D.ol_h, 1, false, true};
+        case WFH_LAND_STORAGE: return {D.ol_storage, 1, false, false};
+        case WFH_LAND_Q_AVERAGE: return {D.ol_q_avg, 1, false, false};
+        case WFH_LAND_QIN_AVERAGE: return {D.ol_qin_avg, 1, false, false};
+        case WFH_LAND_TO_RIVER_AVERAGE: return {D.ol_tor_avg, 1, false, false};
+        case WFH_LAND_INWATER: return {D.ol_inwater, 1, false, false};
+        case WFH_RIVER_Q: return {D.r_q, 1, true, true};
+        case WFH_RIVER_QIN: return {D.r_qin, 1, true, false};
+        case WFH_RIVER_H: return {D.r_h, 1, true, true};
+        case WFH_RIVER_STORAGE: return {D.r_storage, 1, true, false};
+        case WFH_RIVER_Q_AVERAGE: return {D.r_q_avg, 1, true, false};
+        case WFH_RIVER_QIN_AVERAGE: return {D.r_qin_avg, 1, true, false};
+        case WFH_RIVER_INWATER: return {D.r_inwater, 1, true, false};
+        case WFH_BALANCE_LAND_ERROR: return {D.bal_land, 1, false, false};
+        case WFH_BALANCE_LAND_RELERROR: return {D.bal_land_rel, 1, false, false};
+        case WFH_BALANCE_SSF_ERROR: return {D.bal_ssf, 1, false, false};
+        case WFH_BALANCE_OVERLAND_ERROR: return {D.bal_ol, 1, false, false};
+        case WFH_BALANCE_RIVER_ERROR: return {D.bal_r, 1, true, false};
+        case WFH_RIVER_ALPHA: return {const_cast<double*>(D.r_alpha), 1, true, false};
+        default: return {nullptr, 0, false, false};
+    }
+}
+}  // namespace
+
+// parameters stored as broadcast-or-node-order maps
+static bool prm_var(wfh_handle* h, int32_t v, Prm& p) {
+    switch (v) {
+        case WFH_LAND_ALPHA: p = h->d.alpha; return true;
+        case WFH_THETA_FC: p = h->d.tfc; return true;
+        case WFH_SURFACE_FLOW_WIDTH: p = h->d.sfw; return true;
+        case WFH_SSF_QMAX: p = h->d.qmax; return true;
+        default: return false;
+    }
+}
+
+extern "C" int32_t wfh_get(wfh_handle* h, int32_t var, double* dst, int64_t count, int32_t memkind) {
+    if (!h || !dst) return WFH_ERR_ARG;
+    if (memkind != WFH_MEM_HOST) return fail(h, WFH_ERR_ARG, "wfh_get supports WFH_MEM_HOST");
+    int32_t rc = wfh_sync(h);
+    if (rc) return rc;
+    const int64_t n = h->n;
+    Prm pm;
+    if (prm_var(h, var, pm)) {
+        if (count != n) return fail(h, WFH_ERR_ARG, "count mismatch");
+        std::vector<double> tmp(pm.mask ? n : 1);
+        HIPCHK(h, hipMemcpy(tmp.data(), pm.p, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t q = 0; q < n; ++q) dst[h->land_of[q]] = pm.mask ? tmp[q] : tmp[0];
+        return WFH_OK;
+    }
+    if (var == WFH_FLOW_FRACTION_TO_RIVER) {
+        if (count != n) return fail(h, WFH_ERR_ARG, "count mismatch");
+        std::vector<int> ptr(n + 1), idx;
+        std::vector<double> f;
+        HIPCHK(h, hipMemcpy(ptr.data(), h->d.up_ptr, (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+        idx.resize(ptr[n]);
+        f.resize(ptr[n]);
+        HIPCHK(h, hipMemcpy(idx.data(), h->d.up_idx, idx.size() * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(f.data(), h->d.up_f, f.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) dst[i] = 0.0;
+        for (size_t e = 0; e < idx.size(); ++e) dst[h->land_of[idx[e]]] = f[e];
+        return WFH_OK;
+    }
+    VarInfo vi = var_info(h, var);
+    if (!vi.p) return fail(h, WFH_ERR_ARG, "unknown variable");
+    if (vi.river) {
+        if (count != h->nriv) return fail(h, WFH_ERR_ARG, "count mismatch");
+        if (h->nriv) HIPCHK(h, hipMemcpy(dst, vi.p, h->nriv * sizeof(double), hipMemcpyDeviceToHost));
+        return WFH_OK;
+    }
+    if (count != n * vi.layers) return fail(h, WFH_ERR_ARG, "count mismatch");
+    std::vector<double> tmp((size_t)n * vi.layers);
+    HIPCHK(h, hipMemcpy(tmp.data(), vi.p, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < vi.layers; ++k)
+        for (int64_t q = 0; q < n; ++q) dst[k * n + h->land_of[q]] = tmp[k * n + q];
+    return WFH_OK;
+}
+
+extern "C" int32_t wfh_set(wfh_handle* h, int32_t var, const double* src, int64_t count, int32_t memkind) {
+    if (!h || !src) return WFH_ERR_ARG;
+    if (memkind != WFH_MEM_HOST) return fail(h, WFH_ERR_ARG, "wfh_set supports WFH_MEM_HOST");
+    VarInfo vi = var_info(h, var);
+    if (!vi.p || !vi.settable) return fail(h, WFH_ERR_ARG, "variable is not settable");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const int64_t n = h->n;
+    if (vi.river) {
+        if (count != h->nriv) return fail(h, WFH_ERR_ARG, "count mismatch");
+        if (h->nriv) HIPCHK(h, hipMemcpy(vi.p, src, h->nriv * sizeof(double), hipMemcpyHostToDevice));
+        if (var == WFH_RIVER_H) {  // keep storage consistent (sbm_model.jl:174-176)
+            std::vector<double> s(h->nriv), fw(h->nriv), fl(h->nriv);
+            HIPCHK(h, hipMemcpy(fw.data(), h->d.r_fw, h->nriv * sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(h, hipMemcpy(fl.data(), h->d.r_fl, h->nriv * sizeof(double), hipMemcpyDeviceToHost));
+            for (int64_t r = 0; r < h->nriv; ++r) s[r] = src[r] * fw[r] * fl[r];
+            HIPCHK(h, hipMemcpy(h->d.r_storage, s.data(), h->nriv * sizeof(double), hipMemcpyHostToDevice));
+        }
+        return WFH_OK;
+    }
+    if (count != n * vi.layers) return fail(h, WFH_ERR_ARG, "count mismatch");
+    std::vector<double> tmp((size_t)n * vi.layers);
+    for (int k = 0; k < vi.layers; ++k)
+        for (int64_t q = 0; q < n; ++q) tmp[k * n + q] = src[k * n + h->land_of[q]];
+    HIPCHK(h, hipMemcpy(vi.p, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+    return WFH_OK;
+}
+
+extern "C" int32_t wfh_get_topology(wfh_handle* h, int64_t* land_order, int64_t* land_down, int64_t* river_order,
+                                    int64_t* river_down, int64_t* river_land_index, int32_t* max_level) {
+    if (!h) return WFH_ERR_ARG;
+    if (land_order) std::copy(h->land_order.begin(), h->land_order.end(), land_order);
+    if (land_down) std::copy(h->land_down.begin(), h->land_down.end(), land_down);
+    if (river_order) std::copy(h->river_order.begin(), h->river_order.end(), river_order);
+    if (river_down) std::copy(h->river_down.begin(), h->river_down.end(), river_down);
+    if (river_land_index) std::copy(h->river_land.begin(), h->river_land.end(), river_land_index);
+    if (max_level) *max_level = h->max_level;
+    return WFH_OK;
+}
+
+extern "C" int32_t wfh_get_node_order(wfh_handle* h, int64_t* land_index_of_node) {
+    if (!h || !land_index_of_node) return WFH_ERR_ARG;
+    for (int64_t q = 0; q < h->n; ++q) land_index_of_node[q] = h->land_of[q];
+    return WFH_OK;
+}
+
+// checkpoint: prognostic states in node order (states.jl:73-170 set plus the storages that the
+// next step's balance reads)
+static std::vector<std::pair<double*, size_t>> state_blocks(wfh_handle* h) {
+    Dev& D = h->d;
+    const size_t n = (size_t)h->n, r = (size_t)h->nriv;
+    return {{D.canopy, n},     {D.snow_s, n}, {D.snow_w, n}, {D.tsoil, n},          {D.swd, n},
+            {D.ustore, n * h->N}, {D.tsws, n}, {D.ssf_q, n}, {D.ol_q, n},           {D.ol_h, n},
+            {D.ol_storage, n}, {D.r_q, r},     {D.r_h, r},      {D.r_storage, r}};
+}
+
+extern "C" int64_t wfh_state_size(const wfh_handle* h) {
+    if (!h) return -1;
+    size_t b = 4 * sizeof(int64_t);
+    for (auto& s : state_blocks(const_cast<wfh_handle*>(h))) b += s.second * sizeof(double);
+    return (int64_t)b;
+}
+
+extern "C" int32_t wfh_save_state(wfh_handle* h, void* buf, int64_t bytes) {
+    if (!h || !buf || bytes != wfh_state_size(h)) return WFH_ERR_ARG;
+    int32_t rc = wfh_sync(h);
+    if (rc) return rc;
+    int64_t* hdr = (int64_t*)buf;
+    hdr[0] = 0x5746484853544154LL;
+    hdr[1] = h->n;
+    hdr[2] = h->nriv;
+    hdr[3] = h->N;
+    char* p = (char*)buf + 4 * sizeof(int64_t);
+    for (auto& s : state_blocks(h)) {
+        if (s.second) HIPCHK(h, hipMemcpy(p, s.first, s.second * sizeof(double), hipMemcpyDeviceToHost));
+        p += s.second * sizeof(double);
+    }
+    return WFH_OK;
+}
+
+extern "C" int32_t wfh_load_state(wfh_handle* h, const void* buf, int64_t bytes) {
+    if (!h || !buf || bytes != wfh_state_size(h)) return WFH_ERR_ARG;
+    const int64_t* hdr = (const int64_t*)buf;
+    if (hdr[0] != 0x5746484853544154LL || hdr[1] != h->n || hdr[2] != h->nriv || hdr[3] != h->N)
+        return fail(h, WFH_ERR_ARG, "state buffer does not match this model");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const char* p = (const char*)buf + 4 * sizeof(int64_t);
+    for (auto& s : state_blocks(h)) {
+        if (s.second) HIPCHK(h, hipMemcpy(s.first, p, s.second * sizeof(double), hipMemcpyHostToDevice));
+        p += s.second * sizeof(double);
+    }
+    return WFH_OK;
+}
+
+extern "C" int32_t wfh_set_timing(wfh_handle* h, int32_t enable) {
+    if (!h) return WFH_ERR_ARG;
+    h->timing = enable != 0;
+    h->events_recorded = false;
+    return WFH_OK;
+}
+
+extern "C" int32_t wfh_get_timing(wfh_handle* h, double* vertical_ms, double* routing_ms) {
+    if (!h) return WFH_ERR_ARG;
+    if (!h->events_recorded) return fail(h, WFH_ERR_STATE, "no timed step recorded");
+    int32_t rc = wfh_sync(h);
+    if (rc) return rc;
+    float a = 0.f, b = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
+    HIPCHK(h, hipEventElapsedTime(&b, h->ev[1], h->ev[2]));
+    if (vertical_ms) *vertical_ms = a;
+    if (routing_ms) *routing_ms = b;
+    return WFH_OK;
+}
+
+extern "C" const char* wfh_last_error(const wfh_handle* h) {
+    if (!h) return g_create_error.c_str();
+    return h->err.c_str();
+}
+
+extern "C" void wfh_destroy(wfh_handle* h) {
+    if (!h) return;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (void* p : h->allocs) hipFree(p);
+    for (auto& ev : h->ev)
+        if (ev) hipEventDestroy(ev);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+}

@@ -1,0 +1,184 @@
+"""ctypes binding of libwflow_hip.so and the Model class that mirrors the reference step API."""
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import Var
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def library_path():
+    return os.path.join(os.path.dirname(_HERE), "lib", "libwflow_hip.so")
+
+
+def load_library():
+    """Load the HIP library; raises when it has not been built (no fallback path exists)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build()")
+    try:
+        # torch ships its own HIP runtime under the same soname; loading it first makes this
+        # library bind to that single copy, so device buffers can be shared with torch.
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    L = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    i32, i64 = ctypes.c_int32, ctypes.c_int64
+    L.wfh_create.argtypes = [ctypes.POINTER(_abi.WfhConfig), ctypes.POINTER(_abi.WfhStatic), ctypes.POINTER(P)]
+    L.wfh_create.restype = i32
+    L.wfh_set_forcing.argtypes = [P, P, P, P, i32]
+    L.wfh_set_forcing.restype = i32
+    L.wfh_step.argtypes = [P]
+    L.wfh_step.restype = i32
+    L.wfh_get.argtypes = [P, i32, P, i64, i32]
+    L.wfh_get.restype = i32
+    L.wfh_set.argtypes = [P, i32, P, i64, i32]
+    L.wfh_set.restype = i32
+    L.wfh_get_topology.argtypes = [P, P, P, P, P, P, P]
+    L.wfh_get_topology.restype = i32
+    L.wfh_get_node_order.argtypes = [P, P]
+    L.wfh_get_node_order.restype = i32
+    L.wfh_state_size.argtypes = [P]
+    L.wfh_state_size.restype = i64
+    L.wfh_save_state.argtypes = [P, P, i64]
+    L.wfh_save_state.restype = i32
+    L.wfh_load_state.argtypes = [P, P, i64]
+    L.wfh_load_state.restype = i32
+    L.wfh_sync.argtypes = [P]
+    L.wfh_sync.restype = i32
+    L.wfh_set_timing.argtypes = [P, i32]
+    L.wfh_set_timing.restype = i32
+    L.wfh_get_timing.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.wfh_get_timing.restype = i32
+    L.wfh_last_error.argtypes = [P]
+    L.wfh_last_error.restype = ctypes.c_char_p
+    L.wfh_destroy.argtypes = [P]
+    L.wfh_destroy.restype = None
+    _LIB = L
+    return L
+
+
+class WflowError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"wflow_hip error {code}: {msg}")
+        self.code = code
+
+
+class Model:
+    """One Wflow `sbm` model on one GPU (Model(config) for the sbm concept, sbm_model.jl:7-57)."""
+
+    def __init__(self, catchment, water_mass_balance=True, device=0, **cfg_kw):
+        self.lib = load_library()
+        self.catchment = catchment
+        self.cfg = catchment.config(water_mass_balance=water_mass_balance, device=device, **cfg_kw)
+        self._static = catchment.static()
+        h = ctypes.c_void_p()
+        rc = self.lib.wfh_create(ctypes.byref(self.cfg), ctypes.byref(self._static), ctypes.byref(h))
+        if rc != 0:
+            raise WflowError(rc, self.lib.wfh_last_error(None).decode())
+        self.h = h
+        self.n = catchment.n
+        self.nriv = catchment.nriv
+        self.nlayers = catchment.nlayers
+        self._forcing_keep = None
+
+    # ---- lifetime -------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.wfh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise WflowError(rc, self.lib.wfh_last_error(self.h).decode())
+
+    # ---- step ------------------------------------------------------------------------------
+    def set_forcing(self, precipitation, potential_evaporation, temperature, memkind=None):
+        """Forcing for the next step. numpy arrays -> host copy; torch CUDA tensors -> device
+        pointers (land order, or node order with memkind=WFH_MEM_DEVICE_NODE_ORDER)."""
+        arrs = (precipitation, potential_evaporation, temperature)
+        if isinstance(precipitation, np.ndarray):
+            arrs = tuple(np.ascontiguousarray(a, dtype=np.float64) for a in arrs)
+            ptrs = [a.ctypes.data for a in arrs]
+            kind = _abi.WFH_MEM_HOST if memkind is None else memkind
+        else:  # torch tensors on the GPU
+            for a in arrs:
+                assert a.is_cuda and a.dtype.itemsize == 8 and a.is_contiguous() and a.numel() == self.n
+            ptrs = [a.data_ptr() for a in arrs]
+            kind = _abi.WFH_MEM_DEVICE if memkind is None else memkind
+        self._forcing_keep = arrs
+        self._check(self.lib.wfh_set_forcing(self.h, *ptrs, kind))
+
+    def update_model(self):
+        """update_model! (asynchronous on the model's stream)."""
+        self._check(self.lib.wfh_step(self.h))
+
+    def run_timestep(self, precipitation, potential_evaporation, temperature):
+        """run_timestep! with the step's forcing supplied by the caller."""
+        self.set_forcing(precipitation, potential_evaporation, temperature)
+        self.update_model()
+
+    def sync(self):
+        self._check(self.lib.wfh_sync(self.h))
+
+    # ---- BMI-style access --------------------------------------------------------------------
+    def get_value(self, var):
+        cnt = _abi.var_count(var, self.n, self.nriv, self.nlayers)
+        out = np.empty(cnt, np.float64)
+        self._check(self.lib.wfh_get(self.h, int(var), out.ctypes.data, cnt, _abi.WFH_MEM_HOST))
+        return out
+
+    get = get_value
+
+    def set_value(self, var, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        self._check(self.lib.wfh_set(self.h, int(var), v.ctypes.data, v.size, _abi.WFH_MEM_HOST))
+
+    set = set_value
+
+    def topology(self):
+        n, nr = self.n, self.nriv
+        lo, ld = np.empty(n, np.int64), np.empty(n, np.int64)
+        ro, rd, rl = (np.empty(max(nr, 1), np.int64) for _ in range(3))
+        ml = ctypes.c_int32()
+        self._check(self.lib.wfh_get_topology(self.h, lo.ctypes.data, ld.ctypes.data, ro.ctypes.data,
+                                              rd.ctypes.data, rl.ctypes.data, ctypes.addressof(ml)))
+        return lo, ld, ro[:nr], rd[:nr], rl[:nr], ml.value
+
+    def node_order(self):
+        out = np.empty(self.n, np.int64)
+        self._check(self.lib.wfh_get_node_order(self.h, out.ctypes.data))
+        return out
+
+    def save_state(self):
+        size = self.lib.wfh_state_size(self.h)
+        buf = np.empty(size, np.uint8)
+        self._check(self.lib.wfh_save_state(self.h, buf.ctypes.data, size))
+        return buf
+
+    def load_state(self, buf):
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        self._check(self.lib.wfh_load_state(self.h, buf.ctypes.data, buf.size))
+
+    def set_timing(self, enable=True):
+        self._check(self.lib.wfh_set_timing(self.h, int(enable)))
+
+    def timing(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.wfh_get_timing(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value

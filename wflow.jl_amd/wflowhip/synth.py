@@ -1,0 +1,304 @@
+"""Synthetic catchments and forcing that stand in for the netCDF inputs of a Wflow `sbm` model.
+
+Workload specification (all values converted to the reference's internal SI units):
+
+* DEM: white noise shaped to a |k|^-2 power spectrum (amplitude |k|^-1, zero mean), scaled
+  affinely to 0-1500 m, depression-filled by Priority-Flood+epsilon, D8 steepest descent
+  (PCRaster LDD codes, first maximum in code order 1,2,3,4,6,7,8,9 wins, no positive drop ->
+  pit). All raster cells are active; cells are enumerated with x fastest.
+* River cells: upstream area (cells, itself included) > threshold. River width 2-50 m, linear
+  in log(upstream area); bankfull depth 0.27 * width^0.39 m; river length = D8 flow length;
+  river slope = land slope. Manning n 0.036 (river), 0.07 (land).
+* Land slope: gradient magnitude of the unfilled DEM (central differences).
+* Soil: KsatVer lognormal (median 250 mm/d, sigma 0.8), theta_s U[0.35, 0.5],
+  theta_r U[0.03, 0.1], f U[1e-4, 1e-3] mm^-1, soil thickness 2200 mm with layers
+  100/300/800 mm (+ remainder), Brooks-Corey c 9.43/9.82/10.24/10.24, KsatHorFrac 100,
+  reference defaults elsewhere (air entry -10 cm, Feddes h1..h4 = 0/-100/-400,-1000/-16000 cm,
+  alpha_h1 1, cap_hmax 2000 mm, cap_n 2, rootdistpar -500 mm^-1, w_soil 0.1125,
+  cf_soil 0.038, infiltration capacity of compacted soil 10 mm/d, no leakage),
+  compacted fraction 0.01, field capacity derived (Brooks-Corey at h = -1 m).
+* Vegetation: canopy gap fraction U[0.2, 0.8], canopy storage U[0.5, 2.5] mm, rooting depth
+  U[300, 1500] mm, crop factor 1, Gash E/R ratio 0.11.
+* Snow (HBV): degree-day 3.75 mm/degC/d, thresholds 0 degC, interval 1 degC, whc 0.1.
+* Forcing per step: precipitation gamma(shape 0.4, mean 3 mm/d) with a 40 % dry mask;
+  temperature 10 + 12 sin(2 pi (day - 105) / 365) + N(0, 3) - 6.5 degC/km * elevation,
+  hourly steps add 4 sin(2 pi (hour - 9) / 24); PET = clip(3 + 3 sin(seasonal) +
+  0.1 (T - 10), 0, 6) mm/d, hourly steps multiply by a half-sine daylight shape of mean 1.
+"""
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+
+_LIB = None
+MM = 1e-3
+MM_PER_DAY = (1.0 * (1.0 / 86400.0)) * 1e-3
+
+
+def _synth_lib():
+    global _LIB
+    if _LIB is None:
+        here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+        path = os.path.join(here, "lib", "libwfh_synth.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run __graft_entry__.build() first")
+        lib = ctypes.CDLL(path)
+        lib.synth_fill_depressions.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]
+        lib.synth_d8.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p]
+        lib.synth_accumulate.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                                         ctypes.c_void_p]
+        lib.synth_accumulate.restype = ctypes.c_int64
+        _LIB = lib
+    return _LIB
+
+
+def fractal_dem(nx, ny, seed, zmax=1500.0):
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((ny, nx))
+    F = np.fft.rfft2(w)
+    ky = np.fft.fftfreq(ny)[:, None]
+    kx = np.fft.rfftfreq(nx)[None, :]
+    k = np.sqrt(kx * kx + ky * ky)
+    k[0, 0] = 1.0
+    F = F / k
+    F[0, 0] = 0.0
+    z = np.fft.irfft2(F, s=(ny, nx))
+    z = (z - z.min()) / (z.max() - z.min()) * zmax
+    return np.ascontiguousarray(z)  # z[y, x]
+
+
+@dataclass
+class Catchment:
+    nx: int
+    ny: int
+    cell: float
+    dt: float
+    arrays: dict = field(default_factory=dict)  # static maps in SI, land order
+    nlayers_cfg: int = 3
+    layers_mm: tuple = (100.0, 300.0, 800.0)
+    snow: bool = True
+    infiltration_reduction: bool = True
+    max_depth: int = 0
+
+    @property
+    def n(self):
+        return int(self.arrays["cell_index"].shape[0])
+
+    @property
+    def nriv(self):
+        return int(self.arrays["river_slope"].shape[0])
+
+    @property
+    def nlayers(self):
+        return self.nlayers_cfg + 1
+
+    def config(self, water_mass_balance=True, device=0, land_dt=3600.0, river_dt=900.0, subsurface_dt=86400.0):
+        c = _abi.WfhConfig()
+        c.abi_version = _abi.WFH_ABI_VERSION
+        c.device = device
+        c.dt = self.dt
+        c.land_dt = land_dt
+        c.river_dt = river_dt
+        c.subsurface_dt = subsurface_dt
+        c.n_config_layers = self.nlayers_cfg
+        c.snow = int(self.snow)
+        for k, v in enumerate(self.layers_mm):
+            c.soil_layer_thickness_mm[k] = v
+        c.soil_infiltration_reduction = int(self.infiltration_reduction)
+        c.water_mass_balance = int(water_mass_balance)
+        return c
+
+    def static(self):
+        """wfh_static view of the arrays (the arrays stay owned by this object)."""
+        s = _abi.WfhStatic()
+        s.nx, s.ny, s.n, s.nriv = self.nx, self.ny, self.n, self.nriv
+        for name, ctype in _abi.STATIC_FIELDS[4:]:
+            a = self.arrays.get(name)
+            if a is None:
+                setattr(s, name, ctype())
+            else:
+                assert a.flags["C_CONTIGUOUS"], name
+                setattr(s, name, a.ctypes.data_as(ctype))
+        return s
+
+    def subset(self, mask):
+        """The same maps restricted to the cells where mask (land order) is true; used to run
+        complete basins on their own (basins exchange no water)."""
+        mask = np.asarray(mask, bool)
+        out = Catchment(self.nx, self.ny, self.cell, self.dt, {}, self.nlayers_cfg, self.layers_mm, self.snow,
+                        self.infiltration_reduction)
+        riv = self.arrays["river_location"].astype(bool)
+        rmask = mask[riv]
+        n = self.n
+        for k, a in self.arrays.items():
+            if k.startswith("river_") and k not in ("river_location", "river_width_land", "river_length_land"):
+                out.arrays[k] = np.ascontiguousarray(a[rmask])
+            elif a.ndim == 1 and a.shape[0] == n:
+                out.arrays[k] = np.ascontiguousarray(a[mask])
+            elif a.ndim == 1 and a.shape[0] % n == 0:  # layer-major [N][n]
+                out.arrays[k] = np.ascontiguousarray(a.reshape(-1, n)[:, mask].ravel())
+            else:
+                raise ValueError(k)
+        return out
+
+
+def make_catchment(nx, ny=None, cell=1000.0, dt=86400.0, river_threshold=100, seed=0, snow=True,
+                   infiltration_reduction=True):
+    ny = nx if ny is None else ny
+    lib = _synth_lib()
+    n = nx * ny
+    z = fractal_dem(nx, ny, seed)
+    zf = np.empty_like(z)
+    lib.synth_fill_depressions(z.ctypes.data, nx, ny, zf.ctypes.data)
+    ldd = np.empty(n, np.uint8)
+    lib.synth_d8(zf.ctypes.data, nx, ny, cell, ldd.ctypes.data)
+    area = np.empty(n, np.int64)
+    depth = np.empty(n, np.int32)
+    maxdepth = lib.synth_accumulate(ldd.ctypes.data, nx, ny, area.ctypes.data, depth.ctypes.data)
+    if maxdepth < 0:
+        raise RuntimeError("invalid synthetic LDD")
+    gy, gx = np.gradient(z, cell)
+    slope = np.sqrt(gx * gx + gy * gy).ravel()
+    rng = np.random.default_rng(seed + 7919)
+    A = {}
+    A["cell_index"] = np.arange(n, dtype=np.int64)
+    A["ldd"] = ldd
+    A["x_length"] = np.full(n, float(cell))
+    A["y_length"] = np.full(n, float(cell))
+    A["land_slope"] = np.ascontiguousarray(slope)
+    A["land_manning_n"] = np.full(n, 0.07)
+    riv = area > river_threshold
+    A["river_location"] = riv.astype(np.uint8)
+    lo, hi = math.log(river_threshold), math.log(max(area.max(), river_threshold + 1))
+    width = np.clip(2.0 + 48.0 * (np.log(np.maximum(area, 1)) - lo) / (hi - lo), 2.0, 50.0)
+    diag = np.isin(ldd, (1, 3, 7, 9))
+    flen = np.where(diag, cell * math.sqrt(2.0), float(cell))
+    A["river_width_land"] = np.where(riv, width, 0.0)
+    A["river_length_land"] = np.where(riv, flen, 0.0)
+    A["water_fraction"] = np.zeros(n)
+    A["elevation"] = np.ascontiguousarray(zf.ravel())
+    A["canopy_gap_fraction"] = rng.uniform(0.2, 0.8, n)
+    A["max_canopy_storage"] = rng.uniform(0.5, 2.5, n) * MM
+    A["rooting_depth"] = rng.uniform(300.0, 1500.0, n) * MM
+    A["crop_coefficient"] = np.ones(n)
+    A["evap_precip_ratio"] = np.full(n, 0.11)
+    A["cfmax"] = np.full(n, 3.75 * MM_PER_DAY)
+    A["tt"] = np.full(n, 273.15)
+    A["tti"] = np.full(n, 1.0)
+    A["ttm"] = np.full(n, 273.15)
+    A["whc"] = np.full(n, 0.1)
+    A["theta_s"] = rng.uniform(0.35, 0.5, n)
+    A["theta_r"] = rng.uniform(0.03, 0.1, n)
+    A["kv_0"] = 250.0 * np.exp(0.8 * rng.standard_normal(n)) * MM_PER_DAY
+    A["f"] = rng.uniform(1e-4, 1e-3, n) / MM
+    A["air_entry_pressure"] = np.full(n, -10.0 * 1e-2)
+    A["h1"] = np.full(n, 0.0)
+    A["h2"] = np.full(n, -100.0 * 1e-2)
+    A["h3_high"] = np.full(n, -400.0 * 1e-2)
+    A["h3_low"] = np.full(n, -1000.0 * 1e-2)
+    A["h4"] = np.full(n, -16000.0 * 1e-2)
+    A["alpha_h1"] = np.full(n, 1.0)
+    A["soil_thickness"] = np.full(n, 2200.0 * MM)
+    A["infiltcap_compacted"] = np.full(n, 10.0 * MM_PER_DAY)
+    A["max_leakage"] = np.zeros(n)
+    A["compacted_fraction"] = np.full(n, 0.01)
+    A["rootdistpar"] = np.full(n, -500.0 / MM)
+    A["cap_hmax"] = np.full(n, 2000.0 * MM)
+    A["cap_n"] = np.full(n, 2.0)
+    A["w_soil"] = np.full(n, 0.1125)
+    A["cf_soil"] = np.full(n, 0.038)
+    A["brooks_corey_c"] = np.ascontiguousarray(np.repeat([9.43, 9.82, 10.24, 10.24], n).astype(np.float64))
+    A["kvfrac"] = np.ones(4 * n)
+    A["khfrac"] = np.full(n, 100.0)
+    A["river_slope"] = np.ascontiguousarray(slope[riv])
+    A["river_manning_n"] = np.full(int(riv.sum()), 0.036)
+    A["river_bankfull_depth"] = np.ascontiguousarray(0.27 * width[riv] ** 0.39)
+    A["river_length"] = np.ascontiguousarray(flen[riv])
+    A["river_width"] = np.ascontiguousarray(width[riv])
+    c = Catchment(nx, ny, float(cell), float(dt), A, snow=snow, infiltration_reduction=infiltration_reduction)
+    c.max_depth = int(maxdepth)
+    c.upstream_area = area
+    return c
+
+
+def basin_ids(ldd, nx, ny):
+    """Outlet (pit) id of every cell: cells with the same id form one closed basin."""
+    n = nx * ny
+    dxs = np.array([0, -1, 0, 1, -1, 0, 1, -1, 0, 1])
+    dys = np.array([0, -1, -1, -1, 0, 0, 0, 1, 1, 1])
+    idx = np.arange(n)
+    down = np.where(ldd == 5, idx, (idx % nx + dxs[ldd]) + (idx // nx + dys[ldd]) * nx)
+    lab = down.copy()
+    for _ in range(64):  # pointer jumping: log2(path length) rounds
+        nxt = lab[lab]
+        if np.array_equal(nxt, lab):
+            break
+        lab = nxt
+    return lab
+
+
+# ------------------------------------------------------------------------------------------
+# forcing
+# ------------------------------------------------------------------------------------------
+
+def _daylight_shape():
+    h = np.arange(24) + 0.5
+    s = np.maximum(0.0, np.sin(np.pi * (h - 6.0) / 12.0))
+    return s / s.mean()
+
+
+_DAYLIGHT = _daylight_shape()
+
+
+def forcing_numpy(catchment, step, seed=0):
+    """(precipitation [m s-1], PET [m s-1], temperature [K]) for model step `step` (0-based)."""
+    n = catchment.n
+    dt = catchment.dt
+    rng = np.random.default_rng([seed, step, 12345])
+    t_days = step * dt / 86400.0
+    doy = t_days % 365.0
+    hour = (step * dt / 3600.0) % 24.0
+    season = math.sin(2.0 * math.pi * (doy - 105.0) / 365.0)
+    wet = rng.random(n) >= 0.4
+    p = rng.gamma(0.4, 3.0 / 0.4, n) * wet
+    elev = catchment.arrays["elevation"]
+    temp_c = 10.0 + 12.0 * season + rng.normal(0.0, 3.0, n) - 6.5e-3 * elev
+    if dt < 86400.0:
+        temp_c = temp_c + 4.0 * math.sin(2.0 * math.pi * (hour - 9.0) / 24.0)
+    pet = np.clip(3.0 + 3.0 * season + 0.1 * (temp_c - 10.0), 0.0, 6.0)
+    if dt < 86400.0:
+        pet = pet * _DAYLIGHT[int(hour) % 24]
+    return (np.ascontiguousarray(p * MM_PER_DAY), np.ascontiguousarray(pet * MM_PER_DAY),
+            np.ascontiguousarray(temp_c + 273.15))
+
+
+def forcing_torch(catchment, step, elevation_dev, seed=0, generator=None):
+    """Device-side variant of forcing_numpy (same distributions, torch RNG on the GPU)."""
+    import torch
+    n = catchment.n
+    dt = catchment.dt
+    dev = elevation_dev.device
+    g = generator
+    if g is None:
+        g = torch.Generator(device=dev)
+        g.manual_seed((seed * 1000003 + step) & 0x7FFFFFFF)
+    t_days = step * dt / 86400.0
+    doy = t_days % 365.0
+    hour = (step * dt / 3600.0) % 24.0
+    season = math.sin(2.0 * math.pi * (doy - 105.0) / 365.0)
+    wet = (torch.rand(n, generator=g, device=dev, dtype=torch.float64) >= 0.4).to(torch.float64)
+    conc = torch.full((n,), 0.4, device=dev, dtype=torch.float64)
+    p = torch._standard_gamma(conc, generator=g) * (3.0 / 0.4) * wet
+    noise = torch.randn(n, generator=g, device=dev, dtype=torch.float64) * 3.0
+    temp_c = 10.0 + 12.0 * season + noise - 6.5e-3 * elevation_dev
+    if dt < 86400.0:
+        temp_c = temp_c + 4.0 * math.sin(2.0 * math.pi * (hour - 9.0) / 24.0)
+    pet = torch.clamp(3.0 + 3.0 * season + 0.1 * (temp_c - 10.0), 0.0, 6.0)
+    if dt < 86400.0:
+        pet = pet * float(_DAYLIGHT[int(hour) % 24])
+    return (p * MM_PER_DAY).contiguous(), (pet * MM_PER_DAY).contiguous(), (temp_c + 273.15).contiguous()
