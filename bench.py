@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Throughput of the Wflow.jl `sbm` time step (update_model!) on one MI355X.
+
+Workload: the configs[1] generator - 4096 x 4096 cells (16.8 M), dx 250 m, hourly steps,
+river cells where the upstream area exceeds 2000 cells, FP64 states. A "step" is one model
+time step of the whole grid: the vertical column kernel plus the routing kernel (lateral
+subsurface flow, overland and river kinematic waves with their fixed sub-steps, storage and
+water-balance updates). Inputs are resident in HBM before timing starts: the forcing for the
+timed steps is generated on the GPU beforehand (a ring of distinct hourly frames, in the
+library's node order) and bound without copies.
+
+Output: one JSON line with the metric, the roofline of the dominant kernel (algorithmic bytes
+per launch from the library / mean HIP-event duration of that kernel over the timed steps) and
+a CPU baseline (the FP64 oracle timed on this host over a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "wflow.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nx", type=int, default=4096)
+    ap.add_argument("--frames", type=int, default=24, help="distinct forcing frames kept in HBM")
+    ap.add_argument("--cpu-sample-cells", type=int, default=400_000)
+    ap.add_argument("--cpu-sample-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_routing_4096.json"))
+    return ap.parse_args()
+
+
+def cpu_baseline(catchment, frames_host, args):
+    """The oracle (single thread) on complete basins of the same catchment; basins exchange no
+    water, so a set of whole basins is an exact sub-problem."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from wflowhip import synth
+
+    lab = synth.basin_ids(catchment.arrays["ldd"], catchment.nx, catchment.ny)
+    ids, counts = np.unique(lab, return_counts=True)
+    rng = np.random.default_rng(5)
+    order = rng.permutation(len(ids))
+    take, total = [], 0
+    for k in order:
+        if total + counts[k] > args.cpu_sample_cells and total > 0:
+            continue
+        take.append(ids[k])
+        total += counts[k]
+        if total >= 0.9 * args.cpu_sample_cells:
+            break
+    mask = np.isin(lab, np.array(take))
+    sub = catchment.subset(mask)
+    m = O.OracleModel(sub)
+    t = 0.0
+    for k in range(args.cpu_sample_steps):
+        P, E, T = (f[mask] for f in frames_host[k % len(frames_host)])
+        t0 = time.perf_counter()
+        m.step(P, E, T)
+        t += time.perf_counter() - t0
+    m.close()
+    return {
+        "value": sub.n * args.cpu_sample_steps / t,
+        "unit": "cell-timesteps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{len(take)} complete basins ({sub.n} cells) of the same 4096^2 hourly catchment, "
+                  f"{args.cpu_sample_steps} steps, single-threaded C oracle (-O2), update_model! only",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(sharding design in DESIGN.md)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from wflowhip import Model, Var, _abi, synth
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    t0 = time.time()
+    c = synth.make_catchment(args.nx, cell=250.0, dt=3600.0, river_threshold=2000, seed=1)
+    print(f"# catchment {args.nx}^2: n={c.n} nriv={c.nriv} longest path={c.max_depth} "
+          f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    m = Model(c, water_mass_balance=True)
+    print(f"# model created ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    node = m.node_order()
+    node_t = torch.as_tensor(node, device="cuda")
+    elev = torch.as_tensor(c.arrays["elevation"], device="cuda")[node_t].contiguous()
+    nframes = max(1, min(args.frames, args.steps + args.warmup))
+    frames = [synth.forcing_torch(c, k, elev) for k in range(nframes)]
+    torch.cuda.synchronize()
+    print(f"# {nframes} forcing frames on the GPU ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+
+    def step(k):
+        m.set_forcing(*frames[k % nframes], memkind=_abi.WFH_MEM_DEVICE_NODE_ORDER)
+        m.update_model()
+
+    for k in range(args.warmup):
+        step(k)
+    m.sync()
+    torch.cuda.synchronize()
+    m.set_timing(True)
+    t1 = time.perf_counter()
+    for k in range(args.warmup, args.warmup + args.steps):
+        step(k)
+    m.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t1
+    k1_ms, k2_ms = m.timing()
+    k1_bytes, k2_bytes = m.kernel_bytes()
+    bal = float(np.abs(m.get(Var.BALANCE_LAND_ERROR)).max())
+    print(f"# {args.steps} steps in {elapsed:.3f}s: vertical {k1_ms:.3f} ms, routing {k2_ms:.3f} ms per step; "
+          f"max |land balance error| {bal:.2e}", file=sys.stderr, flush=True)
+
+    if k2_ms >= k1_ms:
+        dom, dom_ms, dom_bytes = "k_route", k2_ms, k2_bytes
+    else:
+        dom, dom_ms, dom_bytes = "k_vertical", k1_ms, k1_bytes
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(args.traffic_file):
+        try:
+            tf = json.load(open(args.traffic_file))
+            if tf.get("kernel") == dom and tf.get("n") == c.n:
+                traffic = tf.get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    result = {
+        "metric": "cell-timesteps/s FP64 SBM+kinwave (plain GPU step, configs[1] workload)",
+        "value": c.n * args.steps / elapsed,
+        "unit": "cell-timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (fractal DEM, seeded parameters, GPU-generated forcing)",
+        "config": {"workload": f"configs[1]: {args.nx}x{args.nx} cells, dx 250 m, hourly, river threshold 2000, "
+                               "sbm + kinematic wave (land 1, river 4, subsurface 1 sub-steps)",
+                   "cells": c.n, "river_cells": c.nriv, "longest_flow_path": c.max_depth,
+                   "kernel_ms": {"k_vertical": k1_ms, "k_route": k2_ms},
+                   "kernel_bytes": {"k_vertical": k1_bytes, "k_route": k2_bytes}},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+    }
+    if not args.no_cpu_baseline:
+        frames_host = []
+        inv = np.empty_like(node)
+        inv[node] = np.arange(node.size)
+        for k in range(min(args.cpu_sample_steps, nframes)):
+            frames_host.append(tuple(f.cpu().numpy()[inv] for f in frames[k]))
+        result["cpu_baseline"] = cpu_baseline(c, frames_host, args)
+    else:
+        result["cpu_baseline"] = None
+    m.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

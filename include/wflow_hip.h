@@ -238,10 +238,16 @@ int32_t wfh_save_state(wfh_handle* h, void* buf, int64_t bytes);
 int32_t wfh_load_state(wfh_handle* h, const void* buf, int64_t bytes);
 
 int32_t wfh_sync(wfh_handle* h);
-/* Kernel timing of the last step (ms per kernel phase measured with HIP events on the handle
- * stream); only filled when timing was enabled with wfh_set_timing(h, 1). */
+/* Kernel timing: with wfh_set_timing(h, 1) every following step records HIP events on the
+ * handle's stream around each kernel; wfh_get_timing() returns the mean duration (ms) of the
+ * vertical and the routing kernel over the steps recorded since timing was enabled. */
 int32_t wfh_set_timing(wfh_handle* h, int32_t enable);
 int32_t wfh_get_timing(wfh_handle* h, double* vertical_ms, double* routing_ms);
+/* Algorithmic HBM bytes of one launch of each kernel for this model: every input array read
+ * once and every output array written once (parameter maps stored once when uniform count
+ * 8 bytes), upstream discharge read once per edge and sub-step. No counterpart in the
+ * reference; used for the roofline of the measurement harness. */
+int32_t wfh_kernel_bytes(wfh_handle* h, int64_t* vertical_bytes, int64_t* routing_bytes);
 const char* wfh_last_error(const wfh_handle* h);
 void wfh_destroy(wfh_handle* h);
 

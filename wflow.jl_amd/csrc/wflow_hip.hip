@@ -786,8 +786,9 @@ struct wfh_handle {
     int S_s = 0, S_l = 0, S_r = 0;
     double dts_s[2]{}, dts_l[2]{}, dts_r[2]{};
     hipStream_t stream = nullptr;
-    hipEvent_t ev[3]{};
-    bool timing = false, events_recorded = false;
+    std::vector<hipEvent_t> ev;  // 3 events per timed step: start, after K1, after K2
+    size_t timed_steps = 0;
+    bool timing = false;
     bool forcing_set = false, broken = false;
     int parity = 0;
     int route_grid = 0;
@@ -1436,8 +1437,6 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         h->nchunks = (int)((n + WAVE - 1) / WAVE);
         A.nchunks = h->nchunks;
         if (hipStreamCreate(&h->stream) != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipStreamCreate"); goto done; }
-        for (auto& ev : h->ev)
-            if (hipEventCreate(&ev) != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipEventCreate"); goto done; }
         if ((rc = launch_fill(h, A.slot_ssf, SLOT_EMPTY, (size_t)h->S_s * n))) goto done;
         if ((rc = launch_fill(h, A.slot_ol, SLOT_EMPTY, (size_t)h->S_l * n))) goto done;
         if ((rc = launch_fill(h, A.slot_r, SLOT_EMPTY, (size_t)h->S_r * std::max<int64_t>(nriv, 1)))) goto done;
@@ -1472,20 +1471,31 @@ done:
     return rc;
 }
 
+constexpr size_t MAX_TIMED_STEPS = 100000;
+
+static hipEvent_t* timing_events(wfh_handle* h) {
+    if (!h->timing || h->timed_steps >= MAX_TIMED_STEPS) return nullptr;
+    const size_t need = 3 * (h->timed_steps + 1);
+    while (h->ev.size() < need) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        h->ev.push_back(e);
+    }
+    return &h->ev[3 * h->timed_steps++];
+}
+
 template <int N>
 static void launch_kernels(wfh_handle* h) {
     const int n = (int)h->n;
-    if (h->timing) hipEventRecord(h->ev[0], h->stream);
+    hipEvent_t* ev = timing_events(h);
+    if (ev) hipEventRecord(ev[0], h->stream);
     hipLaunchKernelGGL(k_vertical<N>, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d);
-    if (h->timing) hipEventRecord(h->ev[1], h->stream);
+    if (ev) hipEventRecord(ev[1], h->stream);
     RouteArgs a = h->ra;
     a.parity = h->parity;
     h->parity ^= 1;
     hipLaunchKernelGGL(k_route<N>, dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, a);
-    if (h->timing) {
-        hipEventRecord(h->ev[2], h->stream);
-        h->events_recorded = true;
-    }
+    if (ev) hipEventRecord(ev[2], h->stream);
 }
 
 extern "C" int32_t wfh_set_forcing(wfh_handle* h, const double* P, const double* E, const double* T,
@@ -1777,20 +1787,25 @@ extern "C" int32_t wfh_load_state(wfh_handle* h, const void* buf, int64_t bytes)
 extern "C" int32_t wfh_set_timing(wfh_handle* h, int32_t enable) {
     if (!h) return WFH_ERR_ARG;
     h->timing = enable != 0;
-    h->events_recorded = false;
+    h->timed_steps = 0;
     return WFH_OK;
 }
 
 extern "C" int32_t wfh_get_timing(wfh_handle* h, double* vertical_ms, double* routing_ms) {
     if (!h) return WFH_ERR_ARG;
-    if (!h->events_recorded) return fail(h, WFH_ERR_STATE, "no timed step recorded");
+    if (h->timed_steps == 0) return fail(h, WFH_ERR_STATE, "no timed step recorded");
     int32_t rc = wfh_sync(h);
     if (rc) return rc;
-    float a = 0.f, b = 0.f;
-    HIPCHK(h, hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
-    HIPCHK(h, hipEventElapsedTime(&b, h->ev[1], h->ev[2]));
-    if (vertical_ms) *vertical_ms = a;
-    if (routing_ms) *routing_ms = b;
+    double sa = 0.0, sb = 0.0;
+    for (size_t k = 0; k < h->timed_steps; ++k) {
+        float a = 0.f, b = 0.f;
+        HIPCHK(h, hipEventElapsedTime(&a, h->ev[3 * k], h->ev[3 * k + 1]));
+        HIPCHK(h, hipEventElapsedTime(&b, h->ev[3 * k + 1], h->ev[3 * k + 2]));
+        sa += a;
+        sb += b;
+    }
+    if (vertical_ms) *vertical_ms = sa / h->timed_steps;
+    if (routing_ms) *routing_ms = sb / h->timed_steps;
     return WFH_OK;
 }
 
@@ -1803,8 +1818,55 @@ extern "C" void wfh_destroy(wfh_handle* h) {
     if (!h) return;
     if (h->stream) hipStreamSynchronize(h->stream);
     for (void* p : h->allocs) hipFree(p);
-    for (auto& ev : h->ev)
+    for (hipEvent_t ev : h->ev)
         if (ev) hipEventDestroy(ev);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
+}
+
+extern "C" int32_t wfh_kernel_bytes(wfh_handle* h, int64_t* vertical_bytes, int64_t* routing_bytes) {
+    if (!h) return WFH_ERR_ARG;
+    const Dev& D = h->d;
+    const int64_t n = h->n, nr = h->nriv, N = h->N;
+    const int64_t E = n - [&] {  // edges = cells with a downstream
+        int64_t pits = 0;
+        for (int64_t v : h->land_down) pits += v < 0;
+        return pits;
+    }();
+    int64_t Er = 0;
+    for (int64_t v : h->river_down) Er += v >= 0;
+    auto P = [&](const Prm& p) { return p.mask ? 8 * n : 8; };
+    auto PL = [&](const PrmL& p) { return p.mask ? 8 * n * N : 8 * N; };
+    const int64_t A = 8 * n;
+    // ---- K1 ----
+    int64_t k1 = 3 * A;                                   // forcing
+    k1 += A * (2 + N) + A + A;                            // canopy, tsoil, swd, ustore[N], ol_h (read)
+    k1 += 4 * n + 8 * nr;                                 // river_of, r_h
+    if (D.snow) k1 += 2 * A;                              // snow_s, snow_w
+    if (D.balance) k1 += A;                               // tsws
+    k1 += P(D.gap) + P(D.smax) + P(D.kc) + P(D.rf) + P(D.wf) + P(D.sfrac) + P(D.ts) + P(D.tr) + P(D.tfc) +
+          P(D.d) + P(D.kv0) + P(D.f) + P(D.hb) + P(D.h1) + P(D.h2) + P(D.h3h) + P(D.h3l) + P(D.h4) + P(D.ah1) +
+          P(D.icap_s) + P(D.icap_c) + P(D.compf) + P(D.rd) + P(D.rdp) + P(D.caph) + P(D.capn) + P(D.maxleak) +
+          PL(D.c) + PL(D.kvfrac) + PL(D.rootfrac);
+    if (D.gash) k1 += P(D.er);
+    if (D.snow) k1 += P(D.cfmax) + P(D.tt) + P(D.tti) + P(D.ttm) + P(D.whc) + P(D.wsoil) + P(D.cfsoil);
+    k1 += A * (2 + N) + A * 16;                           // canopy, tsoil, ustore[N], 16 fluxes
+    if (D.snow) k1 += 3 * A;                              // snow_s, snow_w, swe
+    if (D.balance) k1 += A;                               // store_prev
+    // ---- K2 ----
+    int64_t k2 = A * (N + 1) + A * 4 + A * 5;             // ustore[N], zi, ssf_q, recharge, ol q/h/storage; 5 K1 fluxes
+    k2 += A * 2;                                          // canopy, snow (total storage; 1 word without snow)
+    if (D.balance) k2 += 4 * A;                           // P, aet, leakage, store_prev
+    k2 += 4 * (n + 1) + 4 * E + 8 * E + n + 4 * n;        // up_ptr, up_idx, up_f, flags, river_of
+    k2 += P(D.d) + P(D.ts) + P(D.tr) + P(D.rd) + P(D.slope) + P(D.sy) + P(D.fl) + P(D.fw) + P(D.qmax) +
+          P(D.kh0) + P(D.f) + P(D.area) + P(D.alpha) + P(D.sfw) + P(D.rf) + P(D.tfc);
+    k2 += 8 * E * (h->S_s + h->S_l) + 8 * Er * h->S_r;    // upstream discharge per edge and sub-step
+    k2 += 8 * nr * 7 + 4 * (nr + 1) + 4 * Er;             // river states + static, river CSR
+    k2 += A * 14 + A * 2 * N;                             // subsurface + soil outputs, ustore[N], vwc[N]
+    k2 += A * 9;                                          // overland outputs + total storage
+    k2 += 8 * nr * 7;                                     // river outputs
+    if (D.balance) k2 += 4 * A + 8 * nr;                  // balance errors
+    if (vertical_bytes) *vertical_bytes = k1;
+    if (routing_bytes) *routing_bytes = k2;
+    return WFH_OK;
 }

@@ -63,6 +63,8 @@ def load_library():
     L.wfh_last_error.restype = ctypes.c_char_p
     L.wfh_destroy.argtypes = [P]
     L.wfh_destroy.restype = None
+    L.wfh_kernel_bytes.argtypes = [P, P, P]
+    L.wfh_kernel_bytes.restype = i32
     _LIB = L
     return L
 
@@ -177,6 +179,12 @@ class Model:
 
     def set_timing(self, enable=True):
         self._check(self.lib.wfh_set_timing(self.h, int(enable)))
+
+    def kernel_bytes(self):
+        """Algorithmic bytes of one launch of (vertical kernel, routing kernel)."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.wfh_kernel_bytes(self.h, ctypes.addressof(a), ctypes.addressof(b)))
+        return a.value, b.value
 
     def timing(self):
         a, b = ctypes.c_double(), ctypes.c_double()
