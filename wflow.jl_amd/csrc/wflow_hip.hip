@@ -78,6 +78,8 @@ struct Dev {
     const uint8_t* flags;  // bit0: has a land downstream, bit1: river cell, bit2: has a river downstream
     const int* rup_ptr;  // [nriv+1]
     const int* rup_idx;  // [Er] river ids
+    const int* rup_node; // [Er] node ids of the upstream river cells
+    const int* down;     // [n] downstream node id or -1
     // river static (river order)
     const double *r_alpha, *r_fw, *r_fl, *r_area;
     // ---- land states ----
@@ -395,149 +397,199 @@ __device__ __forceinline__ bool slot_take(uint64_t* s, double& v) {
     return true;
 }
 
-enum Phase : int { PH_SSF = 0, PH_STORE = 1, PH_OL = 2, PH_RIVER = 3, PH_FINAL = 4, PH_DONE = 5 };
+// Folds of the upstream discharge in the reference's left-fold order (sum_at, utils.jl:472-477).
+//   FOLD_SSF: acc1 = sum q (1 - f)  (q_in),   acc2 = sum q f  (to_river)   lateral_subsurface_flow.jl:228-240
+//   FOLD_OL:  acc1 = sum q f (to_river),      acc2 = sum q (1 - f) (qin)    surface_kinwave.jl:317-326
+//   FOLD_RIV: acc1 = sum q                                                  surface_kinwave.jl:520
+enum { FOLD_SSF = 0, FOLD_OL = 1, FOLD_RIV = 2 };
 
-template <int N>
-struct Lane {
-    int p, r, phase, sub, e, e_end;
-    double acc1, acc2;  // running upstream folds (kept in the reference's left-fold order)
-    double qnb;         // q_net_bnds of the current subsurface sub-step
-    // subsurface
-    double sq, szi, s_qin_cum, s_q_cum, s_exf_cum, s_qnet_cum, s_tor_cum, rech_cum;
-    Column<N> col;
-    // overland
-    double oq, oh, oqlat, o_q_cum, o_qin_cum, o_tor_cum, oqin, o_inwater, o_store_prev;
-    // river
-    double rq, rh, rqlat, r_q_cum, r_qin_cum, rqin, r_inwater, r_store_prev, r_storage;
-    double o_storage;
-};
-
-template <int N>
-__device__ void lane_init(const Dev& d, Lane<N>& L) {
-    const int p = L.p, n = d.n;
-    L.r = d.river_of[p];
-    L.phase = PH_SSF;
-    L.sub = 0;
-    L.e = d.up_ptr[p];
-    L.e_end = d.up_ptr[p + 1];
-    L.acc1 = 0.0;
-    L.acc2 = 0.0;
-    const double dsoil = ld(d.d, p);
-    const double ts = ld(d.ts, p), tr = ld(d.tr, p), tfc = ld(d.tfc, p);
-    soil_layers<N>(dsoil, d.cfg_cum, d.cfg_thick, L.col.act, L.col.cum);
-#pragma unroll
-    for (int k = 0; k < N; ++k) L.col.ustore[k] = d.ustore[k * n + p];
-    L.col.zi = d.zi[p];
-    L.col.n_unsat = set_layerthickness<N>(L.col.zi, L.col.cum, L.col.act, L.col.thick);
-    L.col.theta_e = ts - tr;
-    L.col.theta_fc_r = tfc - tr;
-    L.sq = d.ssf_q[p];
-    L.szi = L.col.zi;  // routing.subsurface_flow.variables.water_table_depth .= soil's (sbm_model.jl:80)
-    L.s_qin_cum = L.s_q_cum = L.s_exf_cum = L.s_qnet_cum = L.s_tor_cum = L.rech_cum = 0.0;
-    L.oq = d.ol_q[p];
-    L.oh = d.ol_h[p];
-    L.o_store_prev = d.ol_storage[p];
-    if (L.r >= 0) {
-        L.rq = d.r_q[L.r];
-        L.rh = d.r_h[L.r];
-        L.r_store_prev = d.r_storage[L.r];
+template <int FOLD>
+__device__ __forceinline__ void fold_in(double& acc1, double& acc2, double q, double f) {
+    if (FOLD == FOLD_SSF) {
+        acc1 = acc1 + q * (1.0 - f);
+        acc2 = acc2 + q * f;
+    } else if (FOLD == FOLD_OL) {
+        acc1 = acc1 + q * f;
+        acc2 = acc2 + q * (1.0 - f);
+    } else {
+        acc1 = acc1 + q;
     }
 }
 
-// recharge boundary flux for one subsurface sub-step: flux!(::RechargeModel)
-// Wflow/src/routing/subsurface/boundary_conditions.jl:219-236, check_flux :12-21
-template <int N>
-__device__ __forceinline__ void ssf_begin(const Dev& d, Lane<N>& L, double dts) {
-    double fl = d.recharge[L.p] * ld(d.area, L.p);
-    if (L.szi >= ld(d.d, L.p)) fl = wfh_max(0.0, fl);
-    L.rech_cum += fl * dts;
-    L.qnb = 0.0 + fl;
+// One family of routing sub-steps (all subsurface, all overland or all river sub-steps of the
+// model step) for the 64 cells of a wave. Every lane runs the same code; the wave advances a
+// diagonal wavefront over (cell, sub-step): a lane computes sub-step k once all upstream cells
+// have produced their sub-step-k outflow and its in-wave downstream cell has consumed its
+// sub-step k-1 outflow. Outflow to an in-wave consumer stays in a register and travels by lane
+// shuffle (a producer holding sub-step k sits at k+1 until the consumer moves on); outflow to
+// a consumer in another wave goes through its global slot. Returns false when the wave has to
+// stop (a wait exceeded its limit, or another wave raised the abort flag).
+template <int FOLD, class Compute>
+__device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, const int c, const bool part,
+                                           const int S, const int* __restrict__ up_node,
+                                           const int* __restrict__ up_slot, const double* __restrict__ up_f,
+                                           const int e0, const int e1, uint64_t* slots, const size_t stride,
+                                           const int my_slot, const bool pub_cross, const int cons_lane,
+                                           Compute& compute) {
+    int k = part ? 0 : S;
+    int e = e0;
+    double acc1 = 0.0, acc2 = 0.0, qv = 0.0;
+    long spin = 0;
+    while (__any(k < S)) {
+        const bool act = k < S;
+        bool took_any = false;
+        for (int pass = 0; pass < 9; ++pass) {
+            int src = lane;
+            bool inw = false, pend = act && e < e1;
+            int u = 0;
+            if (pend) {
+                u = up_node[e];
+                inw = (u >> 6) == c;
+                if (inw) src = u & 63;
+            }
+            const int ku = __shfl(k, src);
+            const double qu = __shfl(qv, src);
+            bool took = false;
+            if (pend) {
+                double q = 0.0;
+                if (inw) {
+                    if (ku == k + 1) {
+                        q = qu;
+                        took = true;
+                    }
+                } else {
+                    took = slot_take(slots + (size_t)k * stride + up_slot[e], q);
+                }
+                if (took) {
+                    fold_in<FOLD>(acc1, acc2, q, FOLD == FOLD_RIV ? 0.0 : up_f[e]);
+                    ++e;
+                }
+            }
+            if (!__any(took)) break;
+            took_any = true;
+        }
+        const int kc = __shfl(k, cons_lane >= 0 ? cons_lane : lane);
+        const bool can = act && e == e1 && (cons_lane < 0 || kc >= k);
+        if (can) {
+            qv = compute(k, acc1, acc2);
+            if (pub_cross) slot_put(slots + (size_t)k * stride + my_slot, qv);
+            ++k;
+            e = e0;
+            acc1 = 0.0;
+            acc2 = 0.0;
+        }
+        if (__any(can) || took_any) {
+            spin = 0;
+        } else {
+            __builtin_amdgcn_s_sleep(1);
+            ++spin;
+            if ((spin & 255) == 0 && __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                return false;
+            if (spin > SPIN_LIMIT) {
+                __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return false;
+            }
+        }
+    }
+    return true;
 }
 
-// Try to advance one lane through as many phases as its inputs allow. Returns true on progress.
 template <int N>
-__device__ bool lane_advance(const Dev& d, const RouteArgs& a, Lane<N>& L) {
-    const int p = L.p, n = d.n;
+__global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[a.parity ^ 1] = 0ull;  // next launch's counter
+    const int n = d.n;
     const double dt = d.dt;
-    bool progressed = false;
     for (;;) {
-        if (L.phase == PH_SSF) {
-            // ---- kinwave_subsurface_update!  lateral_subsurface_flow.jl:198-273 ----
-            const int k = L.sub;
-            uint64_t* slots = a.slot_ssf + (size_t)k * n;
-            while (L.e < L.e_end) {
-                double qu;
-                if (!slot_take(slots + d.up_idx[L.e], qu)) return progressed;
-                const double fu = d.up_f[L.e];
-                L.acc1 = L.acc1 + qu * (1.0 - fu);
-                L.acc2 = L.acc2 + qu * fu;
-                ++L.e;
-            }
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(&a.counters[a.parity], 1ull);
+        t = __shfl(t, 0);
+        if (t >= (unsigned long long)a.nchunks) break;
+        if (__hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        const int c = (int)t;
+        const int p = c * WAVE + lane;
+        const bool valid = p < n;
+        const int q_ = valid ? p : 0;  // clamped index for loads of inactive lanes
+        const int dn = valid ? d.down[p] : -1;
+        const bool cons_in = dn >= 0 && (dn >> 6) == c;
+        const int e0 = d.up_ptr[q_], e1 = valid ? d.up_ptr[q_ + 1] : e0;
+        const uint8_t fl = valid ? d.flags[p] : 0;
+        const int r = valid ? d.river_of[p] : -1;
+
+        // ---------------- lateral subsurface flow ----------------
+        const double area = ld(d.area, q_);
+        const double dsoil = ld(d.d, q_), ts = ld(d.ts, q_), tr = ld(d.tr, q_), tfc = ld(d.tfc, q_);
+        const double sy = ld(d.sy, q_), slope = ld(d.slope, q_), flen = ld(d.fl, q_), fw = ld(d.fw, q_);
+        const double qmax = ld(d.qmax, q_), kh0 = ld(d.kh0, q_), fk = ld(d.f, q_);
+        const double rech = d.recharge[q_];
+        Column<N> col;
+        soil_layers<N>(dsoil, d.cfg_cum, d.cfg_thick, col.act, col.cum);
+#pragma unroll
+        for (int k = 0; k < N; ++k) col.ustore[k] = d.ustore[k * n + q_];
+        col.zi = d.zi[q_];
+        col.n_unsat = set_layerthickness<N>(col.zi, col.cum, col.act, col.thick);
+        col.theta_e = ts - tr;
+        col.theta_fc_r = tfc - tr;
+        double sq = d.ssf_q[q_], szi = col.zi;  // sbm_model.jl:80 hands the soil water table over
+        double s_qin_cum = 0.0, s_q_cum = 0.0, s_exf_cum = 0.0, s_qnet_cum = 0.0, s_tor_cum = 0.0, rech_cum = 0.0;
+        auto ssf_compute = [&](int k, double qin, double tor) -> double {
             const double dts = (k == a.S_s - 1) ? a.dts_s[1] : a.dts_s[0];
-            const double qin = L.acc1;
-            L.s_tor_cum += L.acc2 * dts;
+            // flux!(::RechargeModel) with check_flux  boundary_conditions.jl:12-21,219-236
+            double flx = rech * area;
+            if (szi >= dsoil) flx = wfh_max(0.0, flx);
+            rech_cum += flx * dts;
+            const double qnb = 0.0 + flx;
+            s_tor_cum += tor * dts;
             double qn, zin, exf, net;
-            kinematic_wave_ssf<N>(qin, L.sq, L.szi, L.qnb, ld(d.slope, p), ld(d.sy, p), ld(d.d, p), dts, ld(d.fl, p),
-                                  ld(d.fw, p), ld(d.qmax, p), ld(d.kh0, p), ld(d.f, p), L.col, qn, zin, exf, net);
-            L.sq = qn;
-            L.szi = zin;
-            L.s_qin_cum += qin * dts;
-            L.s_q_cum += qn * dts;
-            L.s_exf_cum += exf * dts;
-            L.s_qnet_cum += net * ld(d.area, p) * dts;
-            if (d.flags[p] & 1) slot_put(slots + p, qn);
-            L.e = d.up_ptr[p];
-            L.acc1 = 0.0;
-            L.acc2 = 0.0;
-            progressed = true;
-            if (k + 1 < a.S_s) {
-                L.sub = k + 1;
-                ssf_begin<N>(d, L, (k + 1 == a.S_s - 1) ? a.dts_s[1] : a.dts_s[0]);
-            } else {
-                L.phase = PH_STORE;
-            }
-        } else if (L.phase == PH_STORE) {
-            // ---- average_flux_vars!, flux_to_river!  groundwater.jl:630-636, lateral_subsurface_flow.jl:182-196
-            const double s_exf_avg = L.s_exf_cum / dt;
-            const double s_qin_avg = L.s_qin_cum / dt;
-            const double s_q_avg = L.s_q_cum / dt;
-            const double s_qnet_avg = L.s_qnet_cum / dt;
-            const double rech_avg = L.rech_cum / dt;
-            const double s_tor_avg = L.s_tor_cum / dt;
-            d.ssf_q[p] = L.sq;
+            kinematic_wave_ssf<N>(qin, sq, szi, qnb, slope, sy, dsoil, dts, flen, fw, qmax, kh0, fk, col, qn, zin,
+                                  exf, net);
+            sq = qn;
+            szi = zin;
+            s_qin_cum += qin * dts;
+            s_q_cum += qn * dts;
+            s_exf_cum += exf * dts;
+            s_qnet_cum += net * area * dts;
+            return qn;
+        };
+        if (!run_family<FOLD_SSF>(a, lane, c, valid, a.S_s, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ssf,
+                                  (size_t)n, p, (fl & 1) && !cons_in, cons_in ? (dn & 63) : -1, ssf_compute))
+            return;
+
+        // ---------------- average_flux_vars!, update_soil_water_storage! ----------------
+        double net_runoff = 0.0, o_inwater = 0.0, s_tor_avg = 0.0;
+        double usd = 0.0, swd = 0.0;
+        if (valid) {
+            const double s_exf_avg = s_exf_cum / dt;
+            const double s_qin_avg = s_qin_cum / dt;
+            const double s_q_avg = s_q_cum / dt;
+            const double s_qnet_avg = s_qnet_cum / dt;
+            const double rech_avg = rech_cum / dt;
+            s_tor_avg = s_tor_cum / dt;
+            d.ssf_q[p] = sq;
             d.ssf_qin_avg[p] = s_qin_avg;
             d.ssf_q_avg[p] = s_q_avg;
             d.ssf_tor_avg[p] = s_tor_avg;
-            const double area = ld(d.area, p);
-            const double sy = ld(d.sy, p), dsoil = ld(d.d, p);
-            d.ssf_storage[p] = sy * (dsoil - L.szi) * area;
-            // ---- update_soil_water_storage!  soil.jl:1294-1392 ----
-            const double ts = ld(d.ts, p), tr = ld(d.tr, p), tfc = ld(d.tfc, p), rd = ld(d.rd, p);
-            const Column<N>& c = L.col;
-            double usd = 0.0;
+            d.ssf_storage[p] = sy * (dsoil - szi) * area;
+            const double rd = ld(d.rd, p);
             {
                 bool first = true;
 #pragma unroll
                 for (int k = 0; k < N; ++k)
-                    if (k < c.n_unsat) {
-                        usd = first ? c.ustore[k] : usd + c.ustore[k];
+                    if (k < col.n_unsat) {
+                        usd = first ? col.ustore[k] : usd + col.ustore[k];
                         first = false;
                     }
             }
-            const double sbm_runoff =
-                wfh_max(0.0, s_exf_avg + d.sat_excess[p] + d.runoff_land[p] + d.inf_excess[p]);
-            const int nl = [&] {
-                int k2 = 0;
+            const double sbm_runoff = wfh_max(0.0, s_exf_avg + d.sat_excess[p] + d.runoff_land[p] + d.inf_excess[p]);
+            int nl = 0;
 #pragma unroll
-                for (int k = 0; k < N; ++k) k2 += !wfh_isnan(c.act[k]);
-                return k2;
-            }();
+            for (int k = 0; k < N; ++k) nl += !wfh_isnan(col.act[k]);
 #pragma unroll
             for (int k = 0; k < N; ++k) {
                 double v = NaN_;
                 if (k < nl) {
-                    if (k < c.n_unsat)
-                        v = (c.ustore[k] + (c.act[k] - c.thick[k]) * (ts - tr)) / c.act[k] + tr;
+                    if (k < col.n_unsat)
+                        v = (col.ustore[k] + (col.act[k] - col.thick[k]) * (ts - tr)) / col.act[k] + tr;
                     else
                         v = ts;
                 }
@@ -546,11 +598,11 @@ __device__ bool lane_advance(const Dev& d, const RouteArgs& a, Lane<N>& L) {
             double rs_unsat = 0.0;
 #pragma unroll
             for (int k = 0; k < N; ++k)
-                if (k < c.n_unsat)
-                    rs_unsat += wfh_min(1.0, (wfh_max(0.0, rd - c.cum[k]) / c.thick[k])) * c.ustore[k];
-            const double rs_sat = wfh_max(0.0, rd - c.zi) * (ts - tr);
+                if (k < col.n_unsat)
+                    rs_unsat += wfh_min(1.0, (wfh_max(0.0, rd - col.cum[k]) / col.thick[k])) * col.ustore[k];
+            const double rs_sat = wfh_max(0.0, rd - col.zi) * (ts - tr);
             const double rzs = rs_sat + rs_unsat;
-            const double swd = (dsoil - c.zi) * (ts - tr);
+            swd = (dsoil - col.zi) * (ts - tr);
             d.ustoredepth[p] = usd;
             d.swd[p] = swd;
             d.tsws[p] = swd + usd;
@@ -558,21 +610,18 @@ __device__ bool lane_advance(const Dev& d, const RouteArgs& a, Lane<N>& L) {
             d.runoff[p] = sbm_runoff;
             d.rzs[p] = rzs;
             d.vwc_root[p] = rzs / rd + tr;
-            const double net_runoff = sbm_runoff - d.aeow_l[p];
+            net_runoff = sbm_runoff - d.aeow_l[p];
             d.net_runoff[p] = net_runoff;
-            d.zi[p] = c.zi;
+            d.zi[p] = col.zi;
 #pragma unroll
-            for (int k = 0; k < N; ++k) d.ustore[k * n + p] = c.ustore[k];
-            (void)tfc;
+            for (int k = 0; k < N; ++k) d.ustore[k * n + p] = col.ustore[k];
             if (d.balance) {
-                // subsurface flow balance  mass_balance.jl:470-490 (recharge is a boundary flux)
-                double fin = 0.0, fout = 0.0;
+                double fin = 0.0, fout = 0.0;  // mass_balance.jl:470-490
                 if (rech_avg > 0.0)
                     fin += rech_avg;
                 else
                     fout -= rech_avg;
-                const double tin = s_qin_avg + fin, tout = s_q_avg + fout;
-                d.bal_ssf[p] = tin - tout - s_qnet_avg;
+                d.bal_ssf[p] = (s_qin_avg + fin) - (s_q_avg + fout) - s_qnet_avg;
                 // land balance  mass_balance.jl:215-253
                 const double t_in = s_qin_avg / area + (d.P[p] + 0.0) + 0.0 + 0.0 / area;
                 const double vout = net_runoff + d.aet[p] + d.net_runoff_river[p] + d.leakage[p] + 0.0;
@@ -585,182 +634,107 @@ __device__ bool lane_advance(const Dev& d, const RouteArgs& a, Lane<N>& L) {
                 d.bal_land[p] = err;
                 d.bal_land_rel[p] = avg == 0.0 ? 0.0 : err / avg;
             }
-            // ---- overland lateral inflow  surface_kinwave.jl:740-766, 366-371 ----
-            L.o_inwater = (net_runoff + 0.0) * area + 0.0;
-            d.ol_inwater[p] = L.o_inwater;
-            L.oqlat = L.o_inwater / ld(d.fl, p);
-            L.o_q_cum = L.o_qin_cum = L.o_tor_cum = 0.0;
-            L.phase = PH_OL;
-            L.sub = 0;
-            progressed = true;
-        } else if (L.phase == PH_OL) {
-            // ---- kinwave_land_update!  surface_kinwave.jl:293-341 ----
-            const int k = L.sub;
-            uint64_t* slots = a.slot_ol + (size_t)k * n;
-            while (L.e < L.e_end) {
-                double qu;
-                if (!slot_take(slots + d.up_idx[L.e], qu)) return progressed;
-                const double fu = d.up_f[L.e];
-                L.acc1 = L.acc1 + qu * fu;
-                L.acc2 = L.acc2 + qu * (1.0 - fu);
-                ++L.e;
-            }
+            // overland lateral inflow  surface_kinwave.jl:740-766
+            o_inwater = (net_runoff + 0.0) * area + 0.0;
+            d.ol_inwater[p] = o_inwater;
+        }
+
+        // ---------------- overland kinematic wave ----------------
+        const double alpha = ld(d.alpha, q_), sfw = ld(d.sfw, q_);
+        const double oqlat = o_inwater / flen;
+        double oq = d.ol_q[q_], oh = d.ol_h[q_], o_store_prev = d.ol_storage[q_], o_storage = 0.0, oqin = 0.0;
+        double o_q_cum = 0.0, o_qin_cum = 0.0, o_tor_cum = 0.0;
+        auto ol_compute = [&](int k, double tor, double qin_all) -> double {
             const double dts = (k == a.S_l - 1) ? a.dts_l[1] : a.dts_l[0];
-            L.o_tor_cum += L.acc1 * dts;
-            const double sfw = ld(d.sfw, p);
+            o_tor_cum += tor * dts;
             double qin = 0.0;
-            if (sfw > 0.0) qin = L.acc2;
+            if (sfw > 0.0) qin = qin_all;
             double qn, ar;
-            const double flen = ld(d.fl, p);
-            kinematic_wave(qin, L.oq, L.oqlat, ld(d.alpha, p), dts, flen, qn, ar);
-            L.oq = qn;
-            if (sfw > 0.0) L.oh = ar / sfw;
-            L.o_storage = flen * sfw * L.oh;
-            L.o_q_cum += qn * dts;
-            L.o_qin_cum += qin * dts;
-            L.oqin = qin;
-            if (d.flags[p] & 1) slot_put(slots + p, qn);
-            L.e = d.up_ptr[p];
-            L.acc1 = 0.0;
-            L.acc2 = 0.0;
-            progressed = true;
-            if (k + 1 < a.S_l) {
-                L.sub = k + 1;
-            } else {
-                const double q_avg = L.o_q_cum / dt;
-                const double tor_avg = L.o_tor_cum / dt;
-                const double qin_avg = L.o_qin_cum / dt;
-                d.ol_q[p] = L.oq;
-                d.ol_h[p] = L.oh;
-                d.ol_storage[p] = L.o_storage;
-                d.ol_qin[p] = L.oqin;
-                d.ol_q_avg[p] = q_avg;
-                d.ol_tor_avg[p] = tor_avg;
-                d.ol_qin_avg[p] = qin_avg;
-                if (d.balance) {  // overland balance  mass_balance.jl:382-399
-                    const double tin = L.o_inwater + qin_avg, tout = q_avg;
-                    d.bal_ol[p] = tin - tout - (L.o_storage - L.o_store_prev) / dt;
-                }
-                if (L.r >= 0) {
-                    // ---- river lateral inflow  surface_kinwave.jl:710-734, 632 ----
-                    const int r = L.r;
-                    L.r_inwater = (d.ssf_tor_avg[p] + tor_avg + d.net_runoff_river[p] * ld(d.area, p)) +
-                                  0.0 * d.r_area[r];
-                    d.r_inwater[r] = L.r_inwater;
-                    L.rqlat = L.r_inwater / d.r_fl[r];
-                    L.r_q_cum = L.r_qin_cum = 0.0;
-                    L.e = d.rup_ptr[r];
-                    L.e_end = d.rup_ptr[r + 1];
-                    L.phase = PH_RIVER;
-                    L.sub = 0;
-                } else {
-                    L.phase = PH_FINAL;
-                }
+            kinematic_wave(qin, oq, oqlat, alpha, dts, flen, qn, ar);
+            oq = qn;
+            if (sfw > 0.0) oh = ar / sfw;
+            o_storage = flen * sfw * oh;
+            o_q_cum += qn * dts;
+            o_qin_cum += qin * dts;
+            oqin = qin;
+            return qn;
+        };
+        if (!run_family<FOLD_OL>(a, lane, c, valid, a.S_l, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ol, (size_t)n,
+                                 p, (fl & 1) && !cons_in, cons_in ? (dn & 63) : -1, ol_compute))
+            return;
+        double r_inwater = 0.0, o_tor_avg = 0.0;
+        if (valid) {
+            const double q_avg = o_q_cum / dt;
+            o_tor_avg = o_tor_cum / dt;
+            const double qin_avg = o_qin_cum / dt;
+            d.ol_q[p] = oq;
+            d.ol_h[p] = oh;
+            d.ol_storage[p] = o_storage;
+            d.ol_qin[p] = oqin;
+            d.ol_q_avg[p] = q_avg;
+            d.ol_tor_avg[p] = o_tor_avg;
+            d.ol_qin_avg[p] = qin_avg;
+            if (d.balance) d.bal_ol[p] = (o_inwater + qin_avg) - q_avg - (o_storage - o_store_prev) / dt;
+            if (r >= 0) {  // river lateral inflow  surface_kinwave.jl:710-734
+                r_inwater = (s_tor_avg + o_tor_avg + d.net_runoff_river[p] * area) + 0.0 * d.r_area[r];
+                d.r_inwater[r] = r_inwater;
             }
-        } else if (L.phase == PH_RIVER) {
-            // ---- kinwave_river_update!  surface_kinwave.jl:492-566 ----
-            const int k = L.sub, r = L.r;
-            uint64_t* slots = a.slot_r + (size_t)k * d.nriv;
-            while (L.e < L.e_end) {
-                double qu;
-                if (!slot_take(slots + d.rup_idx[L.e], qu)) return progressed;
-                L.acc1 = L.acc1 + qu;
-                ++L.e;
-            }
+        }
+
+        // ---------------- river kinematic wave ----------------
+        const bool riv = r >= 0;
+        const int rr = riv ? r : 0;
+        const int re0 = d.rup_ptr[rr], re1 = riv ? d.rup_ptr[rr + 1] : re0;
+        const bool rcons_in = riv && (fl & 4) && cons_in;  // a river cell drains to the land downstream
+        const double rfl = d.r_fl[rr], rfw = d.r_fw[rr], ralpha = d.r_alpha[rr];
+        const double rqlat = r_inwater / rfl;
+        double rq = d.r_q[rr], rh = d.r_h[rr], r_store_prev = d.r_storage[rr], r_storage = 0.0, rqin = 0.0;
+        double r_q_cum = 0.0, r_qin_cum = 0.0;
+        auto riv_compute = [&](int k, double sumq, double) -> double {
             const double dts = (k == a.S_r - 1) ? a.dts_r[1] : a.dts_r[0];
-            const double rfl = d.r_fl[r];
-            const double qin = 0.0 + L.acc1;
+            const double qin = 0.0 + sumq;
             double inflow = 0.0 / rfl;
             inflow -= 0.0 / rfl;
             double qn, ar;
-            kinematic_wave(qin, L.rq, L.rqlat + inflow, d.r_alpha[r], dts, rfl, qn, ar);
-            L.rq = qn;
-            L.rh = ar / d.r_fw[r];
-            L.r_storage = rfl * ar;
-            L.r_q_cum += qn * dts;
-            L.r_qin_cum += qin * dts;
-            L.rqin = qin;
-            if (d.flags[p] & 4) slot_put(slots + r, qn);
-            L.e = d.rup_ptr[r];
-            L.acc1 = 0.0;
-            progressed = true;
-            if (k + 1 < a.S_r) {
-                L.sub = k + 1;
-            } else {
-                const double q_avg = L.r_q_cum / dt, qin_avg = L.r_qin_cum / dt;
-                d.r_q[r] = L.rq;
-                d.r_h[r] = L.rh;
-                d.r_storage[r] = L.r_storage;
-                d.r_qin[r] = L.rqin;
-                d.r_q_avg[r] = q_avg;
-                d.r_qin_avg[r] = qin_avg;
-                if (d.balance) {  // river balance  mass_balance.jl:290-313
-                    const double tin = L.r_inwater + qin_avg + wfh_max(0.0, 0.0);
-                    const double tout = q_avg + 0.0 + 0.0;
-                    d.bal_r[r] = tin - tout - (L.r_storage - L.r_store_prev) / dt;
-                }
-                L.phase = PH_FINAL;
-            }
-        } else if (L.phase == PH_FINAL) {
-            // ---- update_total_water_storage!  sbm.jl:143-182 ----
-            const double area = ld(d.area, p);
+            kinematic_wave(qin, rq, rqlat + inflow, ralpha, dts, rfl, qn, ar);
+            rq = qn;
+            rh = ar / rfw;
+            r_storage = rfl * ar;
+            r_q_cum += qn * dts;
+            r_qin_cum += qin * dts;
+            rqin = qin;
+            return qn;
+        };
+        if (!run_family<FOLD_RIV>(a, lane, c, riv, a.S_r, d.rup_node, d.rup_idx, nullptr, re0, re1, a.slot_r,
+                                  (size_t)d.nriv, rr, riv && (fl & 4) && !rcons_in, rcons_in ? (dn & 63) : -1,
+                                  riv_compute))
+            return;
+        if (riv) {
+            const double q_avg = r_q_cum / dt, qin_avg = r_qin_cum / dt;
+            d.r_q[r] = rq;
+            d.r_h[r] = rh;
+            d.r_storage[r] = r_storage;
+            d.r_qin[r] = rqin;
+            d.r_q_avg[r] = q_avg;
+            d.r_qin_avg[r] = qin_avg;
+            if (d.balance)  // mass_balance.jl:290-313
+                d.bal_r[r] = (r_inwater + qin_avg + wfh_max(0.0, 0.0)) - (q_avg + 0.0 + 0.0) -
+                             (r_storage - r_store_prev) / dt;
+        }
+
+        // ---------------- update_total_water_storage!  sbm.jl:143-182 ----------------
+        if (valid) {
             double tot = 0.0;
-            if (L.r >= 0) tot = (L.rh * d.r_fw[L.r] * d.r_fl[L.r]) / (area);
+            if (riv) tot = (rh * rfw * rfl) / (area);
             const double ss = d.snow ? d.snow_s[p] : 0.0, sw = d.snow ? d.snow_w[p] : 0.0;
             tot = tot + ((((ss + sw) + 0.0 * 0.0) + d.canopy[p]) + 0.0);
-            const double sub = d.ustoredepth[p] + d.swd[p];
-            const double lat = L.oh * (1.0 - ld(d.rf, p));
+            const double sub = usd + swd;
+            const double lat = oh * (1.0 - ld(d.rf, p));
             tot += sub + lat;
             d.total_storage[p] = tot;
-            L.phase = PH_DONE;
-            return true;
-        } else {
-            return progressed;
         }
     }
 }
-
-template <int N>
-__global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[a.parity ^ 1] = 0ull;  // next launch's counter
-    for (;;) {
-        unsigned long long t = 0;
-        if (lane == 0) t = atomicAdd(&a.counters[a.parity], 1ull);
-        t = __shfl(t, 0);
-        if (t >= (unsigned long long)a.nchunks) break;
-        if (__hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-        Lane<N> L;
-        L.p = (int)t * WAVE + lane;
-        if (L.p < d.n) {
-            lane_init<N>(d, L);
-            ssf_begin<N>(d, L, (a.S_s == 1) ? a.dts_s[1] : a.dts_s[0]);
-        } else {
-            L.phase = PH_DONE;
-        }
-        long spin = 0;
-        for (;;) {
-            const bool active = L.phase != PH_DONE;
-            if (!__any(active)) break;
-            bool prog = false;
-            if (active) prog = lane_advance<N>(d, a, L);
-            if (!__any(prog)) {
-                __builtin_amdgcn_s_sleep(2);
-                ++spin;
-                if ((spin & 255) == 0 &&
-                    __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                    return;
-                if (spin > SPIN_LIMIT) {
-                    __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    return;
-                }
-            } else {
-                spin = 0;
-            }
-        }
-    }
-}
-
 // --------------------------------------------------------------------------------------------
 __global__ void k_fill_u64(uint64_t* p, uint64_t v, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
@@ -1168,15 +1142,29 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
             }
         }
     }
-    std::vector<int> rup_ptr(nriv + 1, 0), rup_idx;
+    std::vector<int> rup_ptr(nriv + 1, 0), rup_idx, rup_node, down_node(n, -1);
     {
         std::vector<std::vector<int>> ups(nriv);
         for (int64_t r = 0; r < nriv; ++r)
             if (h->river_down[r] >= 0) ups[h->river_down[r]].push_back((int)r);
         for (int64_t r = 0; r < nriv; ++r) {
-            for (int u : ups[r]) rup_idx.push_back(u);
+            for (int u : ups[r]) {
+                rup_idx.push_back(u);
+                rup_node.push_back(h->node_of[h->river_land[u]]);
+            }
             rup_ptr[r + 1] = (int)rup_idx.size();
         }
+        for (int64_t q = 0; q < n; ++q) {
+            const int64_t w = h->land_down[h->land_of[q]];
+            down_node[q] = w >= 0 ? h->node_of[w] : -1;
+        }
+        // a river cell's river downstream is the river cell at its land downstream
+        for (int64_t r = 0; r < nriv; ++r)
+            if (h->river_down[r] >= 0 && h->river_land[h->river_down[r]] != h->land_down[h->river_land[r]]) {
+                h->err = "river drainage differs from land drainage";
+                wfh_destroy_keep_error(h, out);
+                return WFH_ERR_ARG;
+            }
     }
 
     // ---------------- land parameters (domain.jl:218-366) ----------------
@@ -1361,6 +1349,8 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         if ((rc = put_raw<uint8_t>(h, flags, D.flags))) goto done;
         if ((rc = put_raw<int>(h, rup_ptr, D.rup_ptr))) goto done;
         if ((rc = put_raw<int>(h, rup_idx, D.rup_idx))) goto done;
+        if ((rc = put_raw<int>(h, rup_node, D.rup_node))) goto done;
+        if ((rc = put_raw<int>(h, down_node, D.down))) goto done;
         if ((rc = put_raw<double>(h, r_alpha, D.r_alpha))) goto done;
         if ((rc = put_raw<double>(h, r_fw, D.r_fw))) goto done;
         if ((rc = put_raw<double>(h, r_fl, D.r_fl))) goto done;
