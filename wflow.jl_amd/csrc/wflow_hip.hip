@@ -46,7 +46,7 @@ namespace {
 constexpr uint64_t SLOT_EMPTY = 0xFFF40000DEADBEEFULL;  // signalling NaN, never produced by arithmetic
 constexpr int WAVE = 64;
 constexpr int ROUTE_BLOCK = 256;
-constexpr long SPIN_LIMIT = 20000000;  // polls (each with a sleep) before a wait is declared stuck
+constexpr long SPIN_LIMIT = 2000000;  // polls without progress (each with a sleep) before a wave gives up
 
 struct Prm {  // parameter map, broadcast when uniform
     const double* p;
@@ -431,43 +431,94 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
                                            const int e0, const int e1, uint64_t* slots, const size_t stride,
                                            const int my_slot, const bool pub_cross, const int cons_lane,
                                            Compute& compute) {
+    // The first PF upstream edges live in registers. Their cross-wave outflow is polled at the
+    // top of an iteration and picked up after the compute block, so the poll latency overlaps
+    // the work of lanes that are ready; the in-order fold reads it from `buf`. Edges beyond PF
+    // (rare: more than PF upstream cells) are polled in order from memory.
+    constexpr int PF = 4;
     int k = part ? 0 : S;
     int e = e0;
     double acc1 = 0.0, acc2 = 0.0, qv = 0.0;
+    const int npf = (e1 - e0) < PF ? (e1 - e0) : PF;
+    int en[PF], es[PF];
+    double ef[PF], buf[PF];
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+        en[j] = c * WAVE;  // in-wave dummy
+        es[j] = 0;
+        ef[j] = 0.0;
+        buf[j] = 0.0;
+        if (j < npf) {
+            en[j] = up_node[e0 + j];
+            es[j] = up_slot[e0 + j];
+            if (FOLD != FOLD_RIV) ef[j] = up_f[e0 + j];
+        }
+    }
+    unsigned have = 0;  // bit j: buf[j] holds the outflow of edge e0 + j for the current sub-step
     long spin = 0;
     while (__any(k < S)) {
         const bool act = k < S;
+        // (1) issue the polls
+        uint64_t v[PF];
+        bool pol[PF];
+#pragma unroll
+        for (int j = 0; j < PF; ++j) {
+            pol[j] = act && j < npf && (en[j] >> 6) != c && !(have & (1u << j)) && e0 + j >= e;
+            v[j] = SLOT_EMPTY;
+            if (pol[j])
+                v[j] = __hip_atomic_load(slots + (size_t)k * stride + es[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // (2) in-order fold of what is available
         bool took_any = false;
         for (int pass = 0; pass < 9; ++pass) {
-            int src = lane;
-            bool inw = false, pend = act && e < e1;
-            int u = 0;
+            const bool pend = act && e < e1;
+            const int j = e - e0;
+            int u = c * WAVE;
+            double f = 0.0;
             if (pend) {
-                u = up_node[e];
-                inw = (u >> 6) == c;
-                if (inw) src = u & 63;
+                if (j < PF) {
+#pragma unroll
+                    for (int jj = 0; jj < PF; ++jj)
+                        if (jj == j) {
+                            u = en[jj];
+                            f = ef[jj];
+                        }
+                } else {
+                    u = up_node[e];
+                    if (FOLD != FOLD_RIV) f = up_f[e];
+                }
             }
+            const bool inw = pend && (u >> 6) == c;
+            const int src = inw ? (u & 63) : lane;
             const int ku = __shfl(k, src);
             const double qu = __shfl(qv, src);
             bool took = false;
+            double q = 0.0;
             if (pend) {
-                double q = 0.0;
                 if (inw) {
                     if (ku == k + 1) {
                         q = qu;
                         took = true;
                     }
+                } else if (j < PF) {
+#pragma unroll
+                    for (int jj = 0; jj < PF; ++jj)
+                        if (jj == j && (have & (1u << jj))) {
+                            q = buf[jj];
+                            took = true;
+                        }
                 } else {
                     took = slot_take(slots + (size_t)k * stride + up_slot[e], q);
                 }
                 if (took) {
-                    fold_in<FOLD>(acc1, acc2, q, FOLD == FOLD_RIV ? 0.0 : up_f[e]);
+                    fold_in<FOLD>(acc1, acc2, q, f);
                     ++e;
                 }
             }
             if (!__any(took)) break;
             took_any = true;
         }
+        // (3) compute the ready lanes
         const int kc = __shfl(k, cons_lane >= 0 ? cons_lane : lane);
         const bool can = act && e == e1 && (cons_lane < 0 || kc >= k);
         if (can) {
@@ -477,8 +528,21 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
             e = e0;
             acc1 = 0.0;
             acc2 = 0.0;
+            have = 0;
         }
-        if (__any(can) || took_any) {
+        // (4) pick up the polled outflow (a lane that polled cannot have computed above)
+        bool fetched = false;
+#pragma unroll
+        for (int j = 0; j < PF; ++j) {
+            if (pol[j] && v[j] != SLOT_EMPTY) {
+                __hip_atomic_store(slots + (size_t)(k) * stride + es[j], SLOT_EMPTY, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                buf[j] = wfh_from_bits(v[j]);
+                have |= 1u << j;
+                fetched = true;
+            }
+        }
+        if (__any(can) || took_any || __any(fetched)) {
             spin = 0;
         } else {
             __builtin_amdgcn_s_sleep(1);
@@ -768,6 +832,7 @@ struct wfh_handle {
     int route_grid = 0;
     int nchunks = 0;
     int max_level = 0;
+    int thin_level = 0;
     std::string err;
     // host topology (land index order unless stated)
     std::vector<int64_t> land_order, land_down, river_order, river_down, river_land;
@@ -1110,15 +1175,77 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
     h->max_level = 0;
     for (int64_t i = 0; i < n; ++i) h->max_level = std::max(h->max_level, level[i]);
     {
+        // Levels holding at least one wave of cells are laid out level by level (ascending land
+        // index inside a level). From the first level with fewer cells than a wave onwards the
+        // cells are laid out in an upstream-first depth-first postorder that visits the deepest
+        // upstream branch last, so a main stem and the cells feeding it stay contiguous and a
+        // wave holds a long piece of one chain. Both parts are topological orders and every
+        // cell of the first part precedes every cell of the second.
+        std::vector<int64_t> per_level(h->max_level + 1, 0);
+        for (int64_t i = 0; i < n; ++i) per_level[level[i]]++;
+        int thin = h->max_level + 1;
+        for (int l = 0; l <= h->max_level; ++l)
+            if (per_level[l] < WAVE) {
+                thin = l;
+                break;
+            }
+        h->thin_level = thin;
         std::vector<int64_t> cnt(h->max_level + 2, 0);
-        for (int64_t i = 0; i < n; ++i) cnt[level[i] + 1]++;
+        for (int64_t i = 0; i < n; ++i)
+            if (level[i] < thin) cnt[level[i] + 1]++;
         for (int l = 0; l <= h->max_level; ++l) cnt[l + 1] += cnt[l];
         h->land_of.assign(n, 0);
         h->node_of.assign(n, 0);
-        for (int64_t i = 0; i < n; ++i) {  // ascending land index within a level
-            const int64_t q = cnt[level[i]]++;
-            h->land_of[q] = (int)i;
-            h->node_of[i] = (int)q;
+        int64_t q_thin = 0;
+        for (int64_t i = 0; i < n; ++i)
+            if (level[i] < thin) {
+                const int64_t q = cnt[level[i]]++;
+                h->land_of[q] = (int)i;
+                h->node_of[i] = (int)q;
+                ++q_thin;
+            }
+        // upstream lists (ascending id) for the DFS
+        std::vector<int64_t> uptr(n + 1, 0), uidx;
+        for (int64_t i = 0; i < n; ++i)
+            if (h->land_down[i] >= 0) uptr[h->land_down[i] + 1]++;
+        for (int64_t i = 0; i < n; ++i) uptr[i + 1] += uptr[i];
+        uidx.resize(uptr[n]);
+        {
+            std::vector<int64_t> fillp(uptr.begin(), uptr.end() - 1);
+            for (int64_t i = 0; i < n; ++i)
+                if (h->land_down[i] >= 0) uidx[fillp[h->land_down[i]]++] = i;
+        }
+        // subtree depth (longest upstream path) is `level`; roots of the thin part are thin cells
+        // whose downstream cell is absent (pits): a thin cell only drains to thin cells
+        std::vector<std::pair<int64_t, int>> stack;  // (cell, next child position)
+        int64_t q = q_thin;
+        for (int64_t root = 0; root < n; ++root) {
+            if (level[root] < thin || h->land_down[root] >= 0) continue;
+            stack.push_back({root, 0});
+            while (!stack.empty()) {
+                auto& top = stack.back();
+                const int64_t v = top.first;
+                // children in thin part, deepest (largest level) last; ties by ascending id
+                std::vector<int64_t> kids;
+                for (int64_t e = uptr[v]; e < uptr[v + 1]; ++e)
+                    if (level[uidx[e]] >= thin) kids.push_back(uidx[e]);
+                std::stable_sort(kids.begin(), kids.end(),
+                                 [&](int64_t a, int64_t b) { return level[a] < level[b]; });
+                if (top.second < (int)kids.size()) {
+                    const int64_t u = kids[top.second++];
+                    stack.push_back({u, 0});
+                } else {
+                    h->land_of[q] = (int)v;
+                    h->node_of[v] = (int)q;
+                    ++q;
+                    stack.pop_back();
+                }
+            }
+        }
+        if (q != n) {
+            h->err = "internal error: node order does not cover every cell";
+            wfh_destroy_keep_error(h, out);
+            return WFH_ERR_ARG;
         }
     }
     // upstream CSR in node order, lists in ascending land index

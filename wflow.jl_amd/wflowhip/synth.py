@@ -223,6 +223,7 @@ def make_catchment(nx, ny=None, cell=1000.0, dt=86400.0, river_threshold=100, se
     c = Catchment(nx, ny, float(cell), float(dt), A, snow=snow, infiltration_reduction=infiltration_reduction)
     c.max_depth = int(maxdepth)
     c.upstream_area = area
+    c.depth = depth  # topological level of every cell (longest upstream path)
     return c
 
 
