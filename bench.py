@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of the Wflow.jl `sbm` time step (update_model!) on one MI355X.
 
-Workload: the configs[1] generator - 4096 x 4096 cells (16.8 M), dx 250 m, hourly steps,
+Workload: 4096 x 4096 cells (16.8 M) from the synthetic generator, dx 250 m, hourly steps,
 river cells where the upstream area exceeds 2000 cells, FP64 states. A "step" is one model
 time step of the whole grid: the vertical column kernel plus the routing kernel (lateral
 subsurface flow, overland and river kinematic waves with their fixed sub-steps, storage and
@@ -141,7 +141,7 @@ def main():
         except Exception:
             traffic = None
     result = {
-        "metric": "cell-timesteps/s FP64 SBM+kinwave (plain GPU step, configs[1] workload)",
+        "metric": "cell-timesteps/s FP64 SBM+kinwave (plain GPU step)",
         "value": c.n * args.steps / elapsed,
         "unit": "cell-timesteps/s",
         "n_gpus": 1,
@@ -153,7 +153,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (fractal DEM, seeded parameters, GPU-generated forcing)",
-        "config": {"workload": f"configs[1]: {args.nx}x{args.nx} cells, dx 250 m, hourly, river threshold 2000, "
+        "config": {"workload": f"{args.nx}x{args.nx} cells, dx 250 m, hourly, river threshold 2000 cells, "
                                "sbm + kinematic wave (land 1, river 4, subsurface 1 sub-steps)",
                    "cells": c.n, "river_cells": c.nriv, "longest_flow_path": c.max_depth,
                    "kernel_ms": {"k_vertical": k1_ms, "k_route": k2_ms},

@@ -116,8 +116,9 @@ def test_topology_bit_exact():
     assert np.all(pos[np.nonzero(has)[0]] < pos[ld[has]])
 
 
-def test_c0_config_365_daily_steps():
-    """256x256, dx 1 km, daily, 365 steps; checked every 5 steps and at the end."""
+def test_daily_256_365_steps():
+    """256x256, dx 1 km, daily, 365 steps (24 land / 96 river sub-steps); checked every 5 steps
+    and at the end."""
     c = synth.make_catchment(256, cell=1000.0, dt=86400.0, river_threshold=100, seed=0)
     gpu, ref = run_pair(c, 365, every=5)
     # identical water-balance closure, and the reference's closure bound where it applies
@@ -191,8 +192,8 @@ def test_step_without_forcing_is_an_error():
     assert e.value.code == _abi.WFH_ERR_STATE
 
 
-def test_c1_hourly_1024_first_24_steps():
-    """configs[1] generator (250 m, hourly, river threshold 2000 cells) at 1024^2, 24 steps."""
+def test_hourly_1024_first_24_steps():
+    """The hourly generator (250 m, river threshold 2000 cells) at 1024^2, 24 steps."""
     c = synth.make_catchment(1024, cell=250.0, dt=3600.0, river_threshold=2000, seed=1)
     gpu, ref = run_pair(c, 24, every=6)
     gpu.close()

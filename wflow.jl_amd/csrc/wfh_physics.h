@@ -225,17 +225,39 @@ WFH_HD double rwu_reduction_feddes(double h, double h1, double h2, double h3, do
 
 // ---- kinematic wave ------------------------------------------------------------------------
 // kinematic_wave  Wflow/src/routing/surface/surface_process.jl:24-70
-WFH_HD void kinematic_wave(double q_in, double q_prev, double q_lat, double alpha, double dt, double dx,
-                           double& q, double& area) {
-    if (q_in + q_prev + q_lat == 0.0) {
+//
+// Split in two so that a routing loop can hoist everything that does not depend on the
+// upstream inflow q_in (the fifth root of q_prev above all) out of the dependent chain:
+// kw_prepare(q_prev, ...) followed by kw_solve(q_in, ...) performs exactly the operations of the
+// reference, in its order.
+struct KwPrep {
+    double dt_dx;  // dt / dx
+    double u_prev; // q_prev^(1/5) (0 for negative q_prev)
+    double a_u3;   // alpha * u_prev * u_prev * u_prev
+    double lat;    // dt * q_lat
+    double q_prev, q_lat;
+};
+
+WFH_HD KwPrep kw_prepare(double q_prev, double q_lat, double alpha, double dt, double dx) {
+    KwPrep p;
+    p.dt_dx = dt / dx;
+    p.u_prev = q_prev >= 0.0 ? wfh_pow(q_prev, 0.2) : 0.0;
+    p.a_u3 = alpha * p.u_prev * p.u_prev * p.u_prev;
+    p.lat = dt * q_lat;
+    p.q_prev = q_prev;
+    p.q_lat = q_lat;
+    return p;
+}
+
+WFH_HD void kw_solve(double q_in, const KwPrep& p, double alpha, double& q, double& area) {
+    if (q_in + p.q_prev + p.q_lat == 0.0) {
         q = 0.0;
         area = 0.0;
         return;
     }
-    const double dt_dx = dt / dx;
-    const double u_prev = q_prev >= 0.0 ? wfh_pow(q_prev, 0.2) : 0.0;
-    const double C = dt_dx * q_in + alpha * u_prev * u_prev * u_prev + dt * q_lat;
-    double u = u_prev > 0.0 ? u_prev : wfh_cbrt(C / alpha);
+    const double dt_dx = p.dt_dx;
+    const double C = dt_dx * q_in + p.a_u3 + p.lat;
+    double u = p.u_prev > 0.0 ? p.u_prev : wfh_cbrt(C / alpha);
     const double c1 = 5.0 * dt_dx, c2 = 3.0 * alpha;
     for (int k = 0; k < 3000; ++k) {
         const double u2 = u * u;
@@ -252,11 +274,14 @@ WFH_HD void kinematic_wave(double q_in, double q_prev, double q_lat, double alph
     area = alpha * u3;
 }
 
+WFH_HD void kinematic_wave(double q_in, double q_prev, double q_lat, double alpha, double dt, double dx,
+                           double& q, double& area) {
+    const KwPrep p = kw_prepare(q_prev, q_lat, alpha, dt, dx);
+    kw_solve(q_in, p, alpha, q, area);
+}
+
 // kw_ssf_newton_raphson  Wflow/src/routing/subsurface/subsurface_process.jl:57-78
-WFH_HD double kw_ssf_newton_raphson(double q, double C, double cel, double dt, double dx) {
-    const double dt_dx = dt / dx;
-    const double cinv = 1.0 / cel;
-    const double df = dt_dx + cinv;
+WFH_HD double kw_ssf_newton_raphson_c(double q, double C, double dt_dx, double cinv, double df) {
     for (int count = 0;; ++count) {
         const double r = dt_dx * q + cinv * q - C;
         q -= (r / df);
@@ -265,6 +290,34 @@ WFH_HD double kw_ssf_newton_raphson(double q, double C, double cel, double dt, d
         if (wfh_fabs(r) <= 1.0e-12 || count >= 3000) break;
     }
     return q;
+}
+
+WFH_HD double kw_ssf_newton_raphson(double q, double C, double cel, double dt, double dx) {
+    const double dt_dx = dt / dx;
+    const double cinv = 1.0 / cel;
+    return kw_ssf_newton_raphson_c(q, C, dt_dx, cinv, dt_dx + cinv);
+}
+
+// The part of one subsurface kinematic-wave solve that does not depend on the upstream inflow:
+// the celerity at the previous water table (ssf_celerity, KhExponential,
+// subsurface_process.jl:5-20) and the Newton-Raphson constants derived from it. A routing loop
+// evaluates this before the inflow arrives, off the cell-to-cell dependency chain.
+struct SsfPrep {
+    double dt_dx;  // dt / dx
+    double cinv;   // 1 / celerity
+    double df;     // dt / dx + 1 / celerity
+    double qp_cel; // q_prev / celerity
+};
+
+WFH_HD SsfPrep ssf_prepare(double q_prev, double zi_prev, double slope, double sy, double dt, double dx, double kh0,
+                           double f) {
+    const double cel = (kh0 * wfh_exp(-f * zi_prev) * slope) / sy;
+    SsfPrep p;
+    p.dt_dx = dt / dx;
+    p.cinv = 1.0 / cel;
+    p.df = p.dt_dx + p.cinv;
+    p.qp_cel = q_prev / cel;
+    return p;
 }
 
 // Unsaturated layers of one column as the subsurface solver sees them.
@@ -337,22 +390,28 @@ WFH_HD void update_ustorelayerdepth(Column<N>& s, double zi_prev, double zi) {
 // kinematic_wave_ssf for the exponential Kh profile
 // Wflow/src/routing/subsurface/subsurface_process.jl:89-172 (ssf_celerity :6-21)
 // zi_prev is the subsurface-flow water table (it can differ from the soil column's after an
-// early return, which sets the former to d and leaves the column alone).
+// early return, which sets the former to d and leaves the column alone); `pp` is
+// ssf_prepare(q_prev, zi_prev, slope, sy, dt, dx, kh0, f).
+//
+// With defer_update set, the closing update_ustorelayerdepth!(soil, zi_prev, zi_out) of the
+// single-step branch is left to the caller (it touches only the column, which nothing in the
+// remainder of the solve reads) and the function returns true; the caller must apply it before
+// the column is read again. The sub-stepped branch always updates the column itself.
 template <int N>
-WFH_HD void kinematic_wave_ssf(double q_in, double q_prev, double zi_prev, double q_net, double slope, double sy,
+WFH_HD bool kinematic_wave_ssf(double q_in, double q_prev, double zi_prev, double q_net, double slope, double sy,
                                double d, double dt, double dx, double dw, double q_max, double kh0, double f,
-                               Column<N>& soil, double& q_out, double& zi_out, double& exf_out, double& net_out) {
+                               const SsfPrep& pp, Column<N>& soil, const bool defer_update, double& q_out,
+                               double& zi_out, double& exf_out, double& net_out) {
     if (q_in + q_prev == 0.0 && q_net <= 0.0) {
         q_out = 0.0;
         zi_out = d;
         exf_out = 0.0;
         net_out = 0.0;
-        return;
+        return false;
     }
     double q = (q_prev + q_in) / 2.0;
-    double cel = (kh0 * wfh_exp(-f * zi_prev) * slope) / sy;
-    double C = (dt / dx) * (q_in + q_net) + q_prev / cel;
-    q = kw_ssf_newton_raphson(q, C, cel, dt, dx);
+    double C = pp.dt_dx * (q_in + q_net) + pp.qp_cel;
+    q = kw_ssf_newton_raphson_c(q, C, pp.dt_dx, pp.cinv, pp.df);
     q = wfh_min(q, (q_max * dw));
     double net = (q_in + q_net - q) / (dw * dx);
     double dh, exf;
@@ -363,13 +422,22 @@ WFH_HD void kinematic_wave_ssf(double q_in, double q_prev, double zi_prev, doubl
         q = wfh_max(q - q_excess, WFH_KIN_WAVE_MIN_FLOW);
     }
     zi = wfh_clamp(zi, 0.0, d);
-    const double its_f = WFH_CEIL(wfh_round_sig12(wfh_fabs(zi - zi_prev) / 0.1));
-    const long its = (long)its_f;
+    // its = Int(ceil(round(x; sigdigits = 12))). Rounding to 12 significant digits moves x by
+    // less than 1e-11 of its value, so every x <= 0.999 gives its <= 1 (the single-step
+    // branch); only larger x need the exact count.
+    const double x = wfh_fabs(zi - zi_prev) / 0.1;
+    double its_f = 1.0;
+    long its = 1;
+    if (x > 0.999) {
+        its_f = WFH_CEIL(wfh_round_sig12(x));
+        its = (long)its_f;
+    }
+    bool deferred = false;
     if (its > 1) {
         const double dt_s = dt / its_f;
         double q_sum = 0.0, exf_sum = 0.0, net_sum = 0.0;
         for (long i = 0; i < its; ++i) {
-            cel = (kh0 * wfh_exp(-f * zi_prev) * slope) / sy;
+            const double cel = (kh0 * wfh_exp(-f * zi_prev) * slope) / sy;
             C = (dt_s / dx) * q_in + q_prev / cel + q_net * (dt_s / dx);
             q = kw_ssf_newton_raphson(q_prev, C, cel, dt_s, dx);
             q = wfh_min(q, (q_max * dw));
@@ -391,6 +459,8 @@ WFH_HD void kinematic_wave_ssf(double q_in, double q_prev, double zi_prev, doubl
         q = q_sum / its_f;
         exf = exf_sum / its_f;
         net = net_sum / its_f;
+    } else if (defer_update) {
+        deferred = true;
     } else {
         update_ustorelayerdepth<N>(soil, zi_prev, zi);
     }
@@ -398,6 +468,7 @@ WFH_HD void kinematic_wave_ssf(double q_in, double q_prev, double zi_prev, doubl
     zi_out = zi;
     exf_out = exf;
     net_out = net;
+    return deferred;
 }
 
 }  // namespace wfh

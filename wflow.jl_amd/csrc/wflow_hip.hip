@@ -596,8 +596,15 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         col.theta_fc_r = tfc - tr;
         double sq = d.ssf_q[q_], szi = col.zi;  // sbm_model.jl:80 hands the soil water table over
         double s_qin_cum = 0.0, s_q_cum = 0.0, s_exf_cum = 0.0, s_qnet_cum = 0.0, s_tor_cum = 0.0, rech_cum = 0.0;
+        // Inflow-independent part of the first sub-step's solve, evaluated before the chain
+        // reaches this cell; the closing soil-column update of the last sub-step is applied
+        // after the family (nothing in the family reads the column after that solve).
+        SsfPrep pp = ssf_prepare(sq, szi, slope, sy, a.S_s == 1 ? a.dts_s[1] : a.dts_s[0], flen, kh0, fk);
+        bool col_upd = false;
+        double col_z0 = 0.0, col_z1 = 0.0;
         auto ssf_compute = [&](int k, double qin, double tor) -> double {
             const double dts = (k == a.S_s - 1) ? a.dts_s[1] : a.dts_s[0];
+            if (k > 0) pp = ssf_prepare(sq, szi, slope, sy, dts, flen, kh0, fk);
             // flux!(::RechargeModel) with check_flux  boundary_conditions.jl:12-21,219-236
             double flx = rech * area;
             if (szi >= dsoil) flx = wfh_max(0.0, flx);
@@ -605,8 +612,12 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             const double qnb = 0.0 + flx;
             s_tor_cum += tor * dts;
             double qn, zin, exf, net;
-            kinematic_wave_ssf<N>(qin, sq, szi, qnb, slope, sy, dsoil, dts, flen, fw, qmax, kh0, fk, col, qn, zin,
-                                  exf, net);
+            if (kinematic_wave_ssf<N>(qin, sq, szi, qnb, slope, sy, dsoil, dts, flen, fw, qmax, kh0, fk, pp, col,
+                                      k == a.S_s - 1, qn, zin, exf, net)) {
+                col_upd = true;
+                col_z0 = szi;
+                col_z1 = zin;
+            }
             sq = qn;
             szi = zin;
             s_qin_cum += qin * dts;
@@ -618,6 +629,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         if (!run_family<FOLD_SSF>(a, lane, c, valid, a.S_s, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ssf,
                                   (size_t)n, p, (fl & 1) && !cons_in, cons_in ? (dn & 63) : -1, ssf_compute))
             return;
+        if (col_upd) update_ustorelayerdepth<N>(col, col_z0, col_z1);
 
         // ---------------- average_flux_vars!, update_soil_water_storage! ----------------
         double net_runoff = 0.0, o_inwater = 0.0, s_tor_avg = 0.0;
@@ -708,13 +720,15 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         const double oqlat = o_inwater / flen;
         double oq = d.ol_q[q_], oh = d.ol_h[q_], o_store_prev = d.ol_storage[q_], o_storage = 0.0, oqin = 0.0;
         double o_q_cum = 0.0, o_qin_cum = 0.0, o_tor_cum = 0.0;
+        KwPrep op = kw_prepare(oq, oqlat, alpha, a.S_l == 1 ? a.dts_l[1] : a.dts_l[0], flen);
         auto ol_compute = [&](int k, double tor, double qin_all) -> double {
             const double dts = (k == a.S_l - 1) ? a.dts_l[1] : a.dts_l[0];
+            if (k > 0) op = kw_prepare(oq, oqlat, alpha, dts, flen);
             o_tor_cum += tor * dts;
             double qin = 0.0;
             if (sfw > 0.0) qin = qin_all;
             double qn, ar;
-            kinematic_wave(qin, oq, oqlat, alpha, dts, flen, qn, ar);
+            kw_solve(qin, op, alpha, qn, ar);
             oq = qn;
             if (sfw > 0.0) oh = ar / sfw;
             o_storage = flen * sfw * oh;
@@ -754,13 +768,16 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         const double rqlat = r_inwater / rfl;
         double rq = d.r_q[rr], rh = d.r_h[rr], r_store_prev = d.r_storage[rr], r_storage = 0.0, rqin = 0.0;
         double r_q_cum = 0.0, r_qin_cum = 0.0;
+        double inflow = 0.0 / rfl;  // reservoir / lake / external inflow terms (none in this model)
+        inflow -= 0.0 / rfl;
+        const double rqlat_all = rqlat + inflow;
+        KwPrep rp = kw_prepare(rq, rqlat_all, ralpha, a.S_r == 1 ? a.dts_r[1] : a.dts_r[0], rfl);
         auto riv_compute = [&](int k, double sumq, double) -> double {
             const double dts = (k == a.S_r - 1) ? a.dts_r[1] : a.dts_r[0];
+            if (k > 0) rp = kw_prepare(rq, rqlat_all, ralpha, dts, rfl);
             const double qin = 0.0 + sumq;
-            double inflow = 0.0 / rfl;
-            inflow -= 0.0 / rfl;
             double qn, ar;
-            kinematic_wave(qin, rq, rqlat + inflow, ralpha, dts, rfl, qn, ar);
+            kw_solve(qin, rp, ralpha, qn, ar);
             rq = qn;
             rh = ar / rfw;
             r_storage = rfl * ar;
