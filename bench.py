@@ -36,6 +36,9 @@ def parse():
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_routing_4096.json"))
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="after the timed steps, run one device-to-device copy of this many MiB (a kernel of "
+                         "known traffic, for scaling hardware byte counters)")
     return ap.parse_args()
 
 
@@ -161,6 +164,15 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
     }
+    if args.calibration_copy_mib > 0:
+        nb = args.calibration_copy_mib << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# calibration copy: {nb} bytes read, {nb} bytes written", file=sys.stderr, flush=True)
+        del src, dst
     if not args.no_cpu_baseline:
         frames_host = []
         inv = np.empty_like(node)

@@ -227,6 +227,69 @@ def make_catchment(nx, ny=None, cell=1000.0, dt=86400.0, river_threshold=100, se
     return c
 
 
+def make_batch(members):
+    """Several catchments as one model: the members are stacked in a virtual raster (width =
+    the widest member, one empty row between members) so that no D8 direction can reach
+    another member; every member keeps its own cells, parameters and local flow network.
+    Cells stay member by member in each member's own order, so member m occupies cells
+    [cell_offsets[m], cell_offsets[m + 1]) and river cells [river_offsets[m],
+    river_offsets[m + 1]) of every per-cell or per-river-cell array. A member's results are
+    those of the member run on its own: basins exchange no water and the step's result does
+    not depend on the order in which basins are visited.
+
+    All members must share the time step, the soil-layer configuration and the snow and
+    infiltration-reduction switches (one model configuration)."""
+    m0 = members[0]
+    for c in members[1:]:
+        if (c.dt, c.nlayers_cfg, tuple(c.layers_mm), c.snow, c.infiltration_reduction) != \
+           (m0.dt, m0.nlayers_cfg, tuple(m0.layers_mm), m0.snow, m0.infiltration_reduction):
+            raise ValueError("batch members must share one model configuration")
+    width = max(c.nx for c in members)
+    names = list(m0.arrays.keys())
+    parts = {k: [] for k in names}
+    cell_off, riv_off, y0 = [0], [0], 0
+    for c in members:
+        n = c.n
+        if set(c.arrays.keys()) != set(names):
+            raise ValueError("batch members must provide the same maps")
+        for k in names:
+            a = c.arrays[k]
+            if k == "cell_index":
+                x = a % c.nx
+                y = a // c.nx
+                parts[k].append(((y + y0) * width + x).astype(np.int64))
+            elif k.startswith("river_") and k not in ("river_location", "river_width_land", "river_length_land"):
+                parts[k].append(a)
+            elif a.ndim == 1 and a.shape[0] == n:
+                parts[k].append(a)
+            elif a.ndim == 1 and a.shape[0] % n == 0:  # layer-major [N][n]
+                parts[k].append(a.reshape(-1, n))
+            else:
+                raise ValueError(k)
+        y0 += c.ny + 1
+        cell_off.append(cell_off[-1] + n)
+        riv_off.append(riv_off[-1] + c.nriv)
+    A = {}
+    for k in names:
+        if parts[k][0].ndim == 2:
+            A[k] = np.ascontiguousarray(np.concatenate(parts[k], axis=1).ravel())
+        else:
+            A[k] = np.ascontiguousarray(np.concatenate(parts[k]))
+    b = Catchment(width, y0 - 1, m0.cell, m0.dt, A, m0.nlayers_cfg, m0.layers_mm, m0.snow, m0.infiltration_reduction)
+    b.max_depth = max(c.max_depth for c in members)
+    b.cell_offsets = np.array(cell_off, np.int64)
+    b.river_offsets = np.array(riv_off, np.int64)
+    if all(hasattr(c, "depth") for c in members):
+        b.depth = np.concatenate([c.depth for c in members])
+    return b
+
+
+def forcing_batch_numpy(members, step, seed=0):
+    """Forcing of a batch: each member's own forcing_numpy, concatenated in member order."""
+    fs = [forcing_numpy(c, step, seed=seed + 7 * m) for m, c in enumerate(members)]
+    return tuple(np.ascontiguousarray(np.concatenate([f[i] for f in fs])) for i in range(3))
+
+
 def basin_ids(ldd, nx, ny):
     """Outlet (pit) id of every cell: cells with the same id form one closed basin."""
     n = nx * ny
