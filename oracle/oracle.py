@@ -100,8 +100,22 @@ def lib():
         L.wfo_soil_evaporation_unsaturated_store.argtypes = [_c, _c, _c, ctypes.c_int, _c, _c]
         L.wfo_soil_evaporation_saturated_store.restype = _c
         L.wfo_soil_evaporation_saturated_store.argtypes = [_c, ctypes.c_int, _c, _c, _c, _c]
+        L.wfo_math.argtypes = [ctypes.c_int32, _D, _D, _D, ctypes.c_int64]
         _lib = L
     return _lib
+
+
+MATH_OPS = {"exp": 0, "log": 1, "pow": 2, "cbrt": 3, "cld": 4, "round_sig12": 5, "min": 6, "max": 7, "pow10": 8}
+
+
+def math(op, x, y=None):
+    """Element-wise shared elementary function (include/wfh_math.h) over numpy arrays."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x) if y is None else np.ascontiguousarray(np.broadcast_to(y, x.shape), dtype=np.float64)
+    out = np.empty_like(x)
+    rc = lib().wfo_math(MATH_OPS[op], x.ctypes.data_as(_D), y.ctypes.data_as(_D), out.ctypes.data_as(_D), x.size)
+    assert rc == 0
+    return out
 
 
 def _out(k):

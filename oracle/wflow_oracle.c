@@ -1597,3 +1597,21 @@ int32_t wfo_set(wfo_model* m, int32_t var, const double* src, int64_t count) {
     default: return WFH_ERR_ARG;
     }
 }
+
+int32_t wfo_math(int32_t op, const double* x, const double* y, double* out, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        switch (op) {
+            case 0: out[i] = wfh_exp(x[i]); break;
+            case 1: out[i] = wfh_log(x[i]); break;
+            case 2: out[i] = wfh_pow(x[i], y[i]); break;
+            case 3: out[i] = wfh_cbrt(x[i]); break;
+            case 4: out[i] = wfh_cld(x[i], y[i]); break;
+            case 5: out[i] = wfh_round_sig12(x[i]); break;
+            case 6: out[i] = wfh_min(x[i], y[i]); break;
+            case 7: out[i] = wfh_max(x[i], y[i]); break;
+            case 8: out[i] = wfh_pow10i((int)x[i]); break;
+            default: return -1;
+        }
+    }
+    return 0;
+}

@@ -60,6 +60,12 @@ int32_t wfo_topology(const wfo_model* m, int64_t* land_order, int64_t* land_down
                      int64_t* river_down, int64_t* river_land_index);
 int64_t wfo_kinwave_iterations(const wfo_model* m);
 
+/* Element-wise evaluation of the shared elementary functions (include/wfh_math.h) for the
+ * accuracy tests: op 0 exp, 1 log, 2 pow(x, y), 3 cbrt, 4 cld(x, y), 5 round(x; sigdigits=12),
+ * 6 Julia min(x, y), 7 Julia max(x, y), 8 10.0^x for integral x >= 0. y is ignored by the
+ * one-argument functions. */
+int32_t wfo_math(int32_t op, const double* x, const double* y, double* out, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif
