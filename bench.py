@@ -15,6 +15,7 @@ a CPU baseline (the FP64 oracle timed on this host over a bounded sample of the 
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -30,6 +31,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=("grid", "batch"), default="grid",
+                    help="grid: one nx^2 catchment at dx 250 m; batch: 64 catchments of log-uniform size "
+                         "(1e4 .. 2e6 cells, dx 1 km, river threshold 100 cells) in one model")
+    ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--nx", type=int, default=4096)
     ap.add_argument("--frames", type=int, default=24, help="distinct forcing frames kept in HBM")
     ap.add_argument("--cpu-sample-cells", type=int, default=400_000)
@@ -51,7 +56,7 @@ def cpu_baseline(catchment, frames_host, args):
     import oracle as O
     from wflowhip import synth
 
-    lab = synth.basin_ids(catchment.arrays["ldd"], catchment.nx, catchment.ny)
+    lab = synth.basin_ids(catchment.arrays["ldd"], catchment.nx, catchment.ny, catchment.arrays["cell_index"])
     ids, counts = np.unique(lab, return_counts=True)
     rng = np.random.default_rng(5)
     order = rng.permutation(len(ids))
@@ -78,7 +83,7 @@ def cpu_baseline(catchment, frames_host, args):
         "unit": "cell-timesteps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{len(take)} complete basins ({sub.n} cells) of the same 4096^2 hourly catchment, "
+        "sample": f"{len(take)} complete basins ({sub.n} cells) of the same workload, "
                   f"{args.cpu_sample_steps} steps, single-threaded C oracle (-O2), update_model! only",
     }
 
@@ -96,8 +101,20 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     t0 = time.time()
-    c = synth.make_catchment(args.nx, cell=250.0, dt=3600.0, river_threshold=2000, seed=1)
-    print(f"# catchment {args.nx}^2: n={c.n} nriv={c.nriv} longest path={c.max_depth} "
+    if args.workload == "grid":
+        c = synth.make_catchment(args.nx, cell=250.0, dt=3600.0, river_threshold=2000, seed=1)
+        workload = (f"{args.nx}x{args.nx} cells, dx 250 m, hourly, river threshold 2000 cells, "
+                    "sbm + kinematic wave (land 1, river 4, subsurface 1 sub-steps)")
+    else:
+        rng = np.random.default_rng(2)
+        sizes = np.exp(rng.uniform(np.log(1e4), np.log(2e6), args.members))
+        members = [synth.make_catchment(int(round(math.sqrt(s))), cell=1000.0, dt=3600.0, river_threshold=100,
+                                        seed=1000 + m) for m, s in enumerate(sizes)]
+        c = synth.make_batch(members)
+        workload = (f"{args.members} catchments in one model, {min(m.n for m in members)} .. "
+                    f"{max(m.n for m in members)} cells each (log-uniform), dx 1 km, hourly, river threshold "
+                    "100 cells, sbm + kinematic wave (land 1, river 4, subsurface 1 sub-steps)")
+    print(f"# {workload}: n={c.n} nriv={c.nriv} longest path={c.max_depth} "
           f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     m = Model(c, water_mass_balance=True)
     print(f"# model created ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
@@ -144,7 +161,8 @@ def main():
         except Exception:
             traffic = None
     result = {
-        "metric": "cell-timesteps/s FP64 SBM+kinwave (plain GPU step)",
+        "metric": "cell-timesteps/s FP64 SBM+kinwave (plain GPU step)" if args.workload == "grid" else
+                  "cell-timesteps/s FP64 SBM+kinwave (batched GPU step)",
         "value": c.n * args.steps / elapsed,
         "unit": "cell-timesteps/s",
         "n_gpus": 1,
@@ -156,8 +174,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (fractal DEM, seeded parameters, GPU-generated forcing)",
-        "config": {"workload": f"{args.nx}x{args.nx} cells, dx 250 m, hourly, river threshold 2000 cells, "
-                               "sbm + kinematic wave (land 1, river 4, subsurface 1 sub-steps)",
+        "config": {"workload": workload,
                    "cells": c.n, "river_cells": c.nriv, "longest_flow_path": c.max_depth,
                    "kernel_ms": {"k_vertical": k1_ms, "k_route": k2_ms},
                    "kernel_bytes": {"k_vertical": k1_bytes, "k_route": k2_bytes}},

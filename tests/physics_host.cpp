@@ -1,0 +1,12 @@
+// Host build of the kernels' per-cell physics (wflow.jl_amd/csrc/wfh_physics.h) for CPU tests:
+// the same source the HIP kernels compile, evaluated on the host so that it can be compared
+// with the oracle without a GPU. Built by the tests with g++ -ffp-contract=off.
+#include "wfh_physics.h"
+
+extern "C" void wfh_host_kinematic_wave(const double* in, double* out, long n) {
+    // in: n rows of (q_in, q_prev, q_lat, alpha, dt, dx); out: n rows of (q, area)
+    for (long i = 0; i < n; ++i) {
+        const double* r = in + 6 * i;
+        wfh::kinematic_wave(r[0], r[1], r[2], r[3], r[4], r[5], out[2 * i], out[2 * i + 1]);
+    }
+}

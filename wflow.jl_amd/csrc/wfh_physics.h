@@ -259,14 +259,30 @@ WFH_HD void kw_solve(double q_in, const KwPrep& p, double alpha, double& q, doub
     const double C = dt_dx * q_in + p.a_u3 + p.lat;
     double u = p.u_prev > 0.0 ? p.u_prev : wfh_cbrt(C / alpha);
     const double c1 = 5.0 * dt_dx, c2 = 3.0 * alpha;
+    // The reference stops on |f| <= 1e-12 or after 3000 updates. For large C (|C| above about
+    // 1e4) the rounding noise of f exceeds 1e-12, the test can never pass and the iteration
+    // settles on a fixed point or a two-cycle of adjacent doubles until the limit. The update
+    // is a fixed function of u (every other input is constant), so once a value repeats the
+    // rest of the sequence is known: the value after update 3000 is taken from the cycle
+    // instead of running the remaining updates. Every residual test the shortcut skips would
+    // have evaluated one of the cycle's values, all of which already failed it.
+    double u_pp = NaN_;  // the value two updates back
     for (int k = 0; k < 3000; ++k) {
         const double u2 = u * u;
         const double u3 = u2 * u;
         const double f_u = u3 * (dt_dx * u2 + alpha) - C;
         if (wfh_fabs(f_u) <= 1.0e-12) break;
         const double df = u2 * (c1 * u2 + c2);
+        const double u_c = u;
         u -= f_u / df;
         if (wfh_isnan(u) || u <= 0.0) u = WFH_KIN_WAVE_MIN_FLOW_QROOT;
+        const int n = k + 1;  // updates done; u is the n-th iterate
+        if (u == u_c) break;  // fixed point: every later iterate is u
+        if (u == u_pp) {      // two-cycle u_c, u, u_c, u, ...: iterate 3000 is u if 3000 - n is even
+            if ((3000 - n) & 1) u = u_c;
+            break;
+        }
+        u_pp = u_c;
     }
     u = wfh_max(u, WFH_KIN_WAVE_MIN_FLOW_QROOT);
     const double u3 = u * u * u;
@@ -422,14 +438,15 @@ WFH_HD bool kinematic_wave_ssf(double q_in, double q_prev, double zi_prev, doubl
         q = wfh_max(q - q_excess, WFH_KIN_WAVE_MIN_FLOW);
     }
     zi = wfh_clamp(zi, 0.0, d);
-    // its = Int(ceil(round(x; sigdigits = 12))). Rounding to 12 significant digits moves x by
-    // less than 1e-11 of its value, so every x <= 0.999 gives its <= 1 (the single-step
-    // branch); only larger x need the exact count.
-    const double x = wfh_fabs(zi - zi_prev) / 0.1;
+    // its = Int(ceil(round(x; sigdigits = 12))), x = |zi - zi_prev| / 0.1. Rounding to 12
+    // significant digits moves x by less than 1e-11 of its value, so every |zi - zi_prev| <=
+    // 0.0998 (x <= 0.998 up to one rounding) gives its <= 1, the single-step branch; only
+    // larger changes need x and the exact count.
+    const double adz = wfh_fabs(zi - zi_prev);
     double its_f = 1.0;
     long its = 1;
-    if (x > 0.999) {
-        its_f = WFH_CEIL(wfh_round_sig12(x));
+    if (adz > 0.0998) {
+        its_f = WFH_CEIL(wfh_round_sig12(adz / 0.1));
         its = (long)its_f;
     }
     bool deferred = false;

@@ -472,6 +472,7 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
         bool took_any = false;
         for (int pass = 0; pass < 9; ++pass) {
             const bool pend = act && e < e1;
+            if (!__any(pend)) break;  // nothing left to fold: skip the shuffles of an empty pass
             const int j = e - e0;
             int u = c * WAVE;
             double f = 0.0;

@@ -290,13 +290,19 @@ def forcing_batch_numpy(members, step, seed=0):
     return tuple(np.ascontiguousarray(np.concatenate([f[i] for f in fs])) for i in range(3))
 
 
-def basin_ids(ldd, nx, ny):
-    """Outlet (pit) id of every cell: cells with the same id form one closed basin."""
-    n = nx * ny
+def basin_ids(ldd, nx, ny, cell_index=None):
+    """Outlet (pit) id of every cell: cells with the same id form one closed basin. Cells sit at
+    raster positions cell_index (default: every cell of the nx x ny raster, in order); the
+    generator's directions never leave the active cells."""
+    n = ldd.shape[0]
     dxs = np.array([0, -1, 0, 1, -1, 0, 1, -1, 0, 1])
     dys = np.array([0, -1, -1, -1, 0, 0, 0, 1, 1, 1])
     idx = np.arange(n)
-    down = np.where(ldd == 5, idx, (idx % nx + dxs[ldd]) + (idx // nx + dys[ldd]) * nx)
+    pos = idx if cell_index is None else np.asarray(cell_index, np.int64)
+    tgt = (pos % nx + dxs[ldd]) + (pos // nx + dys[ldd]) * nx
+    if cell_index is not None:
+        tgt = np.searchsorted(pos, tgt).clip(0, n - 1)
+    down = np.where(ldd == 5, idx, tgt)
     lab = down.copy()
     for _ in range(64):  # pointer jumping: log2(path length) rounds
         nxt = lab[lab]
