@@ -42,8 +42,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_routing_4096.json"))
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
-                    help="after the timed steps, run one device-to-device copy of this many MiB (a kernel of "
-                         "known traffic, for scaling hardware byte counters)")
+                    help="after the timed steps, run one elementwise kernel over this many MiB (reads and "
+                         "writes each byte once: known traffic, for scaling hardware byte counters)")
     return ap.parse_args()
 
 
@@ -186,9 +186,9 @@ def main():
         src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
-        dst.copy_(src)
+        torch.neg(src, out=dst)  # one elementwise kernel: reads nb bytes, writes nb bytes
         torch.cuda.synchronize()
-        print(f"# calibration copy: {nb} bytes read, {nb} bytes written", file=sys.stderr, flush=True)
+        print(f"# calibration kernel (neg): {nb} bytes read, {nb} bytes written", file=sys.stderr, flush=True)
         del src, dst
     if not args.no_cpu_baseline:
         frames_host = []
