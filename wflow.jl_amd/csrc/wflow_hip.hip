@@ -419,10 +419,16 @@ __device__ __forceinline__ void fold_in(double& acc1, double& acc2, double q, do
 // One family of routing sub-steps (all subsurface, all overland or all river sub-steps of the
 // model step) for the 64 cells of a wave. Every lane runs the same code; the wave advances a
 // diagonal wavefront over (cell, sub-step): a lane computes sub-step k once all upstream cells
-// have produced their sub-step-k outflow and its in-wave downstream cell has consumed its
-// sub-step k-1 outflow. Outflow to an in-wave consumer stays in a register and travels by lane
-// shuffle (a producer holding sub-step k sits at k+1 until the consumer moves on); outflow to
-// a consumer in another wave goes through its global slot. Returns false when the wave has to
+// have produced their sub-step-k outflow and, for multi-step families, its in-wave downstream
+// cell has computed sub-step k-1 (so the outflow it is about to replace has been used).
+//
+// In-wave exchange goes through a 64-entry table in LDS: each round every lane writes its
+// (next sub-step, last outflow) pair, then reads the entries of its in-wave upstream cells with
+// independent loads and keeps the values that belong to its current sub-step. Outflow to a
+// consumer in another wave goes through that consumer's global slot (one per edge and
+// sub-step); such slots are polled once per round and picked up in the next round, so the poll
+// latency overlaps the round's work. The upstream outflow is folded in ascending edge order,
+// the reference's sum_at order, as far as it has arrived. Returns false when the wave has to
 // stop (a wait exceeded its limit, or another wave raised the abort flag).
 template <int FOLD, class Compute>
 __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, const int c, const bool part,
@@ -430,11 +436,9 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
                                            const int* __restrict__ up_slot, const double* __restrict__ up_f,
                                            const int e0, const int e1, uint64_t* slots, const size_t stride,
                                            const int my_slot, const bool pub_cross, const int cons_lane,
-                                           Compute& compute) {
-    // The first PF upstream edges live in registers. Their cross-wave outflow is polled at the
-    // top of an iteration and picked up after the compute block, so the poll latency overlaps
-    // the work of lanes that are ready; the in-order fold reads it from `buf`. Edges beyond PF
-    // (rare: more than PF upstream cells) are polled in order from memory.
+                                           int* lds_k, double* lds_q, Compute& compute) {
+    // The first PF upstream edges are buffered in registers; further edges (a cell with more
+    // than PF upstream cells) are taken one per round in order, after the buffered ones.
     constexpr int PF = 4;
     int k = part ? 0 : S;
     int e = e0;
@@ -442,86 +446,92 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
     const int npf = (e1 - e0) < PF ? (e1 - e0) : PF;
     int en[PF], es[PF];
     double ef[PF], buf[PF];
+    uint64_t v[PF];
+    bool pol[PF];
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
-        en[j] = c * WAVE;  // in-wave dummy
+        en[j] = c * WAVE;
         es[j] = 0;
         ef[j] = 0.0;
         buf[j] = 0.0;
+        v[j] = SLOT_EMPTY;
+        pol[j] = false;
         if (j < npf) {
             en[j] = up_node[e0 + j];
             es[j] = up_slot[e0 + j];
             if (FOLD != FOLD_RIV) ef[j] = up_f[e0 + j];
         }
     }
-    unsigned have = 0;  // bit j: buf[j] holds the outflow of edge e0 + j for the current sub-step
+    unsigned have = 0;  // bit j: buf[j] holds edge e0 + j's outflow for sub-step k
     long spin = 0;
     while (__any(k < S)) {
         const bool act = k < S;
-        // (1) issue the polls
-        uint64_t v[PF];
-        bool pol[PF];
+        // (1) publish this lane's state to the wave's table
+        lds_k[lane] = k;
+        lds_q[lane] = qv;
+        __builtin_amdgcn_wave_barrier();
+        // (2) pick up last round's polls; gather the in-wave upstream outflow that is ready
+        bool got = false;
 #pragma unroll
         for (int j = 0; j < PF; ++j) {
-            pol[j] = act && j < npf && (en[j] >> 6) != c && !(have & (1u << j)) && e0 + j >= e;
+            if (pol[j] && v[j] != SLOT_EMPTY) {
+                __hip_atomic_store(slots + (size_t)k * stride + es[j], SLOT_EMPTY, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                buf[j] = wfh_from_bits(v[j]);
+                have |= 1u << j;
+                got = true;
+            }
+            const bool inw = (en[j] >> 6) == c;
+            if (act && j < npf && inw && !(have & (1u << j))) {
+                const int src = en[j] & (WAVE - 1);
+                if (lds_k[src] == k + 1) {
+                    buf[j] = lds_q[src];
+                    have |= 1u << j;
+                    got = true;
+                }
+            }
+        }
+        // (3) issue this round's polls for the cross-wave edges still missing
+#pragma unroll
+        for (int j = 0; j < PF; ++j) {
+            pol[j] = act && j < npf && (en[j] >> 6) != c && !(have & (1u << j));
             v[j] = SLOT_EMPTY;
             if (pol[j])
                 v[j] = __hip_atomic_load(slots + (size_t)k * stride + es[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        // (2) in-order fold of what is available
-        bool took_any = false;
-        for (int pass = 0; pass < 9; ++pass) {
-            const bool pend = act && e < e1;
-            if (!__any(pend)) break;  // nothing left to fold: skip the shuffles of an empty pass
-            const int j = e - e0;
-            int u = c * WAVE;
-            double f = 0.0;
-            if (pend) {
-                if (j < PF) {
+        // (4) fold the buffered prefix in edge order, then at most one further edge
+        if (act) {
 #pragma unroll
-                    for (int jj = 0; jj < PF; ++jj)
-                        if (jj == j) {
-                            u = en[jj];
-                            f = ef[jj];
-                        }
-                } else {
-                    u = up_node[e];
-                    if (FOLD != FOLD_RIV) f = up_f[e];
+            for (int j = 0; j < PF; ++j)
+                if (e == e0 + j && (have & (1u << j))) {
+                    fold_in<FOLD>(acc1, acc2, buf[j], ef[j]);
+                    ++e;
                 }
-            }
-            const bool inw = pend && (u >> 6) == c;
-            const int src = inw ? (u & 63) : lane;
-            const int ku = __shfl(k, src);
-            const double qu = __shfl(qv, src);
-            bool took = false;
-            double q = 0.0;
-            if (pend) {
-                if (inw) {
-                    if (ku == k + 1) {
-                        q = qu;
+            if (e >= e0 + PF && e < e1) {
+                const int u = up_node[e];
+                const double f = (FOLD != FOLD_RIV) ? up_f[e] : 0.0;
+                double q = 0.0;
+                bool took = false;
+                if ((u >> 6) == c) {
+                    if (lds_k[u & (WAVE - 1)] == k + 1) {
+                        q = lds_q[u & (WAVE - 1)];
                         took = true;
                     }
-                } else if (j < PF) {
-#pragma unroll
-                    for (int jj = 0; jj < PF; ++jj)
-                        if (jj == j && (have & (1u << jj))) {
-                            q = buf[jj];
-                            took = true;
-                        }
                 } else {
                     took = slot_take(slots + (size_t)k * stride + up_slot[e], q);
                 }
                 if (took) {
                     fold_in<FOLD>(acc1, acc2, q, f);
                     ++e;
+                    got = true;
                 }
             }
-            if (!__any(took)) break;
-            took_any = true;
         }
-        // (3) compute the ready lanes
-        const int kc = __shfl(k, cons_lane >= 0 ? cons_lane : lane);
-        const bool can = act && e == e1 && (cons_lane < 0 || kc >= k);
+        // (5) compute the ready lanes
+        bool bp = true;
+        if (S > 1 && cons_lane >= 0) bp = lds_k[cons_lane] >= k;
+        const bool can = act && e == e1 && bp;
+        __builtin_amdgcn_wave_barrier();
         if (can) {
             qv = compute(k, acc1, acc2);
             if (pub_cross) slot_put(slots + (size_t)k * stride + my_slot, qv);
@@ -531,19 +541,7 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
             acc2 = 0.0;
             have = 0;
         }
-        // (4) pick up the polled outflow (a lane that polled cannot have computed above)
-        bool fetched = false;
-#pragma unroll
-        for (int j = 0; j < PF; ++j) {
-            if (pol[j] && v[j] != SLOT_EMPTY) {
-                __hip_atomic_store(slots + (size_t)(k) * stride + es[j], SLOT_EMPTY, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                buf[j] = wfh_from_bits(v[j]);
-                have |= 1u << j;
-                fetched = true;
-            }
-        }
-        if (__any(can) || took_any || __any(fetched)) {
+        if (__any(can) || __any(got)) {
             spin = 0;
         } else {
             __builtin_amdgcn_s_sleep(1);
@@ -556,12 +554,17 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
             }
         }
     }
+    __builtin_amdgcn_wave_barrier();
     return true;
 }
 
 template <int N>
 __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
+    __shared__ int x_k[ROUTE_BLOCK];     // per-wave exchange tables of the routing families
+    __shared__ double x_q[ROUTE_BLOCK];
+    int* const lds_k = x_k + (threadIdx.x & ~(WAVE - 1));
+    double* const lds_q = x_q + (threadIdx.x & ~(WAVE - 1));
     if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[a.parity ^ 1] = 0ull;  // next launch's counter
     const int n = d.n;
     const double dt = d.dt;
@@ -628,7 +631,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             return qn;
         };
         if (!run_family<FOLD_SSF>(a, lane, c, valid, a.S_s, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ssf,
-                                  (size_t)n, p, (fl & 1) && !cons_in, cons_in ? (dn & 63) : -1, ssf_compute))
+                                  (size_t)n, p, (fl & 1) && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q, ssf_compute))
             return;
         if (col_upd) update_ustorelayerdepth<N>(col, col_z0, col_z1);
 
@@ -739,7 +742,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             return qn;
         };
         if (!run_family<FOLD_OL>(a, lane, c, valid, a.S_l, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ol, (size_t)n,
-                                 p, (fl & 1) && !cons_in, cons_in ? (dn & 63) : -1, ol_compute))
+                                 p, (fl & 1) && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q, ol_compute))
             return;
         double r_inwater = 0.0, o_tor_avg = 0.0;
         if (valid) {
@@ -789,7 +792,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         };
         if (!run_family<FOLD_RIV>(a, lane, c, riv, a.S_r, d.rup_node, d.rup_idx, nullptr, re0, re1, a.slot_r,
                                   (size_t)d.nriv, rr, riv && (fl & 4) && !rcons_in, rcons_in ? (dn & 63) : -1,
-                                  riv_compute))
+                                  lds_k, lds_q, riv_compute))
             return;
         if (riv) {
             const double q_avg = r_q_cum / dt, qin_avg = r_qin_cum / dt;
