@@ -470,26 +470,27 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
         lds_k[lane] = k;
         lds_q[lane] = qv;
         __builtin_amdgcn_wave_barrier();
-        // (2) pick up last round's polls; gather the in-wave upstream outflow that is ready
+        // (2) pick up last round's polls; gather the in-wave upstream outflow that is ready. The
+        // table entries of all PF producers are read unconditionally (unused edges point at
+        // lane 0), so the reads overlap and one wait covers them.
+        int tk[PF];
+        double tq[PF];
+#pragma unroll
+        for (int j = 0; j < PF; ++j) {
+            tk[j] = lds_k[en[j] & (WAVE - 1)];
+            tq[j] = lds_q[en[j] & (WAVE - 1)];
+        }
         bool got = false;
 #pragma unroll
         for (int j = 0; j < PF; ++j) {
-            if (pol[j] && v[j] != SLOT_EMPTY) {
+            const bool from_slot = pol[j] && v[j] != SLOT_EMPTY;
+            if (from_slot)
                 __hip_atomic_store(slots + (size_t)k * stride + es[j], SLOT_EMPTY, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
-                buf[j] = wfh_from_bits(v[j]);
-                have |= 1u << j;
-                got = true;
-            }
-            const bool inw = (en[j] >> 6) == c;
-            if (act && j < npf && inw && !(have & (1u << j))) {
-                const int src = en[j] & (WAVE - 1);
-                if (lds_k[src] == k + 1) {
-                    buf[j] = lds_q[src];
-                    have |= 1u << j;
-                    got = true;
-                }
-            }
+            const bool from_wave = act && j < npf && (en[j] >> 6) == c && !(have & (1u << j)) && tk[j] == k + 1;
+            buf[j] = from_slot ? wfh_from_bits(v[j]) : (from_wave ? tq[j] : buf[j]);
+            have |= (from_slot || from_wave) ? (1u << j) : 0u;
+            got = got || from_slot || from_wave;
         }
         // (3) issue this round's polls for the cross-wave edges still missing
 #pragma unroll
@@ -528,8 +529,8 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
             }
         }
         // (5) compute the ready lanes
-        bool bp = true;
-        if (S > 1 && cons_lane >= 0) bp = lds_k[cons_lane] >= k;
+        const int kcons = lds_k[cons_lane & (WAVE - 1)];
+        const bool bp = S == 1 || cons_lane < 0 || kcons >= k;
         const bool can = act && e == e1 && bp;
         __builtin_amdgcn_wave_barrier();
         if (can) {
@@ -590,6 +591,23 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         const double sy = ld(d.sy, q_), slope = ld(d.slope, q_), flen = ld(d.fl, q_), fw = ld(d.fw, q_);
         const double qmax = ld(d.qmax, q_), kh0 = ld(d.kh0, q_), fk = ld(d.f, q_);
         const double rech = d.recharge[q_];
+        // Inputs of the later phases (soil storage, balances, overland, river, total storage),
+        // loaded with the chunk's first loads instead of one dependent round trip per phase.
+        // Every one of these arrays is written, in this kernel, only by the lane that reads it,
+        // after the read.
+        const double rd = ld(d.rd, q_);
+        const double sat_exc = d.sat_excess[q_], runoff_l = d.runoff_land[q_], inf_exc = d.inf_excess[q_];
+        const double aeow = d.aeow_l[q_], nrr = d.net_runoff_river[q_], canopy = d.canopy[q_];
+        const double P_in = d.balance ? d.P[q_] : 0.0, aet = d.balance ? d.aet[q_] : 0.0;
+        const double leak = d.balance ? d.leakage[q_] : 0.0, store_prev = d.balance ? d.store_prev[q_] : 0.0;
+        const double snow_s = d.snow ? d.snow_s[q_] : 0.0, snow_w = d.snow ? d.snow_w[q_] : 0.0;
+        const double alpha = ld(d.alpha, q_), sfw = ld(d.sfw, q_), rfrac = ld(d.rf, q_);
+        const double oq0 = d.ol_q[q_], oh0 = d.ol_h[q_], o_store_prev = d.ol_storage[q_];
+        const bool riv = r >= 0;
+        const int rr = riv ? r : 0;
+        const int re0 = d.rup_ptr[rr], re1 = riv ? d.rup_ptr[rr + 1] : re0;
+        const double rfl = d.r_fl[rr], rfw = d.r_fw[rr], ralpha = d.r_alpha[rr], rarea = d.r_area[rr];
+        const double rq0 = d.r_q[rr], rh0 = d.r_h[rr], r_store_prev = d.r_storage[rr];
         Column<N> col;
         soil_layers<N>(dsoil, d.cfg_cum, d.cfg_thick, col.act, col.cum);
 #pragma unroll
@@ -650,7 +668,6 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             d.ssf_q_avg[p] = s_q_avg;
             d.ssf_tor_avg[p] = s_tor_avg;
             d.ssf_storage[p] = sy * (dsoil - szi) * area;
-            const double rd = ld(d.rd, p);
             {
                 bool first = true;
 #pragma unroll
@@ -660,7 +677,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
                         first = false;
                     }
             }
-            const double sbm_runoff = wfh_max(0.0, s_exf_avg + d.sat_excess[p] + d.runoff_land[p] + d.inf_excess[p]);
+            const double sbm_runoff = wfh_max(0.0, s_exf_avg + sat_exc + runoff_l + inf_exc);
             int nl = 0;
 #pragma unroll
             for (int k = 0; k < N; ++k) nl += !wfh_isnan(col.act[k]);
@@ -690,7 +707,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             d.runoff[p] = sbm_runoff;
             d.rzs[p] = rzs;
             d.vwc_root[p] = rzs / rd + tr;
-            net_runoff = sbm_runoff - d.aeow_l[p];
+            net_runoff = sbm_runoff - aeow;
             d.net_runoff[p] = net_runoff;
             d.zi[p] = col.zi;
 #pragma unroll
@@ -703,12 +720,12 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
                     fout -= rech_avg;
                 d.bal_ssf[p] = (s_qin_avg + fin) - (s_q_avg + fout) - s_qnet_avg;
                 // land balance  mass_balance.jl:215-253
-                const double t_in = s_qin_avg / area + (d.P[p] + 0.0) + 0.0 + 0.0 / area;
-                const double vout = net_runoff + d.aet[p] + d.net_runoff_river[p] + d.leakage[p] + 0.0;
+                const double t_in = s_qin_avg / area + (P_in + 0.0) + 0.0 + 0.0 / area;
+                const double vout = net_runoff + aet + nrr + leak + 0.0;
                 const double t_out = s_q_avg / area + vout + 0.0 + 0.0 / area;
-                const double snow = d.snow ? d.snow_s[p] + d.snow_w[p] : 0.0 + 0.0;
-                const double storage = (swd + usd) + d.canopy[p] + snow + 0.0 * 0.0 + 0.0;
-                const double rate = (storage - d.store_prev[p]) / dt;
+                const double snow = d.snow ? snow_s + snow_w : 0.0 + 0.0;
+                const double storage = (swd + usd) + canopy + snow + 0.0 * 0.0 + 0.0;
+                const double rate = (storage - store_prev) / dt;
                 const double err = t_in - t_out - rate;
                 const double avg = (t_in + t_out) / 2.0;
                 d.bal_land[p] = err;
@@ -720,9 +737,8 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         }
 
         // ---------------- overland kinematic wave ----------------
-        const double alpha = ld(d.alpha, q_), sfw = ld(d.sfw, q_);
         const double oqlat = o_inwater / flen;
-        double oq = d.ol_q[q_], oh = d.ol_h[q_], o_store_prev = d.ol_storage[q_], o_storage = 0.0, oqin = 0.0;
+        double oq = oq0, oh = oh0, o_storage = 0.0, oqin = 0.0;
         double o_q_cum = 0.0, o_qin_cum = 0.0, o_tor_cum = 0.0;
         KwPrep op = kw_prepare(oq, oqlat, alpha, a.S_l == 1 ? a.dts_l[1] : a.dts_l[0], flen);
         auto ol_compute = [&](int k, double tor, double qin_all) -> double {
@@ -758,19 +774,15 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             d.ol_qin_avg[p] = qin_avg;
             if (d.balance) d.bal_ol[p] = (o_inwater + qin_avg) - q_avg - (o_storage - o_store_prev) / dt;
             if (r >= 0) {  // river lateral inflow  surface_kinwave.jl:710-734
-                r_inwater = (s_tor_avg + o_tor_avg + d.net_runoff_river[p] * area) + 0.0 * d.r_area[r];
+                r_inwater = (s_tor_avg + o_tor_avg + nrr * area) + 0.0 * rarea;
                 d.r_inwater[r] = r_inwater;
             }
         }
 
         // ---------------- river kinematic wave ----------------
-        const bool riv = r >= 0;
-        const int rr = riv ? r : 0;
-        const int re0 = d.rup_ptr[rr], re1 = riv ? d.rup_ptr[rr + 1] : re0;
         const bool rcons_in = riv && (fl & 4) && cons_in;  // a river cell drains to the land downstream
-        const double rfl = d.r_fl[rr], rfw = d.r_fw[rr], ralpha = d.r_alpha[rr];
         const double rqlat = r_inwater / rfl;
-        double rq = d.r_q[rr], rh = d.r_h[rr], r_store_prev = d.r_storage[rr], r_storage = 0.0, rqin = 0.0;
+        double rq = rq0, rh = rh0, r_storage = 0.0, rqin = 0.0;
         double r_q_cum = 0.0, r_qin_cum = 0.0;
         double inflow = 0.0 / rfl;  // reservoir / lake / external inflow terms (none in this model)
         inflow -= 0.0 / rfl;
@@ -811,10 +823,10 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         if (valid) {
             double tot = 0.0;
             if (riv) tot = (rh * rfw * rfl) / (area);
-            const double ss = d.snow ? d.snow_s[p] : 0.0, sw = d.snow ? d.snow_w[p] : 0.0;
-            tot = tot + ((((ss + sw) + 0.0 * 0.0) + d.canopy[p]) + 0.0);
+            const double ss = d.snow ? snow_s : 0.0, sw = d.snow ? snow_w : 0.0;
+            tot = tot + ((((ss + sw) + 0.0 * 0.0) + canopy) + 0.0);
             const double sub = usd + swd;
-            const double lat = oh * (1.0 - ld(d.rf, p));
+            const double lat = oh * (1.0 - rfrac);
             tot += sub + lat;
             d.total_storage[p] = tot;
         }
