@@ -47,6 +47,12 @@ constexpr uint64_t SLOT_EMPTY = 0xFFF40000DEADBEEFULL;  // signalling NaN, never
 constexpr int WAVE = 64;
 constexpr int ROUTE_BLOCK = 256;
 constexpr long SPIN_LIMIT = 2000000;  // polls without progress (each with a sleep) before a wave gives up
+// true: the routing kernel performs each cell's vertical update itself at the start of the
+// cell's chunk (one launch per step) instead of a separate vertical kernel before it. Measured
+// on MI355X (4096^2 hourly and the 64-catchment batch) the separate kernel is faster: it runs
+// at five waves per SIMD, while the routing kernel's one wave per SIMD leaves the vertical
+// update's latency exposed.
+constexpr bool FUSE_VERTICAL = false;
 
 struct Prm {  // parameter map, broadcast when uniform
     const double* p;
@@ -116,20 +122,29 @@ struct RouteArgs {
 // ============================================================================================
 // K1: vertical column update (update_land_hydrology_model!, Wflow/src/sbm.jl:82-132)
 // ============================================================================================
+// The values of one cell's vertical update that the routing part of the step reads next.
 template <int N>
-__global__ void __launch_bounds__(256) k_vertical(Dev d) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.n) return;
+struct VertOut {
+    double us[N];
+    double zi, recharge, sat_exc, runoff_l, inf_exc, aeow_l, nrr, aet, leak, canopy, snow_s, snow_w, store_prev, P;
+};
+
+// The vertical update of cell i (node order): reads the step's forcing and the cell's states,
+// writes every state and flux of the vertical part back, and returns what routing reads.
+template <int N>
+__device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut<N>& o) {
     const int n = d.n;
     const double dt = d.dt;
     const double P = d.P[i], E = d.E[i], T = d.T[i];
 
     // storage_prev! (Wflow/src/mass_balance.jl:96-115,170-183) from last step's states
     double canopy = d.canopy[i];
-    double ss = d.snow_s[i], sw = d.snow_w[i];
+    double ss = d.snow ? d.snow_s[i] : 0.0, sw = d.snow ? d.snow_w[i] : 0.0;
+    double store_prev = 0.0;
     if (d.balance) {
         const double snow = d.snow ? ss + sw : 0.0 + 0.0;
-        d.store_prev[i] = d.tsws[i] + canopy + snow + 0.0 * 0.0 + 0.0;
+        store_prev = d.tsws[i] + canopy + snow + 0.0 * 0.0 + 0.0;
+        d.store_prev[i] = store_prev;
     }
 
     // ---- interception: canopy.jl:54-98 (Gash, dt >= 23 h) / :115-143 (modified Rutter) ----
@@ -381,6 +396,30 @@ __global__ void __launch_bounds__(256) k_vertical(Dev d) {
     d.net_runoff_river[i] = nrr;
     d.runoff_land[i] = rl;
     d.aeow_l[i] = aol;
+#pragma unroll
+    for (int k = 0; k < N; ++k) o.us[k] = us[k];
+    o.zi = zi;
+    o.recharge = recharge;
+    o.sat_exc = sat_exc;
+    o.runoff_l = rl;
+    o.inf_exc = inf_exc;
+    o.aeow_l = aol;
+    o.nrr = nrr;
+    o.aet = aet;
+    o.leak = leak;
+    o.canopy = canopy;
+    o.snow_s = ss;
+    o.snow_w = sw;
+    o.store_prev = store_prev;
+    o.P = P;
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) k_vertical(Dev d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.n) return;
+    VertOut<N> o;
+    vertical_cell<N>(d, i, o);
 }
 
 // ============================================================================================
@@ -465,7 +504,7 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
     unsigned have = 0;  // bit j: buf[j] holds edge e0 + j's outflow for sub-step k
     long spin = 0;
     while (__any(k < S)) {
-        const bool act = k < S;
+        bool act = k < S;
         // (1) publish this lane's state to the wave's table
         lds_k[lane] = k;
         lds_q[lane] = qv;
@@ -528,21 +567,53 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
                 }
             }
         }
-        // (5) compute the ready lanes
-        const int kcons = lds_k[cons_lane & (WAVE - 1)];
-        const bool bp = S == 1 || cons_lane < 0 || kcons >= k;
-        const bool can = act && e == e1 && bp;
-        __builtin_amdgcn_wave_barrier();
-        if (can) {
-            qv = compute(k, acc1, acc2);
-            if (pub_cross) slot_put(slots + (size_t)k * stride + my_slot, qv);
-            ++k;
-            e = e0;
-            acc1 = 0.0;
-            acc2 = 0.0;
-            have = 0;
+        // (5) compute the ready lanes. Lanes whose in-wave upstream cells complete here are
+        // served within the same round: the producers republish, the consumers gather, fold and
+        // compute in turn, so a chain inside the wave advances several cells per round. Every
+        // pass computes at least one sub-step, so the passes end.
+        bool any_done = false;
+        for (int pass = 0; pass < WAVE * S; ++pass) {
+            const int kcons = lds_k[cons_lane & (WAVE - 1)];
+            const bool bp = S == 1 || cons_lane < 0 || kcons >= k;
+            const bool can = act && e == e1 && bp;
+            __builtin_amdgcn_wave_barrier();
+            if (!__any(can)) break;
+            any_done = true;
+            if (can) {
+                qv = compute(k, acc1, acc2);
+                if (pub_cross) slot_put(slots + (size_t)k * stride + my_slot, qv);
+                ++k;
+                e = e0;
+                acc1 = 0.0;
+                acc2 = 0.0;
+                have = 0;
+            }
+            act = k < S;
+            lds_k[lane] = k;
+            lds_q[lane] = qv;
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int j = 0; j < PF; ++j) {
+                tk[j] = lds_k[en[j] & (WAVE - 1)];
+                tq[j] = lds_q[en[j] & (WAVE - 1)];
+            }
+#pragma unroll
+            for (int j = 0; j < PF; ++j) {
+                const bool from_wave =
+                    act && j < npf && (en[j] >> 6) == c && !(have & (1u << j)) && tk[j] == k + 1;
+                buf[j] = from_wave ? tq[j] : buf[j];
+                have |= from_wave ? (1u << j) : 0u;
+            }
+            if (act) {
+#pragma unroll
+                for (int j = 0; j < PF; ++j)
+                    if (e == e0 + j && (have & (1u << j))) {
+                        fold_in<FOLD>(acc1, acc2, buf[j], ef[j]);
+                        ++e;
+                    }
+            }
         }
-        if (__any(can) || __any(got)) {
+        if (any_done || __any(got)) {
             spin = 0;
         } else {
             __builtin_amdgcn_s_sleep(1);
@@ -590,17 +661,47 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         const double dsoil = ld(d.d, q_), ts = ld(d.ts, q_), tr = ld(d.tr, q_), tfc = ld(d.tfc, q_);
         const double sy = ld(d.sy, q_), slope = ld(d.slope, q_), flen = ld(d.fl, q_), fw = ld(d.fw, q_);
         const double qmax = ld(d.qmax, q_), kh0 = ld(d.kh0, q_), fk = ld(d.f, q_);
-        const double rech = d.recharge[q_];
+        // The cell's vertical update (the first half of the step), either done here at the start
+        // of the chunk or read back from the separate vertical kernel's outputs.
+        VertOut<N> vo;
+        if (FUSE_VERTICAL) {
+            if (valid) {
+                vertical_cell<N>(d, p, vo);
+            } else {
+#pragma unroll
+                for (int k = 0; k < N; ++k) vo.us[k] = 0.0;
+                vo.zi = vo.recharge = vo.sat_exc = vo.runoff_l = vo.inf_exc = vo.aeow_l = vo.nrr = 0.0;
+                vo.aet = vo.leak = vo.canopy = vo.snow_s = vo.snow_w = vo.store_prev = vo.P = 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < N; ++k) vo.us[k] = d.ustore[k * n + q_];
+            vo.zi = d.zi[q_];
+            vo.recharge = d.recharge[q_];
+            vo.sat_exc = d.sat_excess[q_];
+            vo.runoff_l = d.runoff_land[q_];
+            vo.inf_exc = d.inf_excess[q_];
+            vo.aeow_l = d.aeow_l[q_];
+            vo.nrr = d.net_runoff_river[q_];
+            vo.canopy = d.canopy[q_];
+            vo.P = d.balance ? d.P[q_] : 0.0;
+            vo.aet = d.balance ? d.aet[q_] : 0.0;
+            vo.leak = d.balance ? d.leakage[q_] : 0.0;
+            vo.store_prev = d.balance ? d.store_prev[q_] : 0.0;
+            vo.snow_s = d.snow ? d.snow_s[q_] : 0.0;
+            vo.snow_w = d.snow ? d.snow_w[q_] : 0.0;
+        }
+        const double rech = vo.recharge;
         // Inputs of the later phases (soil storage, balances, overland, river, total storage),
         // loaded with the chunk's first loads instead of one dependent round trip per phase.
         // Every one of these arrays is written, in this kernel, only by the lane that reads it,
         // after the read.
         const double rd = ld(d.rd, q_);
-        const double sat_exc = d.sat_excess[q_], runoff_l = d.runoff_land[q_], inf_exc = d.inf_excess[q_];
-        const double aeow = d.aeow_l[q_], nrr = d.net_runoff_river[q_], canopy = d.canopy[q_];
-        const double P_in = d.balance ? d.P[q_] : 0.0, aet = d.balance ? d.aet[q_] : 0.0;
-        const double leak = d.balance ? d.leakage[q_] : 0.0, store_prev = d.balance ? d.store_prev[q_] : 0.0;
-        const double snow_s = d.snow ? d.snow_s[q_] : 0.0, snow_w = d.snow ? d.snow_w[q_] : 0.0;
+        const double sat_exc = vo.sat_exc, runoff_l = vo.runoff_l, inf_exc = vo.inf_exc;
+        const double aeow = vo.aeow_l, nrr = vo.nrr, canopy = vo.canopy;
+        const double P_in = vo.P, aet = vo.aet;
+        const double leak = vo.leak, store_prev = vo.store_prev;
+        const double snow_s = vo.snow_s, snow_w = vo.snow_w;
         const double alpha = ld(d.alpha, q_), sfw = ld(d.sfw, q_), rfrac = ld(d.rf, q_);
         const double oq0 = d.ol_q[q_], oh0 = d.ol_h[q_], o_store_prev = d.ol_storage[q_];
         const bool riv = r >= 0;
@@ -611,8 +712,8 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         Column<N> col;
         soil_layers<N>(dsoil, d.cfg_cum, d.cfg_thick, col.act, col.cum);
 #pragma unroll
-        for (int k = 0; k < N; ++k) col.ustore[k] = d.ustore[k * n + q_];
-        col.zi = d.zi[q_];
+        for (int k = 0; k < N; ++k) col.ustore[k] = vo.us[k];
+        col.zi = vo.zi;
         col.n_unsat = set_layerthickness<N>(col.zi, col.cum, col.act, col.thick);
         col.theta_e = ts - tr;
         col.theta_fc_r = tfc - tr;
@@ -1639,7 +1740,7 @@ static void launch_kernels(wfh_handle* h) {
     const int n = (int)h->n;
     hipEvent_t* ev = timing_events(h);
     if (ev) hipEventRecord(ev[0], h->stream);
-    hipLaunchKernelGGL(k_vertical<N>, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d);
+    if (!FUSE_VERTICAL) hipLaunchKernelGGL(k_vertical<N>, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d);
     if (ev) hipEventRecord(ev[1], h->stream);
     RouteArgs a = h->ra;
     a.parity = h->parity;
@@ -2016,6 +2117,12 @@ extern "C" int32_t wfh_kernel_bytes(wfh_handle* h, int64_t* vertical_bytes, int6
     k2 += A * 9;                                          // overland outputs + total storage
     k2 += 8 * nr * 7;                                     // river outputs
     if (D.balance) k2 += 4 * A + 8 * nr;                  // balance errors
+    if (FUSE_VERTICAL) {
+        // one kernel: the routing part takes the vertical results from registers instead of
+        // reading back ustore[N], zi, recharge, 5 fluxes, canopy, snow and the balance inputs
+        k2 += k1 - (A * (N + 1) + A + A * 5 + A * 2 + (D.balance ? 4 * A : 0));
+        k1 = 0;
+    }
     if (vertical_bytes) *vertical_bytes = k1;
     if (routing_bytes) *routing_bytes = k2;
     return WFH_OK;
