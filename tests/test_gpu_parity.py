@@ -198,3 +198,69 @@ def test_hourly_1024_first_24_steps():
     gpu, ref = run_pair(c, 24, every=6)
     gpu.close()
     ref.close()
+
+
+def test_hourly_4096_first_24_steps_on_basins():
+    """The full 4096^2 hourly workload (250 m, river threshold 2000 cells) on the GPU against
+    the oracle run on complete basins of it (basins exchange no water, so a set of whole basins
+    is an exact sub-problem): random small basins over the first 24 steps, checked every 6th
+    step, and the largest basin (about 6 M cells, holding the 5306-cell longest flow path) over
+    the first 6 steps. Forcing is generated on the GPU in node order; the oracle receives the
+    same values. Every state and flux of every oracle cell must match within RTOL."""
+    import sys
+    import time
+    import torch
+    from wflowhip import _abi
+
+    def say(msg):  # progress outside pytest's capture: the oracle part runs for about a minute
+        sys.__stderr__.write(f"[4096 basins] {msg}\n")
+        sys.__stderr__.flush()
+
+    t0 = time.time()
+    c = synth.make_catchment(4096, cell=250.0, dt=3600.0, river_threshold=2000, seed=1)
+    gpu = Model(c)
+    node = gpu.node_order()
+    inv = np.empty_like(node)
+    inv[node] = np.arange(node.size)
+    node_t = torch.as_tensor(node, device="cuda")
+    elev = torch.as_tensor(c.arrays["elevation"], device="cuda")[node_t].contiguous()
+    lab = synth.basin_ids(c.arrays["ldd"], c.nx, c.ny, c.arrays["cell_index"])
+    ids, counts = np.unique(lab, return_counts=True)
+    rng = np.random.default_rng(24)
+    small, total = [], 0
+    for k in rng.permutation(len(ids)):
+        if counts[k] < 20000 and total < 150000:
+            small.append(ids[k])
+            total += counts[k]
+    riv = c.arrays["river_location"].astype(bool)
+    parts = []
+    for name, basin_set, steps in (("small", np.array(small), 24), ("largest", ids[np.argmax(counts)][None], 6)):
+        mask = np.isin(lab, basin_set)
+        sel = torch.as_tensor(inv[np.nonzero(mask)[0]], device="cuda")  # node positions of the cells
+        parts.append((name, mask, sel, mask[riv], O.OracleModel(c.subset(mask)), steps))
+    say(f"set up in {time.time() - t0:.0f}s: {[(p[0], int(p[1].sum())) for p in parts]}")
+    for k in range(24):
+        P, E, T = synth.forcing_torch(c, k, elev)
+        gpu.set_forcing(P, E, T, memkind=_abi.WFH_MEM_DEVICE_NODE_ORDER)
+        gpu.update_model()
+        check = (k + 1) % 6 == 0
+        for name, mask, sel, rmask, ref, steps in parts:
+            if k >= steps:
+                continue
+            ref.step(*(f[sel].cpu().numpy() for f in (P, E, T)))
+            if not check:
+                continue
+            for v in STATE_VARS + FLUX_VARS:
+                g = gpu.get(v)
+                if v in _abi.RIVER_VARS:
+                    g = g[rmask]
+                elif v in _abi.LAYERED_VARS:
+                    g = g.reshape(-1, c.n)[:, mask].ravel()
+                else:
+                    g = g[mask]
+                e = rel_err(g, ref.get(v))
+                assert e <= RTOL, (name, k, v.name, e)
+        say(f"step {k} done ({time.time() - t0:.0f}s)")
+    gpu.close()
+    for p in parts:
+        p[4].close()
