@@ -495,10 +495,14 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
         buf[j] = 0.0;
         v[j] = SLOT_EMPTY;
         pol[j] = false;
+        // every lane loads PF entries (unused ones at a valid index) so the loads overlap
+        const int ej = j < npf ? e0 + j : (npf > 0 ? e0 : 0);
+        const int un = up_node[ej], us = up_slot[ej];
+        const double uf = (FOLD != FOLD_RIV) ? up_f[ej] : 0.0;
         if (j < npf) {
-            en[j] = up_node[e0 + j];
-            es[j] = up_slot[e0 + j];
-            if (FOLD != FOLD_RIV) ef[j] = up_f[e0 + j];
+            en[j] = un;
+            es[j] = us;
+            ef[j] = uf;
         }
     }
     unsigned have = 0;  // bit j: buf[j] holds edge e0 + j's outflow for sub-step k
@@ -645,16 +649,20 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         if (lane == 0) t = atomicAdd(&a.counters[a.parity], 1ull);
         t = __shfl(t, 0);
         if (t >= (unsigned long long)a.nchunks) break;
-        if (__hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
         const int c = (int)t;
         const int p = c * WAVE + lane;
         const bool valid = p < n;
         const int q_ = valid ? p : 0;  // clamped index for loads of inactive lanes
-        const int dn = valid ? d.down[p] : -1;
+        // Loads are issued for every lane (inactive lanes read cell 0) and selected afterwards:
+        // a load under a per-lane condition becomes a branch, and the compiler then waits for
+        // it before the join instead of overlapping it with the others.
+        const int dn_ = d.down[q_], e0_ = d.up_ptr[q_], e1_ = d.up_ptr[q_ + 1], r_ = d.river_of[q_];
+        const uint8_t fl_ = d.flags[q_];
+        const int dn = valid ? dn_ : -1;
         const bool cons_in = dn >= 0 && (dn >> 6) == c;
-        const int e0 = d.up_ptr[q_], e1 = valid ? d.up_ptr[q_ + 1] : e0;
-        const uint8_t fl = valid ? d.flags[p] : 0;
-        const int r = valid ? d.river_of[p] : -1;
+        const int e0 = e0_, e1 = valid ? e1_ : e0_;
+        const uint8_t fl = valid ? fl_ : 0;
+        const int r = valid ? r_ : -1;
 
         // ---------------- lateral subsurface flow ----------------
         const double area = ld(d.area, q_);
@@ -684,12 +692,14 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             vo.aeow_l = d.aeow_l[q_];
             vo.nrr = d.net_runoff_river[q_];
             vo.canopy = d.canopy[q_];
-            vo.P = d.balance ? d.P[q_] : 0.0;
-            vo.aet = d.balance ? d.aet[q_] : 0.0;
-            vo.leak = d.balance ? d.leakage[q_] : 0.0;
-            vo.store_prev = d.balance ? d.store_prev[q_] : 0.0;
-            vo.snow_s = d.snow ? d.snow_s[q_] : 0.0;
-            vo.snow_w = d.snow ? d.snow_w[q_] : 0.0;
+            const double P_ = d.P[q_], aet_ = d.aet[q_], leak_ = d.leakage[q_], sp_ = d.store_prev[q_];
+            const double ss_ = d.snow_s[q_], sw_ = d.snow_w[q_];
+            vo.P = d.balance ? P_ : 0.0;
+            vo.aet = d.balance ? aet_ : 0.0;
+            vo.leak = d.balance ? leak_ : 0.0;
+            vo.store_prev = d.balance ? sp_ : 0.0;
+            vo.snow_s = d.snow ? ss_ : 0.0;
+            vo.snow_w = d.snow ? sw_ : 0.0;
         }
         const double rech = vo.recharge;
         // Inputs of the later phases (soil storage, balances, overland, river, total storage),
@@ -706,7 +716,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         const double oq0 = d.ol_q[q_], oh0 = d.ol_h[q_], o_store_prev = d.ol_storage[q_];
         const bool riv = r >= 0;
         const int rr = riv ? r : 0;
-        const int re0 = d.rup_ptr[rr], re1 = riv ? d.rup_ptr[rr + 1] : re0;
+        const int re0 = d.rup_ptr[rr], re1_ = d.rup_ptr[riv ? rr + 1 : rr], re1 = riv ? re1_ : re0;
         const double rfl = d.r_fl[rr], rfw = d.r_fw[rr], ralpha = d.r_alpha[rr], rarea = d.r_area[rr];
         const double rq0 = d.r_q[rr], rh0 = d.r_h[rr], r_store_prev = d.r_storage[rr];
         Column<N> col;
