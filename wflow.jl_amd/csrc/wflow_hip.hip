@@ -135,24 +135,49 @@ template <int N>
 __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut<N>& o) {
     const int n = d.n;
     const double dt = d.dt;
+    // Every input of the cell is loaded here, in one block, unconditionally (parameters of
+    // switched-off processes are allocated as zeros): loads spread over the branches below
+    // would each be waited for at the branch's end, one memory round trip after another.
     const double P = d.P[i], E = d.E[i], T = d.T[i];
+    const double canopy0 = d.canopy[i], ss0 = d.snow_s[i], sw0 = d.snow_w[i], tsws0 = d.tsws[i];
+    const double gap = ld(d.gap, i), smax = ld(d.smax, i), kc = ld(d.kc, i), er = ld(d.er, i);
+    const double p_tti = ld(d.tti, i), p_tt = ld(d.tt, i), p_ttm = ld(d.ttm, i), p_cfmax = ld(d.cfmax, i);
+    const double p_whc = ld(d.whc, i), p_wsoil = ld(d.wsoil, i), p_cfsoil = ld(d.cfsoil, i);
+    const int r = d.river_of[i];
+    const double wd_land = d.ol_h[i];
+    const double r_h0 = d.r_h[r >= 0 ? r : 0];
+    const double rf = ld(d.rf, i), wf = ld(d.wf, i), p_sfrac = ld(d.sfrac, i);
+    const double ts = ld(d.ts, i), tr = ld(d.tr, i), tfc = ld(d.tfc, i), dsoil = ld(d.d, i);
+    const double kv0 = ld(d.kv0, i), fk = ld(d.f, i);
+    const double swd = d.swd[i], tsoil0 = d.tsoil[i];
+    const double compf = ld(d.compf, i), icap_s = ld(d.icap_s, i), icap_c = ld(d.icap_c, i);
+    const double rd = ld(d.rd, i), p_h3h = ld(d.h3h, i), p_h3l = ld(d.h3l, i);
+    const double h1 = ld(d.h1, i), h2 = ld(d.h2, i), h4 = ld(d.h4, i), ah1 = ld(d.ah1, i), hb = ld(d.hb, i);
+    const double p_rdp = ld(d.rdp, i), caph = ld(d.caph, i), capn = ld(d.capn, i), maxleak = ld(d.maxleak, i);
+    double us[N], kvf[N], cbc[N], rfr[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        us[k] = d.ustore[k * n + i];
+        kvf[k] = ld(d.kvfrac, k, i);
+        cbc[k] = ld(d.c, k, i);
+        rfr[k] = ld(d.rootfrac, k, i);
+    }
 
     // storage_prev! (Wflow/src/mass_balance.jl:96-115,170-183) from last step's states
-    double canopy = d.canopy[i];
-    double ss = d.snow ? d.snow_s[i] : 0.0, sw = d.snow ? d.snow_w[i] : 0.0;
+    double canopy = canopy0;
+    double ss = d.snow ? ss0 : 0.0, sw = d.snow ? sw0 : 0.0;
     double store_prev = 0.0;
     if (d.balance) {
         const double snow = d.snow ? ss + sw : 0.0 + 0.0;
-        store_prev = d.tsws[i] + canopy + snow + 0.0 * 0.0 + 0.0;
+        store_prev = tsws0 + canopy + snow + 0.0 * 0.0 + 0.0;
         d.store_prev[i] = store_prev;
     }
 
     // ---- interception: canopy.jl:54-98 (Gash, dt >= 23 h) / :115-143 (modified Rutter) ----
-    const double gap = ld(d.gap, i), smax = ld(d.smax, i);
-    const double cpe = ld(d.kc, i) * E * (1.0 - gap);
+    const double cpe = kc * E * (1.0 - gap);
     double tf, ir, sf;
     if (d.gash) {
-        interception_gash(smax, ld(d.er, i), gap, P, cpe, dt, tf, ir, sf);
+        interception_gash(smax, er, gap, P, cpe, dt, tf, ir, sf);
     } else {
         interception_modrut(P, cpe, gap, smax, dt, canopy, tf, ir, sf);
     }
@@ -161,8 +186,7 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
     double wfs;
     if (d.snow) {
         double swe, sro;
-        snow_hbv(tf + sf, T, ld(d.tti, i), ld(d.tt, i), ld(d.ttm, i), ld(d.cfmax, i), ld(d.whc, i), dt, ss, sw, swe,
-                 sro);
+        snow_hbv(tf + sf, T, p_tti, p_tt, p_ttm, p_cfmax, p_whc, dt, ss, sw, swe, sro);
         d.swe[i] = swe;
         wfs = sro + 0.0 * 0.0;  // get_water_flux_surface! with snow (runoff.jl:49-58), no glacier
     } else {
@@ -170,10 +194,7 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
     }
 
     // ---- open water runoff: runoff.jl:61-111 ----
-    const int r = d.river_of[i];
-    const double wd_land = d.ol_h[i];
-    const double wd_river = r >= 0 ? d.r_h[r] : 0.0;
-    const double rf = ld(d.rf, i), wf = ld(d.wf, i);
+    const double wd_river = r >= 0 ? r_h0 : 0.0;
     const double rr = wfh_min(1.0, rf) * wfs;
     const double rl = wfh_min(1.0, wf) * wfs;
     const double aor = rf * wfh_min(wd_river / dt, E);
@@ -182,19 +203,13 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
 
     // ---- soil boundary conditions: soil.jl:643-682 ----
     const double pot_trans = wfh_max(0.0, cpe - ir);
-    double pot_sevap = ld(d.sfrac, i) * E;
+    double pot_sevap = p_sfrac * E;
     pot_sevap = pot_sevap - 0.0;
     const double wfs_soil = wfh_max(((wfs + 0.0) - rr) - rl + 0.0 / dt, 0.0);
 
     // ---- update_soil_water_flow!  soil.jl:1150-1211 ----
-    const double ts = ld(d.ts, i), tr = ld(d.tr, i), tfc = ld(d.tfc, i), dsoil = ld(d.d, i);
-    const double kv0 = ld(d.kv0, i), fk = ld(d.f, i);
     double act[N], cum[N + 1];
     const int nl = soil_layers<N>(dsoil, d.cfg_cum, d.cfg_thick, act, cum);
-    double us[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) us[k] = d.ustore[k * n + i];
-    const double swd = d.swd[i];
     const double swc = dsoil * (ts - tr);
     // update_diagnostic_vars!  soil.jl:1400-1436
     double usd = 0.0;
@@ -214,16 +229,15 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
     double th[N];
     const int nu = set_layerthickness<N>(zi, cum, act, th);
     // soil_temperature!, infiltration_reduction_factor!  soil.jl:685-730
-    double tsoil = d.tsoil[i];
-    if (d.snow) tsoil = tsoil + ld(d.wsoil, i) * (T - tsoil);
+    double tsoil = tsoil0;
+    if (d.snow) tsoil = tsoil + p_wsoil * (T - tsoil);
     double fred = 1.0;
     if (d.snow && d.infred) {
-        const double cf = ld(d.cfsoil, i);
+        const double cf = p_cfsoil;
         const double bb = 1.0 / (1.0 - cf);
         fred = scurve(tsoil, 0.0 + 273.15, bb, 8.0) + cf;
     }
     // infiltration  soil_process.jl:16-41
-    const double compf = ld(d.compf, i), icap_s = ld(d.icap_s, i), icap_c = ld(d.icap_c, i);
     double infilt, inf_exc;
     {
         const double soilinf = wfs_soil * (1.0 - compf);
@@ -242,9 +256,9 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
             if (m >= nu) continue;
             z = (m == 0) ? th[0] : z + th[m];
             const double l_sat = th[m] * (ts - tr);
-            const double kvz = ld(d.kvfrac, m, i) * kv0 * wfh_exp(-fk * z);
+            const double kvz = kvf[m] * kv0 * wfh_exp(-fk * z);
             double u = (m == 0) ? us[m] + infilt * dt : us[m] + flow * dt;
-            unsatzone_flow_layer(u, kvz, l_sat, ld(d.c, m, i), dt, flow);
+            unsatzone_flow_layer(u, kvz, l_sat, cbc[m], dt, flow);
             us[m] = u;
         }
         transfer = flow;
@@ -273,12 +287,9 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
         drainable -= ses * dt;
     }
     // transpiration!  soil.jl:865-975
-    const double rd = ld(d.rd, i);
     double ae_u = 0.0, ae_s, transp;
     {
-        const double h3 = feddes_h3(ld(d.h3h, i), ld(d.h3l, i), pot_trans);
-        const double h1 = ld(d.h1, i), h2 = ld(d.h2, i), h4 = ld(d.h4, i), ah1 = ld(d.ah1, i);
-        const double hb = ld(d.hb, i);
+        const double h3 = feddes_h3(p_h3h, p_h3l, pot_trans);
         double sum_rf = 0.0, rf_low = 0.0;
 #pragma unroll
         for (int k = 0; k < N; ++k) {
@@ -286,9 +297,9 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
             double rfu;
             if (k == nu - 1 && zi < rd) {
                 const double rootlength = wfh_min(act[k], rd - cum[k]);
-                rfu = ld(d.rootfrac, k, i) * (th[k] / rootlength);
+                rfu = rfr[k] * (th[k] / rootlength);
             } else {
-                rfu = ld(d.rootfrac, k, i);
+                rfu = rfr[k];
             }
             sum_rf += rfu;
             rf_low = rfu;
@@ -296,10 +307,10 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
 #pragma unroll
         for (int k = 0; k < N; ++k) {
             if (k >= nu) continue;
-            const double rfu = (k < nu - 1) ? ld(d.rootfrac, k, i) : rf_low;
+            const double rfu = (k < nu - 1) ? rfr[k] : rf_low;
             const double rfs = rd > 0.0 ? wfh_max((1.0 / sum_rf), 1.0) * rfu : 0.0;
             const double vwc = wfh_max(us[k] / th[k], 1e-7);
-            const double head = head_brooks_corey(vwc, ts, tr, ld(d.c, k, i), hb);
+            const double head = head_brooks_corey(vwc, ts, tr, cbc[k], hb);
             const double alpha = rwu_reduction_feddes(head, h1, h2, h3, h4, ah1);
             const double availcap = wfh_min(1.0, wfh_max(0.0, (rd - cum[k]) / th[k]));
             const double maxextr = us[k] * availcap / dt;
@@ -307,7 +318,7 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
             us[k] = us[k] - layer * dt;
             ae_u += layer;
         }
-        const double wetroots = scurve(zi, rd, 1.0, ld(d.rdp, i));
+        const double wetroots = scurve(zi, rd, 1.0, p_rdp);
         const double alpha0 = rwu_reduction_feddes(0.0, h1, h2, h3, h4, ah1);
         const double rest = pot_trans - ae_u;
         ae_s = wfh_min(rest * wetroots * alpha0, drainable / dt);
@@ -343,12 +354,15 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
     // capillary_flux!  soil.jl:1050-1111
     double capflux = 0.0;
     if (nu > 0) {
-        const double ksat = ld(d.kvfrac, nu - 1, i) * kv0 * wfh_exp(-fk * zi);
+        double kvf_nu = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            if (k == nu - 1) kvf_nu = kvf[k];
+        const double ksat = kvf_nu * kv0 * wfh_exp(-fk * zi);
         const double mx = wfh_max(0.0, wfh_min(wfh_min(wfh_min(ksat, ae_u), ucap / dt), drainable / dt));
         double cap;
-        const double caph = ld(d.caph, i);
         if (zi > rd)
-            cap = mx * wfh_pow(1.0 - wfh_min(zi, caph) / (caph), ld(d.capn, i));
+            cap = mx * wfh_pow(1.0 - wfh_min(zi, caph) / (caph), capn);
         else
             cap = 0.0;
         double net = cap;
@@ -364,9 +378,13 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
     // leakage!  soil.jl:1118-1136
     double leak;
     {
-        const double deepksat = ld(d.kvfrac, nl - 1, i) * kv0 * wfh_exp(-fk * dsoil);
+        double kvf_nl = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            if (k == nl - 1) kvf_nl = kvf[k];
+        const double deepksat = kvf_nl * kv0 * wfh_exp(-fk * dsoil);
         const double deeptransfer = wfh_min(drainable / dt, deepksat);
-        leak = wfh_max(0.0, wfh_min(ld(d.maxleak, i), deeptransfer));
+        leak = wfh_max(0.0, wfh_min(maxleak, deeptransfer));
     }
     const double recharge = (transfer - capflux - leak - ae_s - ses);
     double aet = sevap + transp + aor + aol + 0.0;
