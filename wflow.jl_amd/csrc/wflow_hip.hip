@@ -505,23 +505,30 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
     double ef[PF], buf[PF];
     uint64_t v[PF];
     bool pol[PF];
+    // Every lane loads PF entries (unused ones at a valid index); the empty asm takes them all as
+    // operands, so the compiler issues the loads together and waits once, instead of moving each
+    // into the per-lane branch that uses it and waiting for them one by one.
+    int un[PF], us[PF];
+    double uf[PF];
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
-        en[j] = c * WAVE;
-        es[j] = 0;
-        ef[j] = 0.0;
+        const int ej = j < npf ? e0 + j : (npf > 0 ? e0 : 0);
+        un[j] = up_node[ej];
+        us[j] = up_slot[ej];
+        uf[j] = (FOLD != FOLD_RIV) ? up_f[ej] : 0.0;
+    }
+    static_assert(PF == 4, "the operand list below names four edges");
+    asm volatile("" : "+v"(un[0]), "+v"(un[1]), "+v"(un[2]), "+v"(un[3]), "+v"(us[0]), "+v"(us[1]), "+v"(us[2]),
+                 "+v"(us[3]), "+v"(uf[0]), "+v"(uf[1]), "+v"(uf[2]), "+v"(uf[3]));
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+        const bool use = j < npf;
+        en[j] = use ? un[j] : c * WAVE;
+        es[j] = use ? us[j] : 0;
+        ef[j] = use ? uf[j] : 0.0;
         buf[j] = 0.0;
         v[j] = SLOT_EMPTY;
         pol[j] = false;
-        // every lane loads PF entries (unused ones at a valid index) so the loads overlap
-        const int ej = j < npf ? e0 + j : (npf > 0 ? e0 : 0);
-        const int un = up_node[ej], us = up_slot[ej];
-        const double uf = (FOLD != FOLD_RIV) ? up_f[ej] : 0.0;
-        if (j < npf) {
-            en[j] = un;
-            es[j] = us;
-            ef[j] = uf;
-        }
     }
     unsigned have = 0;  // bit j: buf[j] holds edge e0 + j's outflow for sub-step k
     long spin = 0;
