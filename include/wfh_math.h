@@ -231,4 +231,32 @@ WFH_FN double wfh_round_sig12(double x) {
     return (wfh_fabs(r) <= 1.7976931348623157e308) ? r : x;
 }
 
+/* Statistics.quantile!(v, p) of n >= 2 values with the default definition (Hyndman and Fan
+ * type 7, alpha = beta = 1), as Statistics.jl 1.11.1 (the version pinned in Manifest.toml)
+ * evaluates it: m = alpha + p (1 - alpha - beta), aleph = fma(n, p, m),
+ * j = clamp(trunc(aleph), 1, n - 1), gamma = clamp(aleph - j, 0, 1); with a = v(j) and
+ * b = v(j+1) the j-th and (j+1)-th smallest values, the result is a + gamma (b - a) when both are
+ * finite and a ~ b (isapprox: |a - b| <= sqrt(eps) max(|a|, |b|)), otherwise
+ * (1 - gamma) a + gamma b. wfh_quantile_index gives j (1-based); the reference calls it at
+ * Wflow/src/routing/surface/surface_kinwave.jl:698. */
+WFH_FN int64_t wfh_quantile_index(int64_t n, double p, double* gamma) {
+    const double m = 1.0 + p * ((1.0 - 1.0) - 1.0);
+    const double aleph = WFH_FMA((double)n, p, m);
+    int64_t j = (int64_t)aleph; /* trunc */
+    if (j < 1) j = 1;
+    if (j > n - 1) j = n - 1;
+    double g = aleph - (double)j;
+    g = g > 1.0 ? 1.0 : (g < 0.0 ? 0.0 : g);
+    *gamma = g;
+    return j;
+}
+WFH_FN double wfh_quantile_interp(double a, double b, double gamma) {
+    const double inf = 1.7976931348623157e308;
+    const int fin = wfh_fabs(a) <= inf && wfh_fabs(b) <= inf;
+    const double aa = wfh_fabs(a), ab = wfh_fabs(b);
+    const int approx = a == b || (fin && wfh_fabs(a - b) <= 1.4901161193847656e-8 * (aa > ab ? aa : ab));
+    if (fin && approx) return a + gamma * (b - a);
+    return (1.0 - gamma) * a + gamma * b;
+}
+
 #endif /* WFH_MATH_H */

@@ -21,7 +21,8 @@
  *  - Every function returns WFH_OK (0) or a negative status; no exception crosses the ABI.
  *    wfh_last_error() gives the message of the last failure on a handle.
  *  - A handle owns one HIP stream and its device memory. Calls on one handle must come from
- *    one host thread at a time. wfh_step() is asynchronous; wfh_get()/wfh_sync() synchronise.
+ *    one host thread at a time. wfh_step() is asynchronous with fixed sub-steps; wfh_get()/wfh_sync()
+ *    synchronise.
  */
 #ifndef WFLOW_HIP_H
 #define WFLOW_HIP_H
@@ -33,7 +34,7 @@
 extern "C" {
 #endif
 
-#define WFH_ABI_VERSION 1
+#define WFH_ABI_VERSION 2
 #define WFH_MAX_LAYERS 8
 
 enum wfh_status {
@@ -65,6 +66,11 @@ typedef struct wfh_config {
     double soil_layer_thickness_mm[WFH_MAX_LAYERS]; /* soil_layer__thickness [mm]            */
     int32_t soil_infiltration_reduction; /* soil_infiltration_reduction__flag                  */
     int32_t water_mass_balance;          /* water_mass_balance__flag                           */
+    /* kinematic_wave__adaptive_time_step_flag (config_structure.jl:84): 0 = fixed sub-steps of
+     * land_dt / river_dt / subsurface_dt; 1 = adaptive sub-steps (surface_kinwave.jl:674-704,
+     * lateral_subsurface_flow.jl:314-344), the *_dt fields are then ignored.            */
+    int32_t kinematic_wave_adaptive;
+    double subsurface_alpha_coefficient; /* subsurface_kinematic_wave__alpha_coefficient (:88), 1.0 */
 } wfh_config;
 
 /* Static maps as the reference reads them (standard names in brackets), already in SI.
@@ -211,8 +217,15 @@ int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_handle** out
 int32_t wfh_set_forcing(wfh_handle* h, const double* precipitation, const double* potential_evaporation,
                         const double* temperature, int32_t memkind);
 
-/* One update_model! step.  Replaces: update_model!  Wflow/src/sbm_model.jl:60-92 */
+/* One update_model! step.  Replaces: update_model!  Wflow/src/sbm_model.jl:60-92
+ * With kinematic_wave_adaptive set, every routing sub-step length depends on the state the
+ * previous sub-step left (a global quantile or minimum over the cells), so the host reads each
+ * one back before it launches the sub-step: the call returns when the step has completed. */
 int32_t wfh_step(wfh_handle* h);
+
+/* Number of subsurface, overland and river sub-steps the last wfh_step() took (fixed or
+ * adaptive). No counterpart in the reference (its loop counters are local). */
+int32_t wfh_last_substeps(wfh_handle* h, int32_t* subsurface, int32_t* land, int32_t* river);
 
 /* BMI.get_value / set_value (Wflow/src/bmi.jl:203-262), land-index / river-index order.
  * count = number of doubles the caller provides (n, nriv or N*n). */

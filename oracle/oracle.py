@@ -101,6 +101,11 @@ def lib():
         L.wfo_soil_evaporation_saturated_store.restype = _c
         L.wfo_soil_evaporation_saturated_store.argtypes = [_c, ctypes.c_int, _c, _c, _c, _c]
         L.wfo_math.argtypes = [ctypes.c_int32, _D, _D, _D, ctypes.c_int64]
+        L.wfo_quantile.restype = _c
+        L.wfo_quantile.argtypes = [_D, ctypes.c_int64, _c]
+        L.wfo_kw_stable_timestep.restype = _c
+        L.wfo_kw_stable_timestep.argtypes = [_D, _D, _D, ctypes.c_int64, _c, _D]
+        L.wfo_last_substeps.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
         _lib = L
     return _lib
 
@@ -116,6 +121,21 @@ def math(op, x, y=None):
     rc = lib().wfo_math(MATH_OPS[op], x.ctypes.data_as(_D), y.ctypes.data_as(_D), out.ctypes.data_as(_D), x.size)
     assert rc == 0
     return out
+
+
+def quantile(v, p):
+    """Statistics.quantile!(v, p) as the reference's adaptive time step evaluates it."""
+    v = np.array(v, dtype=np.float64)
+    assert v.size >= 2
+    return lib().wfo_quantile(v.ctypes.data_as(_D), v.size, p)
+
+
+def kw_stable_timestep(q, alpha, flow_length, p):
+    """stable_timestep of a kinematic-wave river / overland model (surface_kinwave.jl:674-704)."""
+    q, alpha, fl = (np.ascontiguousarray(x, dtype=np.float64) for x in (q, alpha, flow_length))
+    buf = np.empty(max(q.size, 1), np.float64)
+    return lib().wfo_kw_stable_timestep(q.ctypes.data_as(_D), alpha.ctypes.data_as(_D), fl.ctypes.data_as(_D),
+                                        q.size, p, buf.ctypes.data_as(_D))
 
 
 def _out(k):
@@ -276,3 +296,9 @@ class OracleModel:
 
     def kinwave_iterations(self):
         return lib().wfo_kinwave_iterations(self.h)
+
+    def last_substeps(self):
+        """(subsurface, overland, river) sub-step counts of the last step."""
+        out = (ctypes.c_int32 * 3)()
+        lib().wfo_last_substeps(self.h, out)
+        return tuple(out)

@@ -839,6 +839,9 @@ struct wfo_model {
     /* total storage and balance */
     double *total_storage, *land_store_prev, *ol_store_prev, *r_store_prev;
     double *bal_land, *bal_land_rel, *bal_ssf, *bal_ol, *bal_r;
+    /* adaptive sub-steps: stable time step buffer, sub-step counts of the last step */
+    double* stbuf;
+    int32_t nsub[3];
 };
 
 static double* dalloc(int64_t n) { return (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
@@ -1160,6 +1163,7 @@ wfo_model* wfo_create(const wfh_config* cfg, const wfh_static* st, char* err, in
     m->total_storage = dalloc(n); m->land_store_prev = dalloc(n); m->ol_store_prev = dalloc(n);
     m->r_store_prev = dalloc(nriv); m->bal_land = dalloc(n); m->bal_land_rel = dalloc(n); m->bal_ssf = dalloc(n);
     m->bal_ol = dalloc(n); m->bal_r = dalloc(nriv);
+    m->stbuf = dalloc(n > nriv ? n : nriv);
     free(ldd2);
     return m;
 fail:
@@ -1194,13 +1198,61 @@ void wfo_destroy(wfo_model* m) {
         (void**)&m->r_qin_cum, (void**)&m->r_q_avg, (void**)&m->r_qin_avg, (void**)&m->r_area,
         (void**)&m->precip, (void**)&m->pet, (void**)&m->temp, (void**)&m->total_storage,
         (void**)&m->land_store_prev, (void**)&m->ol_store_prev, (void**)&m->r_store_prev, (void**)&m->bal_land,
-        (void**)&m->bal_land_rel, (void**)&m->bal_ssf, (void**)&m->bal_ol, (void**)&m->bal_r};
+        (void**)&m->bal_land_rel, (void**)&m->bal_ssf, (void**)&m->bal_ol, (void**)&m->bal_r, (void**)&m->stbuf};
     for (size_t k = 0; k < sizeof p / sizeof p[0]; ++k) free(*p[k]);
     free(m);
 }
 
 /* check_timestepsize  Wflow/src/routing/timestepping.jl:11-17 */
 static double check_timestepsize(double ts, double t, double end) { return t + ts > end ? end - t : ts; }
+
+static int cmp_double(const void* a, const void* b) {
+    const double x = *(const double*)a, y = *(const double*)b;
+    return (x > y) - (x < y);
+}
+
+/* quantile!(v, p) of k >= 2 values (Statistics.jl 1.11.1, default type 7; the order statistics
+ * come from a full sort, the interpolation is wfh_quantile_index/_interp). Sorts v. */
+double wfo_quantile(double* v, int64_t k, double p) {
+    qsort(v, (size_t)k, sizeof(double), cmp_double);
+    double g;
+    const int64_t j = wfh_quantile_index(k, p, &g);
+    return wfh_quantile_interp(v[j - 1], v[j], g);
+}
+
+/* stable_timestep(flow_model, flow_length, p) of the kinematic-wave river and overland models
+ * Wflow/src/routing/surface/surface_kinwave.jl:674-704: the p-quantile of the Courant time steps
+ * flow_length / celerity over the cells with q > KIN_WAVE_MIN_FLOW, celerity =
+ * inv(alpha BETA q^(BETA - 1)) with the reference's pow (utils.jl:470); 600 s without flow.
+ * buf holds n values. */
+double wfo_kw_stable_timestep(const double* q, const double* alpha, const double* flow_length, int64_t n,
+                              double p, double* buf) {
+    int64_t k = 0;
+    for (int64_t i = 0; i < n; ++i)
+        if (q[i] > WFH_KIN_WAVE_MIN_FLOW) {
+            const double c = 1.0 / (alpha[i] * 0.6 * wfh_pow(q[i], (0.6 - 1.0)));
+            buf[k++] = flow_length[i] / c;
+        }
+    if (k == 1) return buf[0];
+    if (k > 0) return wfo_quantile(buf, k, p);
+    return 600.0;
+}
+
+/* stable_timestep(::LateralSSFModel, domain)  lateral_subsurface_flow.jl:314-344: the minimum
+ * of flow_length / ssf_celerity over the cells with a positive water table depth (0.5 s when
+ * there is none), times the alpha coefficient. */
+static double ssf_stable_timestep(const wfo_model* m) {
+    double mn = 0.0;
+    int found = 0;
+    for (int64_t i = 0; i < m->n; ++i)
+        if (m->ssf_zi[i] > 0.0) {
+            const double c = wfo_ssf_celerity(m->ssf_zi[i], m->slope[i], m->sy[i], m->kh0[i], m->soil[i].f, NaN);
+            const double v = m->flow_length[i] / c;
+            mn = found ? wfh_min(mn, v) : v;
+            found = 1;
+        }
+    return (found ? mn : 0.5) * m->cfg.subsurface_alpha_coefficient;
+}
 
 /* compute_total_storage  mass_balance.jl:96-115 */
 static double land_total_storage(const wfo_model* m, int64_t i) {
@@ -1275,6 +1327,7 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
         m->rech_rate[i] = m->soil[i].recharge;
         m->ssf_zi[i] = m->soil[i].zi;
     }
+    m->nsub[0] = m->nsub[1] = m->nsub[2] = 0;
     /* ---- update_subsurface_flow_model!  lateral_subsurface_flow.jl:279-304 ---- */
     for (int64_t i = 0; i < n; ++i) {
         m->ssf_tor_cum[i] = 0.0;
@@ -1287,7 +1340,9 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
     {
         double t = 0.0;
         while (t < dt) {
-            double dts = check_timestepsize(m->cfg.subsurface_dt, t, dt);
+            double dts = m->cfg.kinematic_wave_adaptive ? ssf_stable_timestep(m) : m->cfg.subsurface_dt;
+            dts = check_timestepsize(dts, t, dt);
+            ++m->nsub[0];
             for (int64_t i = 0; i < n; ++i) { /* flux!(::RechargeModel)  boundary_conditions.jl:219-236 */
                 m->qnet_bnds[i] = 0.0;
                 double fl = m->rech_rate[i] * m->area[i];
@@ -1346,7 +1401,11 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
     {
         double t = 0.0;
         while (t < dt) { /* update_overland_flow_model!  :347-385 */
-            double dts = check_timestepsize(m->cfg.land_dt, t, dt);
+            double dts = m->cfg.kinematic_wave_adaptive
+                             ? wfo_kw_stable_timestep(m->ol_q, m->alpha_land, m->flow_length, n, 0.02, m->stbuf)
+                             : m->cfg.land_dt;
+            dts = check_timestepsize(dts, t, dt);
+            ++m->nsub[1];
             for (int64_t i = 0; i < n; ++i) m->ol_qin[i] = 0.0;
             for (int64_t k = 0; k < n; ++k) { /* kinwave_land_update!  :293-341 */
                 int64_t v = m->order[k];
@@ -1392,7 +1451,11 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
     {
         double t = 0.0;
         while (t < dt) { /* update_river_flow_model!  :613-662 */
-            double dts = check_timestepsize(m->cfg.river_dt, t, dt);
+            double dts = m->cfg.kinematic_wave_adaptive
+                             ? wfo_kw_stable_timestep(m->r_q, m->r_alpha, m->r_fl, nriv, 0.05, m->stbuf)
+                             : m->cfg.river_dt;
+            dts = check_timestepsize(dts, t, dt);
+            ++m->nsub[2];
             for (int64_t r = 0; r < nriv; ++r) m->r_qin[r] = 0.0;
             for (int64_t k = 0; k < nriv; ++k) { /* kinwave_river_update!  :492-566 */
                 int64_t v = m->rorder[k];
@@ -1469,6 +1532,9 @@ int64_t wfo_n(const wfo_model* m) { return m->n; }
 int64_t wfo_nriv(const wfo_model* m) { return m->nriv; }
 int32_t wfo_nlayers(const wfo_model* m) { return m->N; }
 int64_t wfo_kinwave_iterations(const wfo_model* m) { return m->kw_iters; }
+void wfo_last_substeps(const wfo_model* m, int32_t* out3) {
+    for (int k = 0; k < 3; ++k) out3[k] = m->nsub[k];
+}
 
 int32_t wfo_topology(const wfo_model* m, int64_t* land_order, int64_t* land_down, int64_t* river_order,
                      int64_t* river_down, int64_t* river_land_index) {

@@ -59,6 +59,13 @@ int32_t wfo_nlayers(const wfo_model* m);
 int32_t wfo_topology(const wfo_model* m, int64_t* land_order, int64_t* land_down, int64_t* river_order,
                      int64_t* river_down, int64_t* river_land_index);
 int64_t wfo_kinwave_iterations(const wfo_model* m);
+/* subsurface, overland and river sub-step counts of the last wfo_step */
+void wfo_last_substeps(const wfo_model* m, int32_t* out3);
+/* Statistics.quantile!(v, p) of k >= 2 values (sorts v); the kinematic-wave stable time step
+ * (surface_kinwave.jl:674-704) over n cells, buf holding n values. */
+double wfo_quantile(double* v, int64_t k, double p);
+double wfo_kw_stable_timestep(const double* q, const double* alpha, const double* flow_length, int64_t n,
+                              double p, double* buf);
 
 /* Element-wise evaluation of the shared elementary functions (include/wfh_math.h) for the
  * accuracy tests: op 0 exp, 1 log, 2 pow(x, y), 3 cbrt, 4 cld(x, y), 5 round(x; sigdigits=12),

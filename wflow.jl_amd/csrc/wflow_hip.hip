@@ -119,6 +119,31 @@ struct RouteArgs {
     double dts_s[2], dts_l[2], dts_r[2];  // sub-step lengths: first (S-1) steps and the last one
 };
 
+// State of the adaptive sub-steps (one launch per sub-step and family), kept between launches.
+struct AdArgs {
+    double* szi;           // [n] subsurface water table depth
+    double* scum;          // [6][n] subsurface sums (SsfCum order: qin, q, exf, qnet, tor, rech)
+    double* ocum;          // [3][n] overland sums: q, qin, to_river
+    double* rcum;          // [2][nriv] river sums (river-position order): q, qin
+    const int* rp_r;       // [nriv] river index at each river position
+    const int* rp_up_ptr;  // [nriv+1]
+    const int* rp_up_idx;  // [Er] river positions of the upstream river cells, ascending river index
+    const int* rp_down;    // [nriv] river position downstream, or -1
+    int nchunks_r;
+};
+
+// Order-statistic selection over 64-bit keys (the bit patterns of positive doubles order as
+// the values): one 8-bit digit per pass from the top, then the next larger key.
+struct SelCtl {
+    unsigned long long count;      // eligible values
+    unsigned long long rank;       // rank still to find among the keys matching prefix/mask
+    unsigned long long prefix, mask;
+    unsigned long long bin_count;  // keys equal to the selected one (after the last pass)
+    unsigned long long next;       // smallest key above the selected one
+    unsigned long long minkey;     // smallest key (minimum reduction)
+    unsigned int hist[256];
+};
+
 // ============================================================================================
 // K1: vertical column update (update_land_hydrology_model!, Wflow/src/sbm.jl:82-132)
 // ============================================================================================
@@ -659,6 +684,174 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
     return true;
 }
 
+// Per-cell inputs of the closing subsurface bookkeeping of a step: soil parameters and the
+// vertical update's outputs that enter the soil storage update and the balances.
+struct SsfCellIn {
+    double ts, tr, rd, dsoil, area, sy;
+    double sat_exc, runoff_l, inf_exc, aeow, nrr;
+    double P_in, aet, leak, canopy, snow_s, snow_w, store_prev;
+};
+// Sums over the subsurface sub-steps of flux x sub-step length.
+struct SsfCum {
+    double qin, q, exf, qnet, tor, rech;
+};
+struct SsfEnd {
+    double usd, swd, s_tor_avg, o_inwater;
+};
+
+// average_flux_vars! (groundwater.jl:630-636), update_soil_water_storage! (soil.jl:1294-1392),
+// the subsurface and land balances (mass_balance.jl:215-253, 470-490) and the overland lateral
+// inflow (surface_kinwave.jl:740-766) of cell p after its last subsurface sub-step.
+template <int N>
+__device__ __forceinline__ SsfEnd ssf_finish(const Dev& d, const int p, const Column<N>& col, const SsfCellIn& c,
+                                             const SsfCum& cum, const double sq, const double szi) {
+    const int n = d.n;
+    const double dt = d.dt;
+    const double ts = c.ts, tr = c.tr, rd = c.rd, dsoil = c.dsoil, area = c.area;
+    SsfEnd o;
+    const double s_exf_avg = cum.exf / dt;
+    const double s_qin_avg = cum.qin / dt;
+    const double s_q_avg = cum.q / dt;
+    const double s_qnet_avg = cum.qnet / dt;
+    const double rech_avg = cum.rech / dt;
+    o.s_tor_avg = cum.tor / dt;
+    d.ssf_q[p] = sq;
+    d.ssf_qin_avg[p] = s_qin_avg;
+    d.ssf_q_avg[p] = s_q_avg;
+    d.ssf_tor_avg[p] = o.s_tor_avg;
+    d.ssf_storage[p] = c.sy * (dsoil - szi) * area;
+    double usd = 0.0;
+    {
+        bool first = true;
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            if (k < col.n_unsat) {
+                usd = first ? col.ustore[k] : usd + col.ustore[k];
+                first = false;
+            }
+    }
+    const double sbm_runoff = wfh_max(0.0, s_exf_avg + c.sat_exc + c.runoff_l + c.inf_exc);
+    int nl = 0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) nl += !wfh_isnan(col.act[k]);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double v = NaN_;
+        if (k < nl) {
+            if (k < col.n_unsat)
+                v = (col.ustore[k] + (col.act[k] - col.thick[k]) * (ts - tr)) / col.act[k] + tr;
+            else
+                v = ts;
+        }
+        d.vwc[k * n + p] = v;
+    }
+    double rs_unsat = 0.0;
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+        if (k < col.n_unsat) rs_unsat += wfh_min(1.0, (wfh_max(0.0, rd - col.cum[k]) / col.thick[k])) * col.ustore[k];
+    const double rs_sat = wfh_max(0.0, rd - col.zi) * (ts - tr);
+    const double rzs = rs_sat + rs_unsat;
+    const double swd = (dsoil - col.zi) * (ts - tr);
+    d.ustoredepth[p] = usd;
+    d.swd[p] = swd;
+    d.tsws[p] = swd + usd;
+    d.exfilt[p] = s_exf_avg;
+    d.runoff[p] = sbm_runoff;
+    d.rzs[p] = rzs;
+    d.vwc_root[p] = rzs / rd + tr;
+    const double net_runoff = sbm_runoff - c.aeow;
+    d.net_runoff[p] = net_runoff;
+    d.zi[p] = col.zi;
+#pragma unroll
+    for (int k = 0; k < N; ++k) d.ustore[k * n + p] = col.ustore[k];
+    if (d.balance) {
+        double fin = 0.0, fout = 0.0;  // mass_balance.jl:470-490
+        if (rech_avg > 0.0)
+            fin += rech_avg;
+        else
+            fout -= rech_avg;
+        d.bal_ssf[p] = (s_qin_avg + fin) - (s_q_avg + fout) - s_qnet_avg;
+        // land balance  mass_balance.jl:215-253
+        const double t_in = s_qin_avg / area + (c.P_in + 0.0) + 0.0 + 0.0 / area;
+        const double vout = net_runoff + c.aet + c.nrr + c.leak + 0.0;
+        const double t_out = s_q_avg / area + vout + 0.0 + 0.0 / area;
+        const double snow = d.snow ? c.snow_s + c.snow_w : 0.0 + 0.0;
+        const double storage = (swd + usd) + c.canopy + snow + 0.0 * 0.0 + 0.0;
+        const double rate = (storage - c.store_prev) / dt;
+        const double err = t_in - t_out - rate;
+        const double avg = (t_in + t_out) / 2.0;
+        d.bal_land[p] = err;
+        d.bal_land_rel[p] = avg == 0.0 ? 0.0 : err / avg;
+    }
+    // overland lateral inflow  surface_kinwave.jl:740-766
+    o.o_inwater = (net_runoff + 0.0) * area + 0.0;
+    d.ol_inwater[p] = o.o_inwater;
+    o.usd = usd;
+    o.swd = swd;
+    return o;
+}
+
+// average_flow_vars! of the overland model, its balance (mass_balance.jl:382-399) and the river
+// lateral inflow of a river cell (surface_kinwave.jl:710-734) after cell p's last overland
+// sub-step; returns the river lateral inflow (0 for a cell without river).
+__device__ __forceinline__ double ol_finish(const Dev& d, const int p, const int r, const double oq, const double oh,
+                                            const double o_storage, const double oqin, const double o_q_cum,
+                                            const double o_tor_cum, const double o_qin_cum, const double o_inwater,
+                                            const double o_store_prev, const double s_tor_avg, const double nrr,
+                                            const double area, const double rarea) {
+    const double dt = d.dt;
+    const double q_avg = o_q_cum / dt;
+    const double o_tor_avg = o_tor_cum / dt;
+    const double qin_avg = o_qin_cum / dt;
+    d.ol_q[p] = oq;
+    d.ol_h[p] = oh;
+    d.ol_storage[p] = o_storage;
+    d.ol_qin[p] = oqin;
+    d.ol_q_avg[p] = q_avg;
+    d.ol_tor_avg[p] = o_tor_avg;
+    d.ol_qin_avg[p] = qin_avg;
+    if (d.balance) d.bal_ol[p] = (o_inwater + qin_avg) - q_avg - (o_storage - o_store_prev) / dt;
+    double r_inwater = 0.0;
+    if (r >= 0) {
+        r_inwater = (s_tor_avg + o_tor_avg + nrr * area) + 0.0 * rarea;
+        d.r_inwater[r] = r_inwater;
+    }
+    return r_inwater;
+}
+
+// average_flow_vars! of the river model and its balance (mass_balance.jl:290-313) after river
+// cell r's last sub-step.
+__device__ __forceinline__ void riv_finish(const Dev& d, const int r, const double rq, const double rh,
+                                           const double r_storage, const double rqin, const double r_q_cum,
+                                           const double r_qin_cum, const double r_inwater, const double r_store_prev) {
+    const double dt = d.dt;
+    const double q_avg = r_q_cum / dt, qin_avg = r_qin_cum / dt;
+    d.r_q[r] = rq;
+    d.r_h[r] = rh;
+    d.r_storage[r] = r_storage;
+    d.r_qin[r] = rqin;
+    d.r_q_avg[r] = q_avg;
+    d.r_qin_avg[r] = qin_avg;
+    if (d.balance)
+        d.bal_r[r] = (r_inwater + qin_avg + wfh_max(0.0, 0.0)) - (q_avg + 0.0 + 0.0) - (r_storage - r_store_prev) / dt;
+}
+
+// update_total_water_storage!  sbm.jl:143-182
+__device__ __forceinline__ void total_storage_finish(const Dev& d, const int p, const bool riv, const double rh,
+                                                     const double rfw, const double rfl, const double area,
+                                                     const double snow_s, const double snow_w, const double canopy,
+                                                     const double usd, const double swd, const double oh,
+                                                     const double rfrac) {
+    double tot = 0.0;
+    if (riv) tot = (rh * rfw * rfl) / (area);
+    const double ss = d.snow ? snow_s : 0.0, sw = d.snow ? snow_w : 0.0;
+    tot = tot + ((((ss + sw) + 0.0 * 0.0) + canopy) + 0.0);
+    const double sub = usd + swd;
+    const double lat = oh * (1.0 - rfrac);
+    tot += sub + lat;
+    d.total_storage[p] = tot;
+}
+
 template <int N>
 __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -753,7 +946,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         col.theta_e = ts - tr;
         col.theta_fc_r = tfc - tr;
         double sq = d.ssf_q[q_], szi = col.zi;  // sbm_model.jl:80 hands the soil water table over
-        double s_qin_cum = 0.0, s_q_cum = 0.0, s_exf_cum = 0.0, s_qnet_cum = 0.0, s_tor_cum = 0.0, rech_cum = 0.0;
+        SsfCum cum{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         // Inflow-independent part of the first sub-step's solve, evaluated before the chain
         // reaches this cell; the closing soil-column update of the last sub-step is applied
         // after the family (nothing in the family reads the column after that solve).
@@ -766,9 +959,9 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             // flux!(::RechargeModel) with check_flux  boundary_conditions.jl:12-21,219-236
             double flx = rech * area;
             if (szi >= dsoil) flx = wfh_max(0.0, flx);
-            rech_cum += flx * dts;
+            cum.rech += flx * dts;
             const double qnb = 0.0 + flx;
-            s_tor_cum += tor * dts;
+            cum.tor += tor * dts;
             double qn, zin, exf, net;
             if (kinematic_wave_ssf<N>(qin, sq, szi, qnb, slope, sy, dsoil, dts, flen, fw, qmax, kh0, fk, pp, col,
                                       k == a.S_s - 1, qn, zin, exf, net)) {
@@ -778,10 +971,10 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             }
             sq = qn;
             szi = zin;
-            s_qin_cum += qin * dts;
-            s_q_cum += qn * dts;
-            s_exf_cum += exf * dts;
-            s_qnet_cum += net * area * dts;
+            cum.qin += qin * dts;
+            cum.q += qn * dts;
+            cum.exf += exf * dts;
+            cum.qnet += net * area * dts;
             return qn;
         };
         if (!run_family<FOLD_SSF>(a, lane, c, valid, a.S_s, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ssf,
@@ -790,87 +983,13 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         if (col_upd) update_ustorelayerdepth<N>(col, col_z0, col_z1);
 
         // ---------------- average_flux_vars!, update_soil_water_storage! ----------------
-        double net_runoff = 0.0, o_inwater = 0.0, s_tor_avg = 0.0;
-        double usd = 0.0, swd = 0.0;
-        if (valid) {
-            const double s_exf_avg = s_exf_cum / dt;
-            const double s_qin_avg = s_qin_cum / dt;
-            const double s_q_avg = s_q_cum / dt;
-            const double s_qnet_avg = s_qnet_cum / dt;
-            const double rech_avg = rech_cum / dt;
-            s_tor_avg = s_tor_cum / dt;
-            d.ssf_q[p] = sq;
-            d.ssf_qin_avg[p] = s_qin_avg;
-            d.ssf_q_avg[p] = s_q_avg;
-            d.ssf_tor_avg[p] = s_tor_avg;
-            d.ssf_storage[p] = sy * (dsoil - szi) * area;
-            {
-                bool first = true;
-#pragma unroll
-                for (int k = 0; k < N; ++k)
-                    if (k < col.n_unsat) {
-                        usd = first ? col.ustore[k] : usd + col.ustore[k];
-                        first = false;
-                    }
-            }
-            const double sbm_runoff = wfh_max(0.0, s_exf_avg + sat_exc + runoff_l + inf_exc);
-            int nl = 0;
-#pragma unroll
-            for (int k = 0; k < N; ++k) nl += !wfh_isnan(col.act[k]);
-#pragma unroll
-            for (int k = 0; k < N; ++k) {
-                double v = NaN_;
-                if (k < nl) {
-                    if (k < col.n_unsat)
-                        v = (col.ustore[k] + (col.act[k] - col.thick[k]) * (ts - tr)) / col.act[k] + tr;
-                    else
-                        v = ts;
-                }
-                d.vwc[k * n + p] = v;
-            }
-            double rs_unsat = 0.0;
-#pragma unroll
-            for (int k = 0; k < N; ++k)
-                if (k < col.n_unsat)
-                    rs_unsat += wfh_min(1.0, (wfh_max(0.0, rd - col.cum[k]) / col.thick[k])) * col.ustore[k];
-            const double rs_sat = wfh_max(0.0, rd - col.zi) * (ts - tr);
-            const double rzs = rs_sat + rs_unsat;
-            swd = (dsoil - col.zi) * (ts - tr);
-            d.ustoredepth[p] = usd;
-            d.swd[p] = swd;
-            d.tsws[p] = swd + usd;
-            d.exfilt[p] = s_exf_avg;
-            d.runoff[p] = sbm_runoff;
-            d.rzs[p] = rzs;
-            d.vwc_root[p] = rzs / rd + tr;
-            net_runoff = sbm_runoff - aeow;
-            d.net_runoff[p] = net_runoff;
-            d.zi[p] = col.zi;
-#pragma unroll
-            for (int k = 0; k < N; ++k) d.ustore[k * n + p] = col.ustore[k];
-            if (d.balance) {
-                double fin = 0.0, fout = 0.0;  // mass_balance.jl:470-490
-                if (rech_avg > 0.0)
-                    fin += rech_avg;
-                else
-                    fout -= rech_avg;
-                d.bal_ssf[p] = (s_qin_avg + fin) - (s_q_avg + fout) - s_qnet_avg;
-                // land balance  mass_balance.jl:215-253
-                const double t_in = s_qin_avg / area + (P_in + 0.0) + 0.0 + 0.0 / area;
-                const double vout = net_runoff + aet + nrr + leak + 0.0;
-                const double t_out = s_q_avg / area + vout + 0.0 + 0.0 / area;
-                const double snow = d.snow ? snow_s + snow_w : 0.0 + 0.0;
-                const double storage = (swd + usd) + canopy + snow + 0.0 * 0.0 + 0.0;
-                const double rate = (storage - store_prev) / dt;
-                const double err = t_in - t_out - rate;
-                const double avg = (t_in + t_out) / 2.0;
-                d.bal_land[p] = err;
-                d.bal_land_rel[p] = avg == 0.0 ? 0.0 : err / avg;
-            }
-            // overland lateral inflow  surface_kinwave.jl:740-766
-            o_inwater = (net_runoff + 0.0) * area + 0.0;
-            d.ol_inwater[p] = o_inwater;
-        }
+        SsfEnd se{0.0, 0.0, 0.0, 0.0};
+        if (valid)
+            se = ssf_finish<N>(d, p, col,
+                               SsfCellIn{ts, tr, rd, dsoil, area, sy, sat_exc, runoff_l, inf_exc, aeow, nrr, P_in, aet,
+                                         leak, canopy, snow_s, snow_w, store_prev},
+                               cum, sq, szi);
+        const double o_inwater = se.o_inwater, s_tor_avg = se.s_tor_avg, usd = se.usd, swd = se.swd;
 
         // ---------------- overland kinematic wave ----------------
         const double oqlat = o_inwater / flen;
@@ -896,24 +1015,10 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         if (!run_family<FOLD_OL>(a, lane, c, valid, a.S_l, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ol, (size_t)n,
                                  p, (fl & 1) && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q, ol_compute))
             return;
-        double r_inwater = 0.0, o_tor_avg = 0.0;
-        if (valid) {
-            const double q_avg = o_q_cum / dt;
-            o_tor_avg = o_tor_cum / dt;
-            const double qin_avg = o_qin_cum / dt;
-            d.ol_q[p] = oq;
-            d.ol_h[p] = oh;
-            d.ol_storage[p] = o_storage;
-            d.ol_qin[p] = oqin;
-            d.ol_q_avg[p] = q_avg;
-            d.ol_tor_avg[p] = o_tor_avg;
-            d.ol_qin_avg[p] = qin_avg;
-            if (d.balance) d.bal_ol[p] = (o_inwater + qin_avg) - q_avg - (o_storage - o_store_prev) / dt;
-            if (r >= 0) {  // river lateral inflow  surface_kinwave.jl:710-734
-                r_inwater = (s_tor_avg + o_tor_avg + nrr * area) + 0.0 * rarea;
-                d.r_inwater[r] = r_inwater;
-            }
-        }
+        double r_inwater = 0.0;
+        if (valid)
+            r_inwater = ol_finish(d, p, r, oq, oh, o_storage, oqin, o_q_cum, o_tor_cum, o_qin_cum, o_inwater,
+                                  o_store_prev, s_tor_avg, nrr, area, rarea);
 
         // ---------------- river kinematic wave ----------------
         const bool rcons_in = riv && (fl & 4) && cons_in;  // a river cell drains to the land downstream
@@ -942,32 +1047,394 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
                                   (size_t)d.nriv, rr, riv && (fl & 4) && !rcons_in, rcons_in ? (dn & 63) : -1,
                                   lds_k, lds_q, riv_compute))
             return;
-        if (riv) {
-            const double q_avg = r_q_cum / dt, qin_avg = r_qin_cum / dt;
-            d.r_q[r] = rq;
-            d.r_h[r] = rh;
-            d.r_storage[r] = r_storage;
-            d.r_qin[r] = rqin;
-            d.r_q_avg[r] = q_avg;
-            d.r_qin_avg[r] = qin_avg;
-            if (d.balance)  // mass_balance.jl:290-313
-                d.bal_r[r] = (r_inwater + qin_avg + wfh_max(0.0, 0.0)) - (q_avg + 0.0 + 0.0) -
-                             (r_storage - r_store_prev) / dt;
-        }
+        if (riv) riv_finish(d, r, rq, rh, r_storage, rqin, r_q_cum, r_qin_cum, r_inwater, r_store_prev);
 
         // ---------------- update_total_water_storage!  sbm.jl:143-182 ----------------
-        if (valid) {
-            double tot = 0.0;
-            if (riv) tot = (rh * rfw * rfl) / (area);
-            const double ss = d.snow ? snow_s : 0.0, sw = d.snow ? snow_w : 0.0;
-            tot = tot + ((((ss + sw) + 0.0 * 0.0) + canopy) + 0.0);
-            const double sub = usd + swd;
-            const double lat = oh * (1.0 - rfrac);
-            tot += sub + lat;
-            d.total_storage[p] = tot;
+        if (valid) total_storage_finish(d, p, riv, rh, rfw, rfl, area, snow_s, snow_w, canopy, usd, swd, oh, rfrac);
+    }
+}
+// ============================================================================================
+// Adaptive sub-steps (kinematic_wave__adaptive_time_step_flag): each sub-step's length is a
+// quantile (overland, river) or minimum (subsurface) over the state the previous sub-step left
+// in every cell, so sub-steps of one family cannot overlap; every sub-step is one dataflow
+// launch over the cells (the river one over river cells only), with the carried state in
+// memory. The loop over sub-steps runs on the host (update_subsurface_flow_model!
+// lateral_subsurface_flow.jl:279-304, update_overland_flow_model! surface_kinwave.jl:347-385,
+// update_river_flow_model! :613-662).
+// ============================================================================================
+#define AD_TICKET(NCHUNKS)                                                       \
+    unsigned long long t_ = 0;                                                   \
+    if (lane == 0) t_ = atomicAdd(&a.counters[a.parity], 1ull);                  \
+    t_ = __shfl(t_, 0);                                                          \
+    if (t_ >= (unsigned long long)(NCHUNKS)) break;                              \
+    const int c = (int)t_;
+
+// One subsurface sub-step of length dts (first: the step's first sub-step, last: its last).
+template <int N>
+__global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_ssf(Dev d, RouteArgs a, AdArgs x, double dts, int first,
+                                                        int last) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    __shared__ int x_k[ROUTE_BLOCK];
+    __shared__ double x_q[ROUTE_BLOCK];
+    int* const lds_k = x_k + (threadIdx.x & ~(WAVE - 1));
+    double* const lds_q = x_q + (threadIdx.x & ~(WAVE - 1));
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[a.parity ^ 1] = 0ull;
+    const int n = d.n;
+    for (;;) {
+        AD_TICKET(a.nchunks)
+        const int p = c * WAVE + lane;
+        const bool valid = p < n;
+        const int q_ = valid ? p : 0;
+        const int dn_ = d.down[q_], e0 = d.up_ptr[q_], e1_ = d.up_ptr[q_ + 1];
+        const uint8_t fl_ = d.flags[q_];
+        const int dn = valid ? dn_ : -1;
+        const bool cons_in = dn >= 0 && (dn >> 6) == c;
+        const int e1 = valid ? e1_ : e0;
+        const uint8_t fl = valid ? fl_ : 0;
+        const double area = ld(d.area, q_);
+        const double dsoil = ld(d.d, q_), ts = ld(d.ts, q_), tr = ld(d.tr, q_), tfc = ld(d.tfc, q_);
+        const double sy = ld(d.sy, q_), slope = ld(d.slope, q_), flen = ld(d.fl, q_), fw = ld(d.fw, q_);
+        const double qmax = ld(d.qmax, q_), kh0 = ld(d.kh0, q_), fk = ld(d.f, q_);
+        const double rech = d.recharge[q_];
+        Column<N> col;
+        soil_layers<N>(dsoil, d.cfg_cum, d.cfg_thick, col.act, col.cum);
+#pragma unroll
+        for (int k = 0; k < N; ++k) col.ustore[k] = d.ustore[k * n + q_];
+        col.zi = d.zi[q_];
+        col.n_unsat = set_layerthickness<N>(col.zi, col.cum, col.act, col.thick);
+        col.theta_e = ts - tr;
+        col.theta_fc_r = tfc - tr;
+        double sq = d.ssf_q[q_], szi = x.szi[q_];
+        SsfCum cum{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (!first) {
+            cum.qin = x.scum[q_];
+            cum.q = x.scum[(size_t)n + q_];
+            cum.exf = x.scum[2 * (size_t)n + q_];
+            cum.qnet = x.scum[3 * (size_t)n + q_];
+            cum.tor = x.scum[4 * (size_t)n + q_];
+            cum.rech = x.scum[5 * (size_t)n + q_];
+        }
+        const SsfPrep pp = ssf_prepare(sq, szi, slope, sy, dts, flen, kh0, fk);
+        bool col_upd = false;
+        double col_z0 = 0.0, col_z1 = 0.0;
+        auto ssf_compute = [&](int, double qin, double tor) -> double {
+            // flux!(::RechargeModel) with check_flux  boundary_conditions.jl:12-21,219-236
+            double flx = rech * area;
+            if (szi >= dsoil) flx = wfh_max(0.0, flx);
+            cum.rech += flx * dts;
+            const double qnb = 0.0 + flx;
+            cum.tor += tor * dts;
+            double qn, zin, exf, net;
+            if (kinematic_wave_ssf<N>(qin, sq, szi, qnb, slope, sy, dsoil, dts, flen, fw, qmax, kh0, fk, pp, col, true,
+                                      qn, zin, exf, net)) {
+                col_upd = true;
+                col_z0 = szi;
+                col_z1 = zin;
+            }
+            sq = qn;
+            szi = zin;
+            cum.qin += qin * dts;
+            cum.q += qn * dts;
+            cum.exf += exf * dts;
+            cum.qnet += net * area * dts;
+            return qn;
+        };
+        if (!run_family<FOLD_SSF>(a, lane, c, valid, 1, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ssf, (size_t)n, p,
+                                  (fl & 1) && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q, ssf_compute))
+            return;
+        if (col_upd) update_ustorelayerdepth<N>(col, col_z0, col_z1);
+        if (valid && !last) {
+            d.ssf_q[p] = sq;
+            x.szi[p] = szi;
+            d.zi[p] = col.zi;
+#pragma unroll
+            for (int k = 0; k < N; ++k) d.ustore[k * n + p] = col.ustore[k];
+            x.scum[p] = cum.qin;
+            x.scum[(size_t)n + p] = cum.q;
+            x.scum[2 * (size_t)n + p] = cum.exf;
+            x.scum[3 * (size_t)n + p] = cum.qnet;
+            x.scum[4 * (size_t)n + p] = cum.tor;
+            x.scum[5 * (size_t)n + p] = cum.rech;
+        } else if (valid) {
+            const double P_in = d.balance ? d.P[p] : 0.0, aet = d.balance ? d.aet[p] : 0.0;
+            const double leak = d.balance ? d.leakage[p] : 0.0, sp = d.balance ? d.store_prev[p] : 0.0;
+            const double ss = d.snow ? d.snow_s[p] : 0.0, sw = d.snow ? d.snow_w[p] : 0.0;
+            ssf_finish<N>(d, p, col,
+                          SsfCellIn{ts, tr, ld(d.rd, p), dsoil, area, sy, d.sat_excess[p], d.runoff_land[p],
+                                    d.inf_excess[p], d.aeow_l[p], d.net_runoff_river[p], P_in, aet, leak, d.canopy[p],
+                                    ss, sw, sp},
+                          cum, sq, szi);
         }
     }
 }
+
+// One overland sub-step of length dts.
+__global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_ol(Dev d, RouteArgs a, AdArgs x, double dts, int first,
+                                                       int last) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    __shared__ int x_k[ROUTE_BLOCK];
+    __shared__ double x_q[ROUTE_BLOCK];
+    int* const lds_k = x_k + (threadIdx.x & ~(WAVE - 1));
+    double* const lds_q = x_q + (threadIdx.x & ~(WAVE - 1));
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[a.parity ^ 1] = 0ull;
+    const int n = d.n;
+    for (;;) {
+        AD_TICKET(a.nchunks)
+        const int p = c * WAVE + lane;
+        const bool valid = p < n;
+        const int q_ = valid ? p : 0;
+        const int dn_ = d.down[q_], e0 = d.up_ptr[q_], e1_ = d.up_ptr[q_ + 1];
+        const uint8_t fl_ = d.flags[q_];
+        const int dn = valid ? dn_ : -1;
+        const bool cons_in = dn >= 0 && (dn >> 6) == c;
+        const int e1 = valid ? e1_ : e0;
+        const uint8_t fl = valid ? fl_ : 0;
+        const double alpha = ld(d.alpha, q_), sfw = ld(d.sfw, q_), flen = ld(d.fl, q_);
+        double oq = d.ol_q[q_], oh = d.ol_h[q_];
+        const double o_inwater = d.ol_inwater[q_];
+        double o_q_cum = 0.0, o_qin_cum = 0.0, o_tor_cum = 0.0;
+        if (!first) {
+            o_q_cum = x.ocum[q_];
+            o_qin_cum = x.ocum[(size_t)n + q_];
+            o_tor_cum = x.ocum[2 * (size_t)n + q_];
+        }
+        const double oqlat = o_inwater / flen;
+        double o_storage = 0.0, oqin = 0.0;
+        const KwPrep op = kw_prepare(oq, oqlat, alpha, dts, flen);
+        auto ol_compute = [&](int, double tor, double qin_all) -> double {
+            o_tor_cum += tor * dts;
+            double qin = 0.0;
+            if (sfw > 0.0) qin = qin_all;
+            double qn, ar;
+            kw_solve(qin, op, alpha, qn, ar);
+            oq = qn;
+            if (sfw > 0.0) oh = ar / sfw;
+            o_storage = flen * sfw * oh;
+            o_q_cum += qn * dts;
+            o_qin_cum += qin * dts;
+            oqin = qin;
+            return qn;
+        };
+        if (!run_family<FOLD_OL>(a, lane, c, valid, 1, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ol, (size_t)n, p,
+                                 (fl & 1) && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q, ol_compute))
+            return;
+        if (valid && !last) {
+            d.ol_q[p] = oq;
+            d.ol_h[p] = oh;
+            x.ocum[p] = o_q_cum;
+            x.ocum[(size_t)n + p] = o_qin_cum;
+            x.ocum[2 * (size_t)n + p] = o_tor_cum;
+        } else if (valid) {
+            const int r = d.river_of[p];
+            const double o_store_prev = d.ol_storage[p];
+            ol_finish(d, p, r, oq, oh, o_storage, oqin, o_q_cum, o_tor_cum, o_qin_cum, o_inwater, o_store_prev,
+                      d.ssf_tor_avg[p], d.net_runoff_river[p], ld(d.area, p), r >= 0 ? d.r_area[r] : 0.0);
+        }
+    }
+}
+
+// One river sub-step of length dts, over the river cells in river-position order (river cells
+// sorted by node, a topological order of the river graph).
+__global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_riv(Dev d, RouteArgs a, AdArgs x, double dts, int first,
+                                                        int last) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    __shared__ int x_k[ROUTE_BLOCK];
+    __shared__ double x_q[ROUTE_BLOCK];
+    int* const lds_k = x_k + (threadIdx.x & ~(WAVE - 1));
+    double* const lds_q = x_q + (threadIdx.x & ~(WAVE - 1));
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[a.parity ^ 1] = 0ull;
+    const int nr = d.nriv;
+    for (;;) {
+        AD_TICKET(x.nchunks_r)
+        const int rp = c * WAVE + lane;
+        const bool valid = rp < nr;
+        const int q_ = valid ? rp : 0;
+        const int r_ = x.rp_r[q_], rdn_ = x.rp_down[q_], e0 = x.rp_up_ptr[q_], e1_ = x.rp_up_ptr[q_ + 1];
+        const int r = r_;
+        const int rdn = valid ? rdn_ : -1;
+        const bool cons_in = rdn >= 0 && (rdn >> 6) == c;
+        const int e1 = valid ? e1_ : e0;
+        const double ralpha = d.r_alpha[r], rfw = d.r_fw[r], rfl = d.r_fl[r];
+        double rq = d.r_q[r], rh = d.r_h[r];
+        const double r_inwater = d.r_inwater[r];
+        double r_q_cum = 0.0, r_qin_cum = 0.0;
+        if (!first) {
+            r_q_cum = x.rcum[q_];
+            r_qin_cum = x.rcum[(size_t)nr + q_];
+        }
+        const double rqlat = r_inwater / rfl;
+        double inflow = 0.0 / rfl;  // reservoir / lake / external inflow terms (none in this model)
+        inflow -= 0.0 / rfl;
+        const double rqlat_all = rqlat + inflow;
+        double r_storage = 0.0, rqin = 0.0;
+        const KwPrep rpp = kw_prepare(rq, rqlat_all, ralpha, dts, rfl);
+        auto riv_compute = [&](int, double sumq, double) -> double {
+            const double qin = 0.0 + sumq;
+            double qn, ar;
+            kw_solve(qin, rpp, ralpha, qn, ar);
+            rq = qn;
+            rh = ar / rfw;
+            r_storage = rfl * ar;
+            r_q_cum += qn * dts;
+            r_qin_cum += qin * dts;
+            rqin = qin;
+            return qn;
+        };
+        if (!run_family<FOLD_RIV>(a, lane, c, valid, 1, x.rp_up_idx, x.rp_up_idx, nullptr, e0, e1, a.slot_r,
+                                  (size_t)nr, rp, rdn >= 0 && !cons_in, cons_in ? (rdn & 63) : -1, lds_k, lds_q,
+                                  riv_compute))
+            return;
+        if (valid && !last) {
+            d.r_q[r] = rq;
+            d.r_h[r] = rh;
+            x.rcum[rp] = r_q_cum;
+            x.rcum[(size_t)nr + rp] = r_qin_cum;
+        } else if (valid) {
+            riv_finish(d, r, rq, rh, r_storage, rqin, r_q_cum, r_qin_cum, r_inwater, d.r_storage[r]);
+        }
+    }
+}
+
+// update_total_water_storage! after the adaptive river sub-steps.
+__global__ void k_ad_total(Dev d) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.n) return;
+    const int r = d.river_of[p];
+    const bool riv = r >= 0;
+    const int rr = riv ? r : 0;
+    total_storage_finish(d, p, riv, riv ? d.r_h[rr] : 0.0, riv ? d.r_fw[rr] : 0.0, riv ? d.r_fl[rr] : 0.0,
+                         ld(d.area, p), d.snow_s[p], d.snow_w[p], d.canopy[p], d.ustoredepth[p], d.swd[p], d.ol_h[p],
+                         ld(d.rf, p));
+}
+
+// ---- stable time steps ----
+__global__ void k_sel_reset(SelCtl* s) {
+    s->hist[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) {
+        s->count = 0ull;
+        s->rank = 0ull;
+        s->prefix = 0ull;
+        s->mask = 0ull;
+        s->bin_count = 0ull;
+        s->next = ~0ull;
+        s->minkey = ~0ull;
+    }
+}
+
+// Courant time step flow_length / celerity of every cell with q > KIN_WAVE_MIN_FLOW as a key
+// (all-ones for the others), and their count.  surface_kinwave.jl:684-693
+__global__ void k_kw_keys(const double* __restrict__ q, Prm alpha, Prm flen, int m, uint64_t* keys, SelCtl* s) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool elig = false;
+    if (i < m) {
+        const double qi = q[i];
+        uint64_t key = ~0ull;
+        if (qi > WFH_KIN_WAVE_MIN_FLOW) {
+            const double cel = 1.0 / (ld(alpha, i) * 0.6 * wfh_pow(qi, (0.6 - 1.0)));
+            key = wfh_bits(ld(flen, i) / cel);
+            elig = true;
+        }
+        keys[i] = key;
+    }
+    const unsigned long long b = __ballot(elig);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && b) atomicAdd(&s->count, (unsigned long long)__popcll(b));
+}
+
+// Rank of the lower order statistic of quantile!(v, p) (0-based), from the count.
+__global__ void k_sel_init(SelCtl* s, double p) {
+    const unsigned long long k = s->count;
+    if (k >= 2) {
+        double g;
+        s->rank = (unsigned long long)(wfh_quantile_index((int64_t)k, p, &g) - 1);
+    } else {
+        s->rank = 0ull;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sel_hist(const uint64_t* __restrict__ keys, int m, SelCtl* s, int shift) {
+    __shared__ unsigned int h[256];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const unsigned long long mask = s->mask, prefix = s->prefix;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) {
+        const uint64_t key = keys[i];
+        if ((key & mask) == prefix) atomicAdd(&h[(key >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&s->hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// The digit whose bin holds the wanted rank; clears the histogram for the next pass.
+__global__ void k_sel_pick(SelCtl* s, int shift) {
+    __shared__ unsigned int h[256];
+    h[threadIdx.x] = s->hist[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long r = s->rank;
+        unsigned long long cum = 0ull;  // keys in the bins below B
+        int B = -1;
+        for (int b = 0; b < 256; ++b) {
+            if (r < cum + h[b]) {
+                B = b;
+                break;
+            }
+            cum += h[b];
+        }
+        if (B < 0) {  // rank beyond the keys (no eligible value): any bin, the result is unused
+            B = 255;
+            cum = r;
+        }
+        s->rank = r - cum;
+        s->prefix |= (unsigned long long)B << shift;
+        s->mask |= 0xFFull << shift;
+        s->bin_count = h[B];
+    }
+    s->hist[threadIdx.x] = 0u;
+}
+
+// Smallest key above the selected one, needed when the selected key's multiplicity ends at it.
+__global__ void __launch_bounds__(256) k_sel_next(const uint64_t* __restrict__ keys, int m, SelCtl* s) {
+    if (s->rank + 1 < s->bin_count) return;
+    __shared__ unsigned long long mn;
+    if (threadIdx.x == 0) mn = ~0ull;
+    __syncthreads();
+    const unsigned long long a = s->prefix;
+    unsigned long long best = ~0ull;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) {
+        const uint64_t key = keys[i];
+        if (key > a && key < best) best = key;
+    }
+    atomicMin(&mn, best);
+    __syncthreads();
+    if (threadIdx.x == 0 && mn != ~0ull) atomicMin(&s->next, mn);
+}
+
+// Subsurface Courant time steps: minimum of flow_length / ssf_celerity over the cells with a
+// positive water table depth.  lateral_subsurface_flow.jl:314-344
+__global__ void __launch_bounds__(256) k_ssf_min(Dev d, AdArgs x, SelCtl* s) {
+    __shared__ unsigned long long mn;
+    __shared__ unsigned int cnt;
+    if (threadIdx.x == 0) {
+        mn = ~0ull;
+        cnt = 0u;
+    }
+    __syncthreads();
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < d.n) {
+        const double z = x.szi[i];
+        if (z > 0.0) {
+            const double cel = (ld(d.kh0, i) * wfh_exp(-ld(d.f, i) * z) * ld(d.slope, i)) / ld(d.sy, i);
+            atomicMin(&mn, (unsigned long long)wfh_bits(ld(d.fl, i) / cel));
+            atomicAdd(&cnt, 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && cnt) {
+        atomicMin(&s->minkey, mn);
+        atomicAdd(&s->count, (unsigned long long)cnt);
+    }
+}
+
 // --------------------------------------------------------------------------------------------
 __global__ void k_fill_u64(uint64_t* p, uint64_t v, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
@@ -1014,6 +1481,12 @@ struct wfh_handle {
     double *fP = nullptr, *fE = nullptr, *fT = nullptr;  // node-order forcing buffers
     int* d_land_of = nullptr;
     std::vector<double> staging;
+    // adaptive sub-steps
+    AdArgs ad{};
+    SelCtl* sel = nullptr;       // device
+    SelCtl* sel_host = nullptr;  // pinned read-back
+    uint64_t* keys = nullptr;    // [max(n, nriv)]
+    int32_t nsub[3]{};           // sub-steps of the last step
 };
 
 namespace {
@@ -1267,9 +1740,12 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
     h->cfg = *cfg;
     const int64_t n = st->n;
     const int N = cfg->n_config_layers + 1;
-    if (n < 1 || n > (int64_t)INT32_MAX / 2 || N < 2 || N > 6 || cfg->dt <= 0.0 || cfg->land_dt <= 0.0 ||
-        cfg->river_dt <= 0.0 || cfg->subsurface_dt <= 0.0) {
-        h->err = "unsupported sizes (1 <= n < 2^30, 1 <= soil_layer__thickness count <= 5) or non-positive time step";
+    const bool adaptive = cfg->kinematic_wave_adaptive != 0;
+    if (n < 1 || n > (int64_t)INT32_MAX / 2 || N < 2 || N > 6 || !(cfg->dt > 0.0) ||
+        (!adaptive && !(cfg->land_dt > 0.0 && cfg->river_dt > 0.0 && cfg->subsurface_dt > 0.0)) ||
+        (adaptive && !(cfg->subsurface_alpha_coefficient > 0.0 && cfg->subsurface_alpha_coefficient < 1e300))) {
+        h->err = "unsupported sizes (1 <= n < 2^30, 1 <= soil_layer__thickness count <= 5), non-positive time step "
+                 "or non-positive subsurface alpha coefficient";
         wfh_destroy_keep_error(h, out);
         return WFH_ERR_ARG;
     }
@@ -1697,10 +2173,16 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         }
     }
     {
-        // routing sub-steps (fixed kinematic-wave time steps, config_structure.jl:84-88)
-        h->S_s = sub_steps(cfg->dt, cfg->subsurface_dt, h->dts_s);
-        h->S_l = sub_steps(cfg->dt, cfg->land_dt, h->dts_l);
-        h->S_r = sub_steps(cfg->dt, cfg->river_dt, h->dts_r);
+        // routing sub-steps (fixed kinematic-wave time steps, config_structure.jl:84-88); the
+        // adaptive launches take one sub-step each
+        if (adaptive) {
+            h->S_s = h->S_l = h->S_r = 1;
+            for (int k = 0; k < 2; ++k) h->dts_s[k] = h->dts_l[k] = h->dts_r[k] = cfg->dt;
+        } else {
+            h->S_s = sub_steps(cfg->dt, cfg->subsurface_dt, h->dts_s);
+            h->S_l = sub_steps(cfg->dt, cfg->land_dt, h->dts_l);
+            h->S_r = sub_steps(cfg->dt, cfg->river_dt, h->dts_r);
+        }
         hipError_t e;
         RouteArgs& A = h->ra;
         A.slot_ssf = dalloc<uint64_t>(h, (size_t)h->S_s * n, e);
@@ -1746,6 +2228,52 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         if (cus < 1) cus = 1;
         const int want = (h->nchunks + (ROUTE_BLOCK / WAVE) - 1) / (ROUTE_BLOCK / WAVE);
         h->route_grid = std::max(1, std::min(want, per_cu * cus));
+        if (adaptive) {
+            // river positions: river cells sorted by node, a topological order of the river graph
+            // because every river edge is a land edge and node order is topological
+            AdArgs& X = h->ad;
+            std::vector<std::pair<int, int>> nr(nriv);
+            for (int64_t r = 0; r < nriv; ++r) nr[r] = {h->node_of[h->river_land[r]], (int)r};
+            std::sort(nr.begin(), nr.end());
+            std::vector<int> rp_r(nriv), rpos(nriv), rp_ptr(nriv + 1, 0), rp_idx, rp_down(nriv, -1);
+            for (int64_t k = 0; k < nriv; ++k) {
+                rp_r[k] = nr[k].second;
+                rpos[nr[k].second] = (int)k;
+            }
+            std::vector<std::vector<int>> ups(nriv);
+            for (int64_t r = 0; r < nriv; ++r)
+                if (h->river_down[r] >= 0) ups[h->river_down[r]].push_back((int)r);  // ascending river index
+            for (int64_t k = 0; k < nriv; ++k) {
+                const int r = rp_r[k];
+                for (int u : ups[r]) {
+                    if (rpos[u] >= k) {
+                        rc = fail(h, WFH_ERR_CYCLE, "internal error: river positions are not topological");
+                        goto done;
+                    }
+                    rp_idx.push_back(rpos[u]);
+                }
+                rp_ptr[k + 1] = (int)rp_idx.size();
+                rp_down[k] = h->river_down[r] >= 0 ? rpos[h->river_down[r]] : -1;
+            }
+            if ((rc = put_raw<int>(h, rp_r, X.rp_r))) goto done;
+            if ((rc = put_raw<int>(h, rp_ptr, X.rp_up_ptr))) goto done;
+            if ((rc = put_raw<int>(h, rp_idx, X.rp_up_idx))) goto done;
+            if ((rc = put_raw<int>(h, rp_down, X.rp_down))) goto done;
+            X.nchunks_r = (int)((nriv + WAVE - 1) / WAVE);
+            hipError_t ea = hipSuccess, eb;
+            X.szi = dalloc<double>(h, n, eb); if (eb != hipSuccess) ea = eb;
+            X.scum = dalloc<double>(h, 6 * (size_t)n, eb); if (eb != hipSuccess) ea = eb;
+            X.ocum = dalloc<double>(h, 3 * (size_t)n, eb); if (eb != hipSuccess) ea = eb;
+            X.rcum = dalloc<double>(h, 2 * (size_t)std::max<int64_t>(nriv, 1), eb); if (eb != hipSuccess) ea = eb;
+            h->keys = dalloc<uint64_t>(h, (size_t)std::max<int64_t>(n, nriv), eb); if (eb != hipSuccess) ea = eb;
+            h->sel = dalloc<SelCtl>(h, 1, eb); if (eb != hipSuccess) ea = eb;
+            if (ea != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipMalloc adaptive sub-step state"); goto done; }
+            if (hipHostMalloc((void**)&h->sel_host, sizeof(SelCtl), hipHostMallocDefault) != hipSuccess) {
+                h->sel_host = nullptr;
+                rc = fail(h, WFH_ERR_HIP, "hipHostMalloc");
+                goto done;
+            }
+        }
         if (hipStreamSynchronize(h->stream) != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "init sync"); goto done; }
     }
     D.P = h->fP;
@@ -1782,6 +2310,128 @@ static void launch_kernels(wfh_handle* h) {
     h->parity ^= 1;
     hipLaunchKernelGGL(k_route<N>, dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, a);
     if (ev) hipEventRecord(ev[2], h->stream);
+}
+
+// ---- adaptive sub-steps ----
+constexpr int MAX_ADAPTIVE_SUBSTEPS = 1000000;  // per family and step: a guard, the reference has none
+
+static int32_t read_sel(wfh_handle* h) {
+    HIPCHK(h, hipMemcpyAsync(h->sel_host, h->sel, sizeof(SelCtl), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    int flag = 0;
+    HIPCHK(h, hipMemcpy(&flag, h->ra.abort_flag, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) {
+        h->broken = true;
+        return fail(h, WFH_ERR_DEVICE, "routing kernel aborted: a dependency wait exceeded its spin limit");
+    }
+    return WFH_OK;
+}
+
+// stable_timestep of the subsurface model  lateral_subsurface_flow.jl:314-344
+static int32_t ssf_stable_timestep(wfh_handle* h, double& dts) {
+    hipLaunchKernelGGL(k_sel_reset, dim3(1), dim3(256), 0, h->stream, h->sel);
+    hipLaunchKernelGGL(k_ssf_min, dim3((unsigned)((h->n + 255) / 256)), dim3(256), 0, h->stream, h->d, h->ad, h->sel);
+    HIPCHK(h, hipGetLastError());
+    int32_t rc = read_sel(h);
+    if (rc) return rc;
+    const double dt_min = h->sel_host->count > 0 ? wfh_from_bits(h->sel_host->minkey) : 0.5;
+    dts = dt_min * h->cfg.subsurface_alpha_coefficient;
+    return WFH_OK;
+}
+
+// stable_timestep of a kinematic-wave model over m cells  surface_kinwave.jl:674-704: the
+// quantile's two order statistics come from an 8-pass radix selection on the device.
+static int32_t kw_stable_timestep(wfh_handle* h, const double* q, Prm alpha, Prm flen, int m, double p, double& dts) {
+    if (m == 0) {
+        dts = 600.0;
+        return WFH_OK;
+    }
+    const unsigned g1 = (unsigned)((m + 255) / 256);
+    const unsigned gs = std::min<unsigned>(g1, 1024u);
+    hipLaunchKernelGGL(k_sel_reset, dim3(1), dim3(256), 0, h->stream, h->sel);
+    hipLaunchKernelGGL(k_kw_keys, dim3(g1), dim3(256), 0, h->stream, q, alpha, flen, m, h->keys, h->sel);
+    hipLaunchKernelGGL(k_sel_init, dim3(1), dim3(1), 0, h->stream, h->sel, p);
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        hipLaunchKernelGGL(k_sel_hist, dim3(gs), dim3(256), 0, h->stream, (const uint64_t*)h->keys, m, h->sel, shift);
+        hipLaunchKernelGGL(k_sel_pick, dim3(1), dim3(256), 0, h->stream, h->sel, shift);
+    }
+    hipLaunchKernelGGL(k_sel_next, dim3(gs), dim3(256), 0, h->stream, (const uint64_t*)h->keys, m, h->sel);
+    HIPCHK(h, hipGetLastError());
+    int32_t rc = read_sel(h);
+    if (rc) return rc;
+    const SelCtl& c = *h->sel_host;
+    if (c.count == 0) {
+        dts = 600.0;
+    } else if (c.count == 1) {
+        dts = wfh_from_bits(c.prefix);
+    } else {
+        double g;
+        wfh_quantile_index((int64_t)c.count, p, &g);
+        const double a = wfh_from_bits(c.prefix);
+        const double b = c.rank + 1 < c.bin_count ? a : wfh_from_bits(c.next);
+        dts = wfh_quantile_interp(a, b, g);
+    }
+    return WFH_OK;
+}
+
+// check_timestepsize  routing/timestepping.jl:11-17
+static double check_timestepsize(double ts, double t, double end) { return t + ts > end ? end - t : ts; }
+
+template <int N>
+static int32_t launch_adaptive(wfh_handle* h) {
+    const int n = (int)h->n, nriv = (int)h->nriv;
+    const double dt = h->cfg.dt;
+    hipEvent_t* ev = timing_events(h);
+    if (ev) hipEventRecord(ev[0], h->stream);
+    hipLaunchKernelGGL(k_vertical<N>, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d);
+    if (ev) hipEventRecord(ev[1], h->stream);
+    const int nch_r = h->ad.nchunks_r;
+    const int grid_r = std::max(1, std::min(h->route_grid, (nch_r + (ROUTE_BLOCK / WAVE) - 1) / (ROUTE_BLOCK / WAVE)));
+    auto args = [&]() {
+        RouteArgs a = h->ra;
+        a.parity = h->parity;
+        h->parity ^= 1;
+        return a;
+    };
+    int32_t rc;
+    // subsurface: the water table handed over by the soil (sbm_model.jl:74-81)
+    HIPCHK(h, hipMemcpyAsync(h->ad.szi, h->d.zi, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    for (int fam = 0; fam < 3; ++fam) {
+        double t = 0.0;
+        int k = 0;
+        while (t < dt) {
+            double dts;
+            if (fam == 0)
+                rc = ssf_stable_timestep(h, dts);
+            else if (fam == 1)
+                rc = kw_stable_timestep(h, h->d.ol_q, h->d.alpha, h->d.fl, n, 0.02, dts);
+            else
+                rc = kw_stable_timestep(h, h->d.r_q, Prm{h->d.r_alpha, 0xFFFFFFFFu}, Prm{h->d.r_fl, 0xFFFFFFFFu}, nriv,
+                                        0.05, dts);
+            if (rc) return rc;
+            dts = check_timestepsize(dts, t, dt);
+            if (!(dts > 0.0) || ++k > MAX_ADAPTIVE_SUBSTEPS)
+                return fail(h, WFH_ERR_STATE, "adaptive sub-step length not positive or too many sub-steps");
+            const double tn = t + dts;
+            const int first = k == 1, last = !(tn < dt);
+            if (fam == 0)
+                hipLaunchKernelGGL(k_ad_ssf<N>, dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(), h->ad,
+                                   dts, first, last);
+            else if (fam == 1)
+                hipLaunchKernelGGL(k_ad_ol, dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(), h->ad,
+                                   dts, first, last);
+            else if (nriv > 0)
+                hipLaunchKernelGGL(k_ad_riv, dim3(grid_r), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(), h->ad, dts,
+                                   first, last);
+            HIPCHK(h, hipGetLastError());
+            t = tn;
+        }
+        h->nsub[fam] = k;
+    }
+    hipLaunchKernelGGL(k_ad_total, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d);
+    if (ev) hipEventRecord(ev[2], h->stream);
+    HIPCHK(h, hipGetLastError());
+    return read_sel(h);
 }
 
 extern "C" int32_t wfh_set_forcing(wfh_handle* h, const double* P, const double* E, const double* T,
@@ -1830,6 +2480,15 @@ extern "C" int32_t wfh_step(wfh_handle* h) {
     if (h->broken) return fail(h, WFH_ERR_STATE, "handle is in an error state (a previous step failed)");
     if (!h->forcing_set) return fail(h, WFH_ERR_STATE, "wfh_set_forcing must be called before wfh_step");
     HIPCHK(h, hipSetDevice(h->cfg.device));
+    if (h->cfg.kinematic_wave_adaptive) {
+        switch (h->N) {
+            case 2: return launch_adaptive<2>(h);
+            case 3: return launch_adaptive<3>(h);
+            case 4: return launch_adaptive<4>(h);
+            case 5: return launch_adaptive<5>(h);
+            default: return launch_adaptive<6>(h);
+        }
+    }
     switch (h->N) {
         case 2: launch_kernels<2>(h); break;
         case 3: launch_kernels<3>(h); break;
@@ -1838,6 +2497,17 @@ extern "C" int32_t wfh_step(wfh_handle* h) {
         default: launch_kernels<6>(h); break;
     }
     HIPCHK(h, hipGetLastError());
+    h->nsub[0] = h->S_s;
+    h->nsub[1] = h->S_l;
+    h->nsub[2] = h->S_r;
+    return WFH_OK;
+}
+
+extern "C" int32_t wfh_last_substeps(wfh_handle* h, int32_t* subsurface, int32_t* land, int32_t* river) {
+    if (!h) return WFH_ERR_ARG;
+    if (subsurface) *subsurface = h->nsub[0];
+    if (land) *land = h->nsub[1];
+    if (river) *river = h->nsub[2];
     return WFH_OK;
 }
 
@@ -2104,6 +2774,7 @@ extern "C" void wfh_destroy(wfh_handle* h) {
     if (!h) return;
     if (h->stream) hipStreamSynchronize(h->stream);
     for (void* p : h->allocs) hipFree(p);
+    if (h->sel_host) hipHostFree(h->sel_host);
     for (hipEvent_t ev : h->ev)
         if (ev) hipEventDestroy(ev);
     if (h->stream) hipStreamDestroy(h->stream);

@@ -7,7 +7,7 @@ test-suite checks the sizes against the compiled library's view of them.
 import ctypes
 import enum
 
-WFH_ABI_VERSION = 1
+WFH_ABI_VERSION = 2
 WFH_MAX_LAYERS = 8
 
 WFH_OK = 0
@@ -40,6 +40,8 @@ class WfhConfig(ctypes.Structure):
         ("soil_layer_thickness_mm", ctypes.c_double * WFH_MAX_LAYERS),
         ("soil_infiltration_reduction", ctypes.c_int32),
         ("water_mass_balance", ctypes.c_int32),
+        ("kinematic_wave_adaptive", ctypes.c_int32),
+        ("subsurface_alpha_coefficient", ctypes.c_double),
     ]
 
 
@@ -140,7 +142,7 @@ SETTABLE_VARS = {Var.CANOPY_STORAGE, Var.SNOW_STORAGE, Var.SNOW_WATER, Var.SOIL_
 EXPORTS = ["wfh_create", "wfh_set_forcing", "wfh_step", "wfh_get", "wfh_set", "wfh_get_topology",
            "wfh_state_size", "wfh_save_state", "wfh_load_state", "wfh_sync", "wfh_set_timing",
            "wfh_get_timing", "wfh_last_error", "wfh_destroy", "wfh_get_node_order",
-           "wfh_kernel_bytes"]
+           "wfh_kernel_bytes", "wfh_last_substeps"]
 
 
 def var_count(var, n, nriv, nlayers):

@@ -65,6 +65,8 @@ def load_library():
     L.wfh_destroy.restype = None
     L.wfh_kernel_bytes.argtypes = [P, P, P]
     L.wfh_kernel_bytes.restype = i32
+    L.wfh_last_substeps.argtypes = [P, P, P, P]
+    L.wfh_last_substeps.restype = i32
     _LIB = L
     return L
 
@@ -185,6 +187,12 @@ class Model:
         a, b = ctypes.c_int64(), ctypes.c_int64()
         self._check(self.lib.wfh_kernel_bytes(self.h, ctypes.addressof(a), ctypes.addressof(b)))
         return a.value, b.value
+
+    def last_substeps(self):
+        """(subsurface, overland, river) sub-step counts of the last step."""
+        v = [ctypes.c_int32() for _ in range(3)]
+        self._check(self.lib.wfh_last_substeps(self.h, *(ctypes.addressof(x) for x in v)))
+        return tuple(x.value for x in v)
 
     def timing(self):
         a, b = ctypes.c_double(), ctypes.c_double()

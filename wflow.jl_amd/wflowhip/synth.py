@@ -97,7 +97,8 @@ class Catchment:
     def nlayers(self):
         return self.nlayers_cfg + 1
 
-    def config(self, water_mass_balance=True, device=0, land_dt=3600.0, river_dt=900.0, subsurface_dt=86400.0):
+    def config(self, water_mass_balance=True, device=0, land_dt=3600.0, river_dt=900.0, subsurface_dt=86400.0,
+               adaptive=False, subsurface_alpha=1.0):
         c = _abi.WfhConfig()
         c.abi_version = _abi.WFH_ABI_VERSION
         c.device = device
@@ -111,6 +112,8 @@ class Catchment:
             c.soil_layer_thickness_mm[k] = v
         c.soil_infiltration_reduction = int(self.infiltration_reduction)
         c.water_mass_balance = int(water_mass_balance)
+        c.kinematic_wave_adaptive = int(adaptive)
+        c.subsurface_alpha_coefficient = subsurface_alpha
         return c
 
     def static(self):
