@@ -40,7 +40,12 @@ def parse():
     ap.add_argument("--cpu-sample-cells", type=int, default=400_000)
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--adaptive", action="store_true",
+                    help="adaptive kinematic-wave sub-steps (kinematic_wave__adaptive_time_step_flag) instead of "
+                         "the fixed ones; not the headline line: no roofline, every sub-step waits for the last")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_routing_4096.json"))
+    ap.add_argument("--fp64-file", default=os.path.join(ROOT, "profiles", "fp64_4096.json"),
+                    help="FP64 VALU FLOPs per launch from a counter run (SQ_INSTS_VALU_*_F64) of the same workload")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="after the timed steps, run one elementwise kernel over this many MiB (reads and "
                          "writes each byte once: known traffic, for scaling hardware byte counters)")
@@ -70,7 +75,7 @@ def cpu_baseline(catchment, frames_host, args):
             break
     mask = np.isin(lab, np.array(take))
     sub = catchment.subset(mask)
-    m = O.OracleModel(sub)
+    m = O.OracleModel(sub, adaptive=args.adaptive)
     t = 0.0
     for k in range(args.cpu_sample_steps):
         P, E, T = (f[mask] for f in frames_host[k % len(frames_host)])
@@ -84,7 +89,8 @@ def cpu_baseline(catchment, frames_host, args):
         "cores": 1,
         "kind": "port",
         "sample": f"{len(take)} complete basins ({sub.n} cells) of the same workload, "
-                  f"{args.cpu_sample_steps} steps, single-threaded C oracle (-O2), update_model! only",
+                  f"{args.cpu_sample_steps} steps, single-threaded C oracle (-O2), update_model! only"
+                  + (" (adaptive sub-steps of the sample itself)" if args.adaptive else ""),
     }
 
 
@@ -116,7 +122,9 @@ def main():
                     "100 cells, sbm + kinematic wave (land 1, river 4, subsurface 1 sub-steps)")
     print(f"# {workload}: n={c.n} nriv={c.nriv} longest path={c.max_depth} "
           f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
-    m = Model(c, water_mass_balance=True)
+    if args.adaptive:
+        workload = workload.replace("(land 1, river 4, subsurface 1 sub-steps)", "(adaptive sub-steps)")
+    m = Model(c, water_mass_balance=True, adaptive=args.adaptive)
     print(f"# model created ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     node = m.node_order()
     node_t = torch.as_tensor(node, device="cuda")
@@ -136,8 +144,11 @@ def main():
     torch.cuda.synchronize()
     m.set_timing(True)
     t1 = time.perf_counter()
+    subs = []
     for k in range(args.warmup, args.warmup + args.steps):
         step(k)
+        if args.adaptive:
+            subs.append(m.last_substeps())
     m.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t1
@@ -160,9 +171,12 @@ def main():
                 traffic = tf.get("bytes_per_launch")
         except Exception:
             traffic = None
+    metric = ("cell-timesteps/s FP64 SBM+kinwave (plain GPU step)" if args.workload == "grid" else
+              "cell-timesteps/s FP64 SBM+kinwave (batched GPU step)")
+    if args.adaptive:
+        metric = metric.replace(" step)", " step, adaptive sub-steps)")
     result = {
-        "metric": "cell-timesteps/s FP64 SBM+kinwave (plain GPU step)" if args.workload == "grid" else
-                  "cell-timesteps/s FP64 SBM+kinwave (batched GPU step)",
+        "metric": metric,
         "value": c.n * args.steps / elapsed,
         "unit": "cell-timesteps/s",
         "n_gpus": 1,
@@ -181,6 +195,25 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
     }
+    # FP64 VALU rate: FLOPs per launch from the counter run of the same workload over this run's
+    # live kernel times
+    if os.path.exists(args.fp64_file) and not args.adaptive:
+        try:
+            ff = json.load(open(args.fp64_file))
+            if ff.get("n") == c.n:
+                result["fp64_valu"] = {
+                    k: {"achieved": ff[k]["flops_per_launch"] / (ms * 1e-3) / 1e12, "peak": ff["peak_tflops"],
+                        "unit": "TFLOP/s", "frac": ff[k]["flops_per_launch"] / (ms * 1e-3) / 1e12 / ff["peak_tflops"]}
+                    for k, ms in (("k_route", k2_ms), ("k_vertical", k1_ms))}
+        except Exception:
+            pass
+    if args.adaptive:
+        # the routing time covers all sub-step launches and the host read-backs between them
+        result["roofline"] = None
+        result["config"]["kernel_ms"] = {"k_vertical": k1_ms, "routing": k2_ms}
+        del result["config"]["kernel_bytes"]
+        result["config"]["substeps_mean"] = [float(np.mean([s[i] for s in subs])) for i in range(3)]
+        result["config"]["substeps_max"] = [int(max(s[i] for s in subs)) for i in range(3)]
     if args.calibration_copy_mib > 0:
         nb = args.calibration_copy_mib << 20
         src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")

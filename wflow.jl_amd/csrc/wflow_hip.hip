@@ -1307,6 +1307,16 @@ __global__ void k_ad_total(Dev d) {
                          ld(d.rf, p));
 }
 
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        const unsigned lo = __shfl_xor((unsigned)v, o), hi = __shfl_xor((unsigned)(v >> 32), o);
+        const unsigned long long w = ((unsigned long long)hi << 32) | lo;
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
 // ---- stable time steps ----
 __global__ void k_sel_reset(SelCtl* s) {
     s->hist[threadIdx.x] = 0u;
@@ -1395,43 +1405,36 @@ __global__ void k_sel_pick(SelCtl* s, int shift) {
 // Smallest key above the selected one, needed when the selected key's multiplicity ends at it.
 __global__ void __launch_bounds__(256) k_sel_next(const uint64_t* __restrict__ keys, int m, SelCtl* s) {
     if (s->rank + 1 < s->bin_count) return;
-    __shared__ unsigned long long mn;
-    if (threadIdx.x == 0) mn = ~0ull;
-    __syncthreads();
     const unsigned long long a = s->prefix;
     unsigned long long best = ~0ull;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) {
         const uint64_t key = keys[i];
         if (key > a && key < best) best = key;
     }
-    atomicMin(&mn, best);
-    __syncthreads();
-    if (threadIdx.x == 0 && mn != ~0ull) atomicMin(&s->next, mn);
+    best = wave_min_u64(best);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && best != ~0ull) atomicMin(&s->next, best);
 }
 
 // Subsurface Courant time steps: minimum of flow_length / ssf_celerity over the cells with a
-// positive water table depth.  lateral_subsurface_flow.jl:314-344
+// positive water table depth (reduced per wave, then one atomic per wave).
+// lateral_subsurface_flow.jl:314-344
 __global__ void __launch_bounds__(256) k_ssf_min(Dev d, AdArgs x, SelCtl* s) {
-    __shared__ unsigned long long mn;
-    __shared__ unsigned int cnt;
-    if (threadIdx.x == 0) {
-        mn = ~0ull;
-        cnt = 0u;
-    }
-    __syncthreads();
     const int i = blockIdx.x * 256 + threadIdx.x;
+    unsigned long long key = ~0ull;
+    bool elig = false;
     if (i < d.n) {
         const double z = x.szi[i];
         if (z > 0.0) {
             const double cel = (ld(d.kh0, i) * wfh_exp(-ld(d.f, i) * z) * ld(d.slope, i)) / ld(d.sy, i);
-            atomicMin(&mn, (unsigned long long)wfh_bits(ld(d.fl, i) / cel));
-            atomicAdd(&cnt, 1u);
+            key = wfh_bits(ld(d.fl, i) / cel);
+            elig = true;
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0 && cnt) {
-        atomicMin(&s->minkey, mn);
-        atomicAdd(&s->count, (unsigned long long)cnt);
+    const unsigned long long b = __ballot(elig);
+    key = wave_min_u64(key);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && b) {
+        atomicMin(&s->minkey, key);
+        atomicAdd(&s->count, (unsigned long long)__popcll(b));
     }
 }
 
