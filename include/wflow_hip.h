@@ -129,6 +129,26 @@ typedef struct wfh_static {
     const double* river_bankfull_depth;
     const double* river_length;
     const double* river_width;
+    /* reservoirs (reservoir__flag; Wflow/src/routing/surface/reservoir.jl, network.jl:296-360),
+     * nres = 0 for none. Linear storage curve; outflow types 3 (modified Puls) and 4 (simple),
+     * or the observed outflow where given. */
+    int64_t nres;
+    const int64_t* reservoir_outlet;       /* [nres] land index of each outlet, ascending; a river
+                                              cell whose river flows on to a river cell   */
+    const int32_t* reservoir_coverage;     /* [n] 1-based reservoir whose area holds the cell
+                                              (reservoir_area__count), 0 = none; NULL = outlets */
+    const int32_t* reservoir_outflow_type; /* [nres] ReservoirOutflowType: 3 or 4            */
+    const double* reservoir_area;          /* [nres] [m2]                                      */
+    const double* reservoir_waterlevel;    /* [nres] initial water level [m]                   */
+    const double* reservoir_demand;        /* [nres] simple: [m3 s-1]                          */
+    const double* reservoir_maximum_release;         /* [nres] simple: [m3 s-1]               */
+    const double* reservoir_maximum_storage;         /* [nres] simple (and observed): [m3]    */
+    const double* reservoir_target_full_fraction;    /* [nres] simple [-]                     */
+    const double* reservoir_target_minimum_fraction; /* [nres] simple [-]                     */
+    const double* reservoir_threshold;               /* [nres] modified Puls [m]              */
+    const double* reservoir_rating_curve_coefficient;/* [nres] modified Puls [-]              */
+    const double* reservoir_external_inflow; /* [nres] or NULL (0) [m3 s-1]; negative = abstraction */
+    const double* reservoir_outflow_obs;     /* [nres] or NULL (NaN = not observed) [m3 s-1]       */
 } wfh_static;
 
 /* Readable / writable variables (BMI get_value / set_value equivalents, bmi.jl:203-262). */
@@ -199,6 +219,17 @@ typedef enum wfh_var {
     WFH_THETA_FC,                /* [n]    */
     WFH_SURFACE_FLOW_WIDTH,      /* [n]    */
     WFH_FLOW_FRACTION_TO_RIVER,  /* [n]    */
+    /* reservoirs [nres] (reservoir.jl ReservoirVariables / ReservoirBC) */
+    WFH_RESERVOIR_STORAGE,
+    WFH_RESERVOIR_WATERLEVEL,
+    WFH_RESERVOIR_OUTFLOW,
+    WFH_RESERVOIR_OUTFLOW_AVERAGE,
+    WFH_RESERVOIR_INFLOW_AVERAGE,
+    WFH_RESERVOIR_ACTEVAP_AVERAGE,       /* [m s-1] */
+    WFH_RESERVOIR_ABSTRACTION_AVERAGE,   /* actual abstraction from negative external inflow */
+    WFH_RESERVOIR_PRECIPITATION,         /* mean over the reservoir area [m s-1] */
+    WFH_RESERVOIR_EVAPORATION,           /* mean potential evaporation [m s-1] */
+    WFH_BALANCE_RESERVOIR_ERROR,
     WFH_VAR_COUNT
 } wfh_var;
 

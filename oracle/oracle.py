@@ -106,6 +106,9 @@ def lib():
         L.wfo_kw_stable_timestep.restype = _c
         L.wfo_kw_stable_timestep.argtypes = [_D, _D, _D, ctypes.c_int64, _c, _D]
         L.wfo_last_substeps.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
+        L.wfo_update_reservoir.argtypes = [ctypes.c_void_p, _c, _c]
+        L.wfo_update_reservoir_at_river.restype = _c
+        L.wfo_update_reservoir_at_river.argtypes = [ctypes.c_void_p, _c, _c]
         _lib = L
     return _lib
 
@@ -121,6 +124,38 @@ def math(op, x, y=None):
     rc = lib().wfo_math(MATH_OPS[op], x.ctypes.data_as(_D), y.ctypes.data_as(_D), out.ctypes.data_as(_D), x.size)
     assert rc == 0
     return out
+
+
+class Reservoir(ctypes.Structure):
+    """wfo_reservoir (oracle/wflow_oracle.h): one reservoir of the reference's ReservoirModel."""
+    _fields_ = [("outflow_type", ctypes.c_int32)] + [(f, ctypes.c_double) for f in (
+        "area", "demand", "maximum_release", "maximum_storage", "target_full_fraction", "target_minimum_fraction",
+        "threshold", "rating_curve_coefficient", "outflow_obs", "external_inflow", "precipitation", "evaporation",
+        "inflow_overland", "inflow_subsurface", "storage", "waterlevel", "outflow", "inflow_cumulative",
+        "outflow_cumulative", "actevap_cumulative", "abstraction_cumulative")]
+
+    @classmethod
+    def make(cls, **kw):
+        r = cls()
+        for f, t in cls._fields_:
+            if t is ctypes.c_double:
+                setattr(r, f, float("nan"))
+        for f in ("external_inflow", "inflow_overland", "inflow_subsurface", "inflow_cumulative",
+                  "outflow_cumulative", "actevap_cumulative", "abstraction_cumulative"):
+            setattr(r, f, 0.0)
+        for k, v in kw.items():
+            setattr(r, k, float("nan") if v is None else v)
+        return r
+
+
+def update_reservoir(res, inflow, dt):
+    """update_reservoir_model!(reservoir_model, i, inflow, dt)  reservoir.jl:583-635."""
+    lib().wfo_update_reservoir(ctypes.byref(res), inflow, dt)
+
+
+def update_reservoir_at_river(res, q_v, dt):
+    """update_reservoir_model! at a river cell (surface_kinwave.jl:441-489); returns the outflow."""
+    return lib().wfo_update_reservoir_at_river(ctypes.byref(res), q_v, dt)
 
 
 def quantile(v, p):
@@ -256,6 +291,7 @@ class OracleModel:
             raise RuntimeError(err.value.decode())
         self.n = lib().wfo_n(self.h)
         self.nriv = lib().wfo_nriv(self.h)
+        self.nres = getattr(catchment, "nres", 0)
         self.nlayers = lib().wfo_nlayers(self.h)
 
     def close(self):
@@ -272,7 +308,7 @@ class OracleModel:
             raise RuntimeError(f"wfo_step failed {rc}")
 
     def get(self, var):
-        cnt = _abi.var_count(var, self.n, self.nriv, self.nlayers)
+        cnt = _abi.var_count(var, self.n, self.nriv, self.nlayers, self.nres)
         out = np.empty(cnt, np.float64)
         rc = lib().wfo_get(self.h, int(var), out.ctypes.data_as(_D), cnt)
         if rc != 0:

@@ -538,6 +538,93 @@ void wfo_compute_mass_balance_error(double tin, double tout, double rate, double
 /* ------------------------------------------------------------------------------------ */
 
 /* update_diagnostic_vars!  soil.jl:1400-1436 */
+/* ---- reservoirs (Wflow/src/routing/surface/reservoir.jl), linear storage curve ---- */
+
+/* update_reservoir_simple  reservoir.jl:389-421; inflow, precipitation, evaporation in m3 s-1 */
+static void reservoir_simple(const wfo_reservoir* r, double inflow, double precipitation, double evaporation,
+                             double dt, double* outflow, double* storage_out) {
+    double storage = r->storage + (inflow + precipitation - evaporation) * dt;
+    storage = wfh_max(storage, 0.0);
+    double fill_fraction = storage / r->maximum_storage;
+    double fac = wfo_scurve(fill_fraction, r->target_minimum_fraction, 1.0, 30.0);
+    double demand_release = wfh_min(fac * r->demand, storage / dt);
+    storage -= demand_release * dt;
+    double release_wanted = wfh_max(0.0, (storage - r->maximum_storage * r->target_full_fraction) / dt);
+    double overflow_q = wfh_max(0.0, (storage - r->maximum_storage) / dt);
+    double release_realized = wfh_min(release_wanted, overflow_q + r->maximum_release - demand_release);
+    storage -= release_realized * dt;
+    *outflow = release_realized + demand_release;
+    *storage_out = storage;
+}
+
+/* update_reservoir_modified_puls  reservoir.jl:427-456 */
+static void reservoir_modified_puls(const wfo_reservoir* r, double inflow, double precipitation, double evaporation,
+                                    double dt, double* outflow_out, double* storage_out) {
+    double res_factor = r->area / (dt * sqrt(r->rating_curve_coefficient));
+    double si_factor = r->storage / dt + precipitation - evaporation + inflow;
+    double si_factor_adj = si_factor - r->area * r->threshold / dt;
+    double outflow;
+    if (si_factor_adj > 0.0) {
+        double quadratic_sol_term = -res_factor + sqrt((res_factor * res_factor + 4.0 * si_factor_adj));
+        outflow = quadratic_sol_term > 0.0 ? 0.25 * (quadratic_sol_term * quadratic_sol_term) : 0.0;
+    } else {
+        outflow = 0.0;
+    }
+    outflow = wfh_min(outflow, si_factor);
+    *outflow_out = outflow;
+    *storage_out = (si_factor - outflow) * dt;
+}
+
+/* update_reservoir_outflow_obs  reservoir.jl:555-575 */
+static void reservoir_outflow_obs(const wfo_reservoir* r, double inflow, double precipitation, double evaporation,
+                                  double dt, double* outflow_out, double* storage_out) {
+    double storage_input = wfh_max(r->storage / dt + precipitation - evaporation + inflow, 0.0);
+    double outflow = wfh_min(r->outflow_obs, storage_input);
+    double storage = (storage_input - outflow) * dt;
+    if (!wfh_isnan(r->maximum_storage)) {
+        double overflow = wfh_max(0.0, (storage - r->maximum_storage) / dt);
+        storage -= overflow * dt;
+        outflow += overflow;
+    }
+    *outflow_out = outflow;
+    *storage_out = storage;
+}
+
+void wfo_update_reservoir(wfo_reservoir* r, double inflow, double dt) {
+    double precipitation = r->precipitation * r->area;
+    double available_storage = r->storage + (inflow + precipitation) * dt;
+    double potential_evaporation = r->evaporation * r->area;
+    double evaporation = wfh_min(available_storage / dt, potential_evaporation);
+    double outflow, storage;
+    if (!wfh_isnan(r->outflow_obs))
+        reservoir_outflow_obs(r, inflow, precipitation, evaporation, dt, &outflow, &storage);
+    else if (r->outflow_type == 3)
+        reservoir_modified_puls(r, inflow, precipitation, evaporation, dt, &outflow, &storage);
+    else
+        reservoir_simple(r, inflow, precipitation, evaporation, dt, &outflow, &storage);
+    double waterlevel = r->waterlevel + (storage - r->storage) / r->area;
+    r->storage = storage;
+    r->waterlevel = waterlevel;
+    r->outflow = outflow;
+    r->inflow_cumulative += inflow * dt;
+    r->outflow_cumulative += outflow * dt;
+    r->actevap_cumulative += evaporation / r->area * dt;
+}
+
+double wfo_update_reservoir_at_river(wfo_reservoir* r, double q_v, double dt) {
+    double inflow_ext = r->external_inflow, inflow;
+    if (inflow_ext < 0.0) {
+        double abstraction = wfh_min(-inflow_ext, (r->storage / dt) * 0.98);
+        r->abstraction_cumulative += abstraction * dt;
+        inflow = -abstraction;
+    } else {
+        inflow = inflow_ext;
+    }
+    double net_inflow = q_v + r->inflow_overland + r->inflow_subsurface + inflow;
+    wfo_update_reservoir(r, net_inflow, dt);
+    return r->outflow;
+}
+
 void wfo_soil_update_diagnostic_vars(wfo_soil* s) {
     s->ustoredepth = sum_first(s->ustore, s->nl);
     s->zi = wfh_max(0.0, s->soil_thickness - s->satwaterdepth / (s->theta_s - s->theta_r));
@@ -842,6 +929,14 @@ struct wfo_model {
     /* adaptive sub-steps: stable time step buffer, sub-step counts of the last step */
     double* stbuf;
     int32_t nsub[3];
+    /* upstream lists without reservoir outlets (filter_upstream_nodes, utils.jl:61-71) */
+    int64_t *fup_ptr, *fup_idx, *rfup_ptr, *rfup_idx;
+    /* reservoirs */
+    int64_t nres;
+    wfo_reservoir* res;
+    int64_t *res_land, *res_river, *land_res, *river_res, *cov_ptr, *cov_idx;
+    uint8_t* land_cov;
+    double *res_store_prev, *res_outflow_avg, *res_inflow_avg, *res_actevap_avg, *res_abs_avg, *bal_res;
 };
 
 static double* dalloc(int64_t n) { return (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
@@ -1047,6 +1142,109 @@ wfo_model* wfo_create(const wfh_config* cfg, const wfh_static* st, char* err, in
             m->r_area[r] = m->area[m->river_land[r]];
         }
     }
+    /* ---- reservoirs (network.jl:296-360, reservoir.jl:47-300) and the upstream lists without
+     * reservoir outlets (domain.jl:95-121, filter_upstream_nodes utils.jl:61-71) ---- */
+    m->nres = st->nres;
+    m->land_res = ialloc(n);
+    m->river_res = ialloc(nriv);
+    for (int64_t i = 0; i < n; ++i) m->land_res[i] = -1;
+    for (int64_t r = 0; r < nriv; ++r) m->river_res[r] = -1;
+    m->land_cov = (uint8_t*)calloc((size_t)n, 1);
+    if (m->nres < 0) {
+        snprintf(err, (size_t)errlen, "nres < 0");
+        goto fail;
+    }
+    if (m->nres > 0) {
+        const int64_t nr = m->nres;
+        if (!st->reservoir_outlet || !st->reservoir_outflow_type || !st->reservoir_area || !st->reservoir_waterlevel) {
+            snprintf(err, (size_t)errlen, "reservoir_outlet / outflow_type / area / waterlevel missing");
+            goto fail;
+        }
+        m->res = (wfo_reservoir*)calloc((size_t)nr, sizeof(wfo_reservoir));
+        m->res_land = ialloc(nr);
+        m->res_river = ialloc(nr);
+        int64_t* taken = ialloc(nriv > 0 ? nriv : 1); /* downstream river cells fed by a reservoir */
+        for (int64_t k = 0; k < nr; ++k) {
+            const int64_t i = st->reservoir_outlet[k];
+            if (i < 0 || i >= n || (k > 0 && i <= st->reservoir_outlet[k - 1]) || m->land_river[i] < 0 ||
+                m->rdown[m->land_river[i]] < 0 || taken[m->rdown[m->land_river[i]]]) {
+                snprintf(err, (size_t)errlen, "reservoir %lld: outlet must be a river cell with a river cell "
+                         "downstream, outlets ascending, one reservoir per downstream cell", (long long)k);
+                free(taken);
+                goto fail;
+            }
+            taken[m->rdown[m->land_river[i]]] = 1;
+            m->res_land[k] = i;
+            m->res_river[k] = m->land_river[i];
+            m->land_res[i] = k;
+            m->river_res[m->land_river[i]] = k;
+            const int32_t t = st->reservoir_outflow_type[k];
+            if (t != 3 && t != 4) {
+                snprintf(err, (size_t)errlen, "reservoir %lld: outflow type %d not supported", (long long)k, t);
+                free(taken);
+                goto fail;
+            }
+            wfo_reservoir* R = &m->res[k];
+            R->outflow_type = t;
+            R->area = st->reservoir_area[k];
+#define RES_PRM(field, name, dflt) R->field = st->name ? st->name[k] : (dflt)
+            RES_PRM(demand, reservoir_demand, NaN);
+            RES_PRM(maximum_release, reservoir_maximum_release, NaN);
+            RES_PRM(maximum_storage, reservoir_maximum_storage, NaN);
+            RES_PRM(target_full_fraction, reservoir_target_full_fraction, NaN);
+            RES_PRM(target_minimum_fraction, reservoir_target_minimum_fraction, NaN);
+            RES_PRM(threshold, reservoir_threshold, NaN);
+            RES_PRM(rating_curve_coefficient, reservoir_rating_curve_coefficient, NaN);
+            RES_PRM(external_inflow, reservoir_external_inflow, 0.0);
+            RES_PRM(outflow_obs, reservoir_outflow_obs, NaN);
+#undef RES_PRM
+            R->waterlevel = st->reservoir_waterlevel[k];
+            R->storage = R->area * R->waterlevel; /* initialize_storage, linear  reservoir.jl:343-362 */
+            R->outflow = NaN;
+        }
+        free(taken);
+        /* coverage lists in ascending land index (findall order of reservoir_area__count) */
+        m->cov_ptr = ialloc(nr + 1);
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t id = st->reservoir_coverage ? st->reservoir_coverage[i] : m->land_res[i] + 1;
+            if (id < 0 || id > nr) {
+                snprintf(err, (size_t)errlen, "reservoir_coverage out of range");
+                goto fail;
+            }
+            if (id > 0) m->cov_ptr[id]++;
+        }
+        for (int64_t k = 0; k < nr; ++k) {
+            if (m->cov_ptr[k + 1] == 0) {
+                snprintf(err, (size_t)errlen, "reservoir %lld has an empty area", (long long)k);
+                goto fail;
+            }
+            m->cov_ptr[k + 1] += m->cov_ptr[k];
+        }
+        m->cov_idx = ialloc(m->cov_ptr[nr]);
+        int64_t* fill = ialloc(nr);
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t id = st->reservoir_coverage ? st->reservoir_coverage[i] : m->land_res[i] + 1;
+            if (id > 0) {
+                m->cov_idx[m->cov_ptr[id - 1] + fill[id - 1]++] = i;
+                m->land_cov[i] = 1;
+            }
+        }
+        free(fill);
+        m->res_store_prev = dalloc(nr); m->res_outflow_avg = dalloc(nr); m->res_inflow_avg = dalloc(nr);
+        m->res_actevap_avg = dalloc(nr); m->res_abs_avg = dalloc(nr); m->bal_res = dalloc(nr);
+    }
+    {
+        int64_t* df = ialloc(n);
+        for (int64_t i = 0; i < n; ++i) df[i] = m->land_res[i] >= 0 ? -1 : m->down[i];
+        m->fup_ptr = ialloc(n + 1);
+        build_upstream(n, df, m->fup_ptr, &m->fup_idx);
+        free(df);
+        int64_t* rf = ialloc(nriv > 0 ? nriv : 1);
+        for (int64_t r = 0; r < nriv; ++r) rf[r] = m->river_res[r] >= 0 ? -1 : m->rdown[r];
+        m->rfup_ptr = ialloc(nriv + 1);
+        build_upstream(nriv, rf, m->rfup_ptr, &m->rfup_idx);
+        free(rf);
+    }
     /* ---- vegetation, snow ---- */
     m->gap = dalloc(n); m->smax = dalloc(n); m->kc = dalloc(n); m->er = dalloc(n);
     m->canopy = dalloc(n); m->canopy_potevap = dalloc(n); m->interception_rate = dalloc(n);
@@ -1198,7 +1396,12 @@ void wfo_destroy(wfo_model* m) {
         (void**)&m->r_qin_cum, (void**)&m->r_q_avg, (void**)&m->r_qin_avg, (void**)&m->r_area,
         (void**)&m->precip, (void**)&m->pet, (void**)&m->temp, (void**)&m->total_storage,
         (void**)&m->land_store_prev, (void**)&m->ol_store_prev, (void**)&m->r_store_prev, (void**)&m->bal_land,
-        (void**)&m->bal_land_rel, (void**)&m->bal_ssf, (void**)&m->bal_ol, (void**)&m->bal_r, (void**)&m->stbuf};
+        (void**)&m->bal_land_rel, (void**)&m->bal_ssf, (void**)&m->bal_ol, (void**)&m->bal_r, (void**)&m->stbuf,
+        (void**)&m->fup_ptr, (void**)&m->fup_idx, (void**)&m->rfup_ptr, (void**)&m->rfup_idx, (void**)&m->res,
+        (void**)&m->res_land, (void**)&m->res_river, (void**)&m->land_res, (void**)&m->river_res,
+        (void**)&m->cov_ptr, (void**)&m->cov_idx, (void**)&m->land_cov, (void**)&m->res_store_prev,
+        (void**)&m->res_outflow_avg, (void**)&m->res_inflow_avg, (void**)&m->res_actevap_avg, (void**)&m->res_abs_avg,
+        (void**)&m->bal_res};
     for (size_t k = 0; k < sizeof p / sizeof p[0]; ++k) free(*p[k]);
     free(m);
 }
@@ -1269,12 +1472,31 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
     memcpy(m->precip, precip, sizeof(double) * (size_t)n);
     memcpy(m->pet, pet, sizeof(double) * (size_t)n);
     memcpy(m->temp, temp, sizeof(double) * (size_t)n);
-    if (m->cfg.water_mass_balance) { /* storage_prev!  mass_balance.jl:170-183 */
+    /* precipitation and potential evaporation over each reservoir area: their mean goes to the
+     * reservoir and the land model gets zero there (update_forcing!  io.jl:125-143; the mean is a
+     * left fold over the area's cells in ascending index, then / count) */
+    for (int64_t k = 0; k < m->nres; ++k) {
+        double sp = 0.0, se = 0.0;
+        for (int64_t e = m->cov_ptr[k]; e < m->cov_ptr[k + 1]; ++e) {
+            sp += m->precip[m->cov_idx[e]];
+            se += m->pet[m->cov_idx[e]];
+        }
+        const double cnt = (double)(m->cov_ptr[k + 1] - m->cov_ptr[k]);
+        m->res[k].precipitation = sp / cnt;
+        m->res[k].evaporation = se / cnt;
+    }
+    for (int64_t i = 0; i < n && m->nres > 0; ++i)
+        if (m->land_cov[i]) {
+            m->precip[i] = 0.0;
+            m->pet[i] = 0.0;
+        }
+    if (m->cfg.water_mass_balance) { /* storage_prev!  mass_balance.jl:156-183 */
         for (int64_t i = 0; i < n; ++i) {
             m->land_store_prev[i] = land_total_storage(m, i);
             m->ol_store_prev[i] = m->ol_storage[i];
         }
         for (int64_t r = 0; r < nriv; ++r) m->r_store_prev[r] = m->r_storage[r];
+        for (int64_t k = 0; k < m->nres; ++k) m->res_store_prev[k] = m->res[k].storage;
     }
     /* ---- update_land_hydrology_model!  sbm.jl:82-132 ---- */
     for (int64_t i = 0; i < n; ++i) { /* interception  canopy.jl:54-98 / 115-143 */
@@ -1353,12 +1575,12 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
             for (int64_t k = 0; k < n; ++k) { /* kinwave_subsurface_update!  :198-273 */
                 int64_t v = m->order[k];
                 double qin = 0.0, tor = 0.0;
-                for (int64_t e = m->up_ptr[v]; e < m->up_ptr[v + 1]; ++e) {
-                    int64_t u = m->up_idx[e];
+                for (int64_t e = m->fup_ptr[v]; e < m->fup_ptr[v + 1]; ++e) {
+                    int64_t u = m->fup_idx[e];
                     qin = qin + m->ssf_q[u] * (1.0 - m->fftr[u]);
                 }
-                for (int64_t e = m->up_ptr[v]; e < m->up_ptr[v + 1]; ++e) {
-                    int64_t u = m->up_idx[e];
+                for (int64_t e = m->fup_ptr[v]; e < m->fup_ptr[v + 1]; ++e) {
+                    int64_t u = m->fup_idx[e];
                     tor = tor + m->ssf_q[u] * m->fftr[u];
                 }
                 m->ssf_qin[v] = qin;
@@ -1410,15 +1632,15 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
             for (int64_t k = 0; k < n; ++k) { /* kinwave_land_update!  :293-341 */
                 int64_t v = m->order[k];
                 double tor = 0.0;
-                for (int64_t e = m->up_ptr[v]; e < m->up_ptr[v + 1]; ++e) {
-                    int64_t u = m->up_idx[e];
+                for (int64_t e = m->fup_ptr[v]; e < m->fup_ptr[v + 1]; ++e) {
+                    int64_t u = m->fup_idx[e];
                     tor = tor + m->ol_q[u] * m->fftr[u];
                 }
                 m->ol_tor_cum[v] += tor * dts;
                 if (m->sfw[v] > 0.0) {
                     double qin = 0.0;
-                    for (int64_t e = m->up_ptr[v]; e < m->up_ptr[v + 1]; ++e) {
-                        int64_t u = m->up_idx[e];
+                    for (int64_t e = m->fup_ptr[v]; e < m->fup_ptr[v + 1]; ++e) {
+                        int64_t u = m->fup_idx[e];
                         qin = qin + m->ol_q[u] * (1.0 - m->fftr[u]);
                     }
                     m->ol_qin[v] = qin;
@@ -1440,6 +1662,16 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
             m->ol_qin_avg[i] = m->ol_qin_cum[i] / dt;
         }
     }
+    for (int64_t k = 0; k < m->nres; ++k) {
+        /* update_inflow!  surface_kinwave.jl:772-788; set_reservoir_vars!  :227-237 */
+        wfo_reservoir* R = &m->res[k];
+        R->inflow_overland = m->ol_q_avg[m->res_land[k]];
+        R->inflow_subsurface = m->ssf_q_avg[m->res_land[k]];
+        R->inflow_cumulative = 0.0;
+        R->abstraction_cumulative = 0.0;
+        R->outflow_cumulative = 0.0;
+        R->actevap_cumulative = 0.0;
+    }
     for (int64_t r = 0; r < nriv; ++r) { /* update_lateral_inflow! (river)  :710-734 */
         int64_t i = m->river_land[r];
         m->r_inwater[r] = (m->ssf_tor_avg[i] + m->ol_tor_avg[i] + m->net_runoff_river[i] * m->area[i]) +
@@ -1460,7 +1692,7 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
             for (int64_t k = 0; k < nriv; ++k) { /* kinwave_river_update!  :492-566 */
                 int64_t v = m->rorder[k];
                 double s = 0.0;
-                for (int64_t e = m->rup_ptr[v]; e < m->rup_ptr[v + 1]; ++e) s = s + m->r_q[m->rup_idx[e]];
+                for (int64_t e = m->rfup_ptr[v]; e < m->rfup_ptr[v + 1]; ++e) s = s + m->r_q[m->rfup_idx[e]];
                 m->r_qin[v] += s;
                 double inflow = 0.0 / m->r_fl[v];
                 inflow -= 0.0 / m->r_fl[v];
@@ -1468,6 +1700,8 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
                 wfo_kinematic_wave_it(m->r_qin[v], m->r_q[v], m->r_qlat[v] + inflow, m->r_alpha[v], dts, m->r_fl[v],
                                       o, &m->kw_iters);
                 m->r_q[v] = o[0];
+                if (m->river_res[v] >= 0) /* the reservoir's outflow is the downstream inflow */
+                    m->r_qin[m->rdown[v]] = wfo_update_reservoir_at_river(&m->res[m->river_res[v]], m->r_q[v], dts);
                 m->r_h[v] = o[1] / m->r_fw[v];
                 m->r_storage[v] = m->r_fl[v] * o[1];
                 m->r_q_cum[v] += m->r_q[v] * dts;
@@ -1478,6 +1712,11 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
         for (int64_t r = 0; r < nriv; ++r) {
             m->r_q_avg[r] = m->r_q_cum[r] / dt;
             m->r_qin_avg[r] = m->r_qin_cum[r] / dt;
+        }
+        for (int64_t k = 0; k < m->nres; ++k) { /* average_reservoir_vars!  :244-258 */
+            m->res_outflow_avg[k] = m->res[k].outflow_cumulative / dt;
+            m->res_inflow_avg[k] = m->res[k].inflow_cumulative / dt;
+            m->res_abs_avg[k] = m->res[k].abstraction_cumulative / dt;
         }
     }
     /* ---- update_total_water_storage!  sbm.jl:143-182 ---- */
@@ -1520,6 +1759,16 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
             wfo_compute_mass_balance_error(m->r_inwater[r] + m->r_qin_avg[r] + wfh_max(0.0, 0.0),
                                            m->r_q_avg[r] + 0.0 + 0.0, (m->r_storage[r] - m->r_store_prev[r]) / dt, o);
             m->bal_r[r] = o[0];
+        }
+        for (int64_t k = 0; k < m->nres; ++k) { /* compute_flow_balance!(::ReservoirModel)  :261-283 */
+            const wfo_reservoir* R = &m->res[k];
+            double o[2];
+            m->res_inflow_avg[k] = R->inflow_cumulative / dt;
+            m->res_actevap_avg[k] = R->actevap_cumulative / dt;
+            double total_in = m->res_inflow_avg[k] + R->precipitation * R->area;
+            double total_out = m->res_outflow_avg[k] + m->res_actevap_avg[k] * R->area;
+            wfo_compute_mass_balance_error(total_in, total_out, (R->storage - m->res_store_prev[k]) / dt, o);
+            m->bal_res[k] = o[0];
         }
     }
     return 0;
@@ -1564,6 +1813,27 @@ int32_t wfo_get(const wfo_model* m, int32_t var, double* dst, int64_t count) {
     if (var == WFH_USTORELAYERDEPTH || var == WFH_VWC) need = (int64_t)N * n;
     if ((var >= WFH_RIVER_Q && var <= WFH_RIVER_INWATER) || var == WFH_BALANCE_RIVER_ERROR || var == WFH_RIVER_ALPHA)
         need = nr;
+    if (var >= WFH_RESERVOIR_STORAGE && var <= WFH_BALANCE_RESERVOIR_ERROR) {
+        if (count != m->nres) return WFH_ERR_ARG;
+        for (int64_t k = 0; k < m->nres; ++k) {
+            const wfo_reservoir* R = &m->res[k];
+            double v = 0.0;
+            switch (var) {
+            case WFH_RESERVOIR_STORAGE: v = R->storage; break;
+            case WFH_RESERVOIR_WATERLEVEL: v = R->waterlevel; break;
+            case WFH_RESERVOIR_OUTFLOW: v = R->outflow; break;
+            case WFH_RESERVOIR_OUTFLOW_AVERAGE: v = m->res_outflow_avg[k]; break;
+            case WFH_RESERVOIR_INFLOW_AVERAGE: v = m->res_inflow_avg[k]; break;
+            case WFH_RESERVOIR_ACTEVAP_AVERAGE: v = m->res_actevap_avg[k]; break;
+            case WFH_RESERVOIR_ABSTRACTION_AVERAGE: v = m->res_abs_avg[k]; break;
+            case WFH_RESERVOIR_PRECIPITATION: v = R->precipitation; break;
+            case WFH_RESERVOIR_EVAPORATION: v = R->evaporation; break;
+            default: v = m->bal_res[k]; break;
+            }
+            dst[k] = v;
+        }
+        return 0;
+    }
     if (count != need) return WFH_ERR_ARG;
     switch (var) {
     case WFH_CANOPY_STORAGE: ARR(m->canopy, n);
@@ -1660,6 +1930,14 @@ int32_t wfo_set(wfo_model* m, int32_t var, const double* src, int64_t count) {
     case WFH_LAND_H: if (count != n) return WFH_ERR_ARG; memcpy(m->ol_h, src, 8 * (size_t)n); return 0;
     case WFH_RIVER_Q: if (count != nr) return WFH_ERR_ARG; memcpy(m->r_q, src, 8 * (size_t)nr); return 0;
     case WFH_RIVER_H: if (count != nr) return WFH_ERR_ARG; memcpy(m->r_h, src, 8 * (size_t)nr); return 0;
+    case WFH_RESERVOIR_STORAGE:
+    case WFH_RESERVOIR_WATERLEVEL:
+        if (count != m->nres) return WFH_ERR_ARG;
+        for (int64_t k = 0; k < m->nres; ++k) {
+            if (var == WFH_RESERVOIR_STORAGE) m->res[k].storage = src[k];
+            else m->res[k].waterlevel = src[k];
+        }
+        return 0;
     default: return WFH_ERR_ARG;
     }
 }

@@ -42,6 +42,24 @@ typedef struct wfo_soil {
     double vwc[WFO_NMAX], rvwc[WFO_NMAX], root_zone_storage, vwc_root, rvwc_root;
 } wfo_soil;
 
+/* One reservoir (ReservoirParameters / ReservoirVariables / ReservoirBC of
+ * Wflow/src/routing/surface/reservoir.jl, linear storage curve). */
+typedef struct wfo_reservoir {
+    int32_t outflow_type; /* 3 modified Puls, 4 simple */
+    double area, demand, maximum_release, maximum_storage, target_full_fraction, target_minimum_fraction;
+    double threshold, rating_curve_coefficient, outflow_obs, external_inflow;
+    double precipitation, evaporation;           /* [m s-1] over the reservoir area */
+    double inflow_overland, inflow_subsurface;   /* [m3 s-1] */
+    double storage, waterlevel, outflow;
+    double inflow_cumulative, outflow_cumulative, actevap_cumulative, abstraction_cumulative;
+} wfo_reservoir;
+
+/* update_reservoir_model!(reservoir_model, i, inflow, dt)  reservoir.jl:583-635 */
+void wfo_update_reservoir(wfo_reservoir* r, double inflow, double dt);
+/* update_reservoir_model!(reservoir_model, river_flow_vars, network, v, dt)  surface_kinwave.jl:441-489
+ * for river discharge q_v at the outlet; returns the outflow that becomes the downstream qin. */
+double wfo_update_reservoir_at_river(wfo_reservoir* r, double q_v, double dt);
+
 typedef struct wfo_model wfo_model;
 
 #ifdef __cplusplus

@@ -81,7 +81,8 @@ struct Dev {
     const int* up_idx;   // [E] node ids of upstream cells
     const double* up_f;  // [E] flow_fraction_to_river of the upstream cell
     const int* river_of; // [n] river id or -1
-    const uint8_t* flags;  // bit0: has a land downstream, bit1: river cell, bit2: has a river downstream
+    const uint8_t* flags;  // bit0: has a land downstream, bit1: river cell, bit2: has a river downstream,
+                           // bit3: in a reservoir's area (no precipitation / evaporation on land)
     const int* rup_ptr;  // [nriv+1]
     const int* rup_idx;  // [Er] river ids
     const int* rup_node; // [Er] node ids of the upstream river cells
@@ -105,6 +106,17 @@ struct Dev {
     double cfg_thick[WFH_MAX_LAYERS], cfg_cum[WFH_MAX_LAYERS + 1];
     double dt;
     int gash, snow, infred, balance;
+    // ---- reservoirs (nres > 0) ----
+    int nres;
+    const int* res_of_r;       // [nriv] reservoir whose outlet is the river cell, or -1
+    const double* rup_res;     // [Er] 1 when the edge's upstream river cell is a reservoir outlet
+    const int *res_cov_ptr, *res_cov_idx;  // nodes of each reservoir's area, ascending land index
+    const int* res_type;       // [nres] 3 modified Puls, 4 simple
+    const double *res_area, *res_demand, *res_maxrel, *res_maxsto, *res_tff, *res_tmf, *res_thr, *res_rcc;
+    const double *res_ext, *res_obs;
+    double *res_storage, *res_wl, *res_outflow, *res_P, *res_E;
+    double* res_cum;           // [4][nres] sums of the step: inflow, outflow, actual evaporation, abstraction
+    double *res_outflow_avg, *res_inflow_avg, *res_actevap_avg, *res_abs_avg, *res_store_prev, *bal_res;
 };
 
 struct RouteArgs {
@@ -129,6 +141,8 @@ struct AdArgs {
     const int* rp_up_ptr;  // [nriv+1]
     const int* rp_up_idx;  // [Er] river positions of the upstream river cells, ascending river index
     const int* rp_down;    // [nriv] river position downstream, or -1
+    const double* rp_up_res;  // [Er] 1 when the edge's upstream river cell is a reservoir outlet
+    const int* rp_node;    // [nriv] node of the river cell at each river position
     int nchunks_r;
 };
 
@@ -163,7 +177,8 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
     // Every input of the cell is loaded here, in one block, unconditionally (parameters of
     // switched-off processes are allocated as zeros): loads spread over the branches below
     // would each be waited for at the branch's end, one memory round trip after another.
-    const double P = d.P[i], E = d.E[i], T = d.T[i];
+    const double P_in = d.P[i], E_in = d.E[i], T = d.T[i];
+    const uint8_t fl = d.nres > 0 ? d.flags[i] : 0;
     const double canopy0 = d.canopy[i], ss0 = d.snow_s[i], sw0 = d.snow_w[i], tsws0 = d.tsws[i];
     const double gap = ld(d.gap, i), smax = ld(d.smax, i), kc = ld(d.kc, i), er = ld(d.er, i);
     const double p_tti = ld(d.tti, i), p_tt = ld(d.tt, i), p_ttm = ld(d.ttm, i), p_cfmax = ld(d.cfmax, i);
@@ -179,6 +194,9 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
     const double rd = ld(d.rd, i), p_h3h = ld(d.h3h, i), p_h3l = ld(d.h3l, i);
     const double h1 = ld(d.h1, i), h2 = ld(d.h2, i), h4 = ld(d.h4, i), ah1 = ld(d.ah1, i), hb = ld(d.hb, i);
     const double p_rdp = ld(d.rdp, i), caph = ld(d.caph, i), capn = ld(d.capn, i), maxleak = ld(d.maxleak, i);
+    // precipitation and potential evaporation over a reservoir's area go to the reservoir
+    // (update_forcing!  io.jl:125-143)
+    const double P = (fl & 8) ? 0.0 : P_in, E = (fl & 8) ? 0.0 : E_in;
     double us[N], kvf[N], cbc[N], rfr[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
@@ -483,7 +501,9 @@ __device__ __forceinline__ bool slot_take(uint64_t* s, double& v) {
 //   FOLD_SSF: acc1 = sum q (1 - f)  (q_in),   acc2 = sum q f  (to_river)   lateral_subsurface_flow.jl:228-240
 //   FOLD_OL:  acc1 = sum q f (to_river),      acc2 = sum q (1 - f) (qin)    surface_kinwave.jl:317-326
 //   FOLD_RIV: acc1 = sum q                                                  surface_kinwave.jl:520
-enum { FOLD_SSF = 0, FOLD_OL = 1, FOLD_RIV = 2 };
+//   FOLD_RIV_RES (a model with reservoirs): acc1 = sum q over the non-reservoir upstream cells,
+//             acc2 = outflow of an upstream reservoir (f = 1 on that edge)  surface_kinwave.jl:478,520
+enum { FOLD_SSF = 0, FOLD_OL = 1, FOLD_RIV = 2, FOLD_RIV_RES = 3 };
 
 template <int FOLD>
 __device__ __forceinline__ void fold_in(double& acc1, double& acc2, double q, double f) {
@@ -493,6 +513,8 @@ __device__ __forceinline__ void fold_in(double& acc1, double& acc2, double q, do
     } else if (FOLD == FOLD_OL) {
         acc1 = acc1 + q * f;
         acc2 = acc2 + q * (1.0 - f);
+    } else if (FOLD == FOLD_RIV_RES && f != 0.0) {
+        acc2 = q;  // a reservoir's outflow is assigned to the downstream inflow (surface_kinwave.jl:478)
     } else {
         acc1 = acc1 + q;
     }
@@ -684,6 +706,116 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
     return true;
 }
 
+// update_reservoir_model! at the river cell that is reservoir ri's outlet (node p), for its
+// river discharge q_v of a sub-step of length dts: external inflow / abstraction
+// (surface_kinwave.jl:441-489), the outflow of the simple, modified Puls or observed-outflow
+// update and the linear storage curve (reservoir.jl:389-456, 555-635). The reservoir's state
+// lives in memory and only this lane touches it. first / last: the step's first / last river
+// sub-step (set_reservoir_vars!, average_reservoir_vars!, the reservoir balance). Returns the
+// outflow, which becomes the downstream river cell's inflow.
+__device__ __forceinline__ double reservoir_step(const Dev& d, const int ri, const int p, const double q_v,
+                                              const double dts, const bool first, const bool last) {
+    const int nr = d.nres;
+    const double area = d.res_area[ri];
+    double storage0 = d.res_storage[ri];
+    double cin = first ? 0.0 : d.res_cum[ri], cout = first ? 0.0 : d.res_cum[nr + ri];
+    double cev = first ? 0.0 : d.res_cum[2 * nr + ri], cab = first ? 0.0 : d.res_cum[3 * nr + ri];
+    if (first && d.balance) d.res_store_prev[ri] = storage0;
+    // update_inflow!  surface_kinwave.jl:772-799: the outlet cell's overland and subsurface flow
+    const double io = d.ol_q_avg[p], is = d.ssf_q_avg[p];
+    const double inflow_ext = d.res_ext[ri];
+    double inflow;
+    if (inflow_ext < 0.0) {
+        const double abstraction = wfh_min(-inflow_ext, (storage0 / dts) * 0.98);
+        cab += abstraction * dts;
+        inflow = -abstraction;
+    } else {
+        inflow = inflow_ext;
+    }
+    const double net_inflow = q_v + io + is + inflow;
+    const double precipitation = d.res_P[ri] * area;
+    const double available_storage = storage0 + (net_inflow + precipitation) * dts;
+    const double potential_evaporation = d.res_E[ri] * area;
+    const double evaporation = wfh_min(available_storage / dts, potential_evaporation);
+    const double obs = d.res_obs[ri], maxsto = d.res_maxsto[ri];
+    double outflow, storage;
+    if (!wfh_isnan(obs)) {  // update_reservoir_outflow_obs  reservoir.jl:555-575
+        const double storage_input = wfh_max(storage0 / dts + precipitation - evaporation + net_inflow, 0.0);
+        outflow = wfh_min(obs, storage_input);
+        storage = (storage_input - outflow) * dts;
+        if (!wfh_isnan(maxsto)) {
+            const double overflow = wfh_max(0.0, (storage - maxsto) / dts);
+            storage -= overflow * dts;
+            outflow += overflow;
+        }
+    } else if (d.res_type[ri] == 3) {  // update_reservoir_modified_puls  reservoir.jl:427-456
+        const double res_factor = area / (dts * sqrt(d.res_rcc[ri]));
+        const double si_factor = storage0 / dts + precipitation - evaporation + net_inflow;
+        const double si_factor_adj = si_factor - area * d.res_thr[ri] / dts;
+        outflow = 0.0;
+        if (si_factor_adj > 0.0) {
+            const double qst = -res_factor + sqrt((res_factor * res_factor + 4.0 * si_factor_adj));
+            if (qst > 0.0) outflow = 0.25 * (qst * qst);
+        }
+        outflow = wfh_min(outflow, si_factor);
+        storage = (si_factor - outflow) * dts;
+    } else {  // update_reservoir_simple  reservoir.jl:389-421
+        storage = storage0 + (net_inflow + precipitation - evaporation) * dts;
+        storage = wfh_max(storage, 0.0);
+        const double fill_fraction = storage / maxsto;
+        const double fac = 1.0 / (1.0 + wfh_exp(-30.0 * (fill_fraction - d.res_tmf[ri])));  // scurve
+        const double demand_release = wfh_min(fac * d.res_demand[ri], storage / dts);
+        storage -= demand_release * dts;
+        const double release_wanted = wfh_max(0.0, (storage - maxsto * d.res_tff[ri]) / dts);
+        const double overflow_q = wfh_max(0.0, (storage - maxsto) / dts);
+        const double release_realized = wfh_min(release_wanted, overflow_q + d.res_maxrel[ri] - demand_release);
+        storage -= release_realized * dts;
+        outflow = release_realized + demand_release;
+    }
+    d.res_wl[ri] = d.res_wl[ri] + (storage - storage0) / area;
+    d.res_storage[ri] = storage;
+    d.res_outflow[ri] = outflow;
+    cin += net_inflow * dts;
+    cout += outflow * dts;
+    cev += evaporation / area * dts;
+    d.res_cum[ri] = cin;
+    d.res_cum[nr + ri] = cout;
+    d.res_cum[2 * nr + ri] = cev;
+    d.res_cum[3 * nr + ri] = cab;
+    if (last) {
+        const double dt = d.dt;
+        // average_reservoir_vars!  surface_kinwave.jl:244-258
+        d.res_outflow_avg[ri] = cout / dt;
+        d.res_inflow_avg[ri] = cin / dt;
+        d.res_abs_avg[ri] = cab / dt;
+        if (d.balance) {  // compute_flow_balance!(::ReservoirModel)  mass_balance.jl:261-283
+            const double ev_avg = cev / dt;
+            d.res_actevap_avg[ri] = ev_avg;
+            const double total_in = cin / dt + d.res_P[ri] * area;
+            const double total_out = cout / dt + ev_avg * area;
+            d.bal_res[ri] = total_in - total_out - (storage - d.res_store_prev[ri]) / dt;
+        }
+    }
+    return outflow;
+}
+
+// Mean precipitation and potential evaporation over each reservoir's area (update_forcing!
+// io.jl:125-143): a left fold over the area's cells in ascending land index, divided by their
+// count. One thread per reservoir.
+__global__ void k_res_forcing(Dev d) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.nres) return;
+    double sp = 0.0, se = 0.0;
+    const int e0 = d.res_cov_ptr[k], e1 = d.res_cov_ptr[k + 1];
+    for (int e = e0; e < e1; ++e) {
+        const int q = d.res_cov_idx[e];
+        sp += d.P[q];
+        se += d.E[q];
+    }
+    d.res_P[k] = sp / (double)(e1 - e0);
+    d.res_E[k] = se / (double)(e1 - e0);
+}
+
 // Per-cell inputs of the closing subsurface bookkeeping of a step: soil parameters and the
 // vertical update's outputs that enter the soil storage update and the balances.
 struct SsfCellIn {
@@ -852,7 +984,9 @@ __device__ __forceinline__ void total_storage_finish(const Dev& d, const int p, 
     d.total_storage[p] = tot;
 }
 
-template <int N>
+// RES: the model has reservoirs (a separate instantiation keeps their code, and its registers,
+// out of the kernel that runs without them).
+template <int N, bool RES>
 __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
     __shared__ int x_k[ROUTE_BLOCK];     // per-wave exchange tables of the routing families
@@ -878,6 +1012,8 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         const uint8_t fl_ = d.flags[q_];
         const int dn = valid ? dn_ : -1;
         const bool cons_in = dn >= 0 && (dn >> 6) == c;
+        // in-wave land consumer: a reservoir outlet (flag bit 0 clear) feeds no land cell
+        const bool lcons_in = RES ? (cons_in && (fl_ & 1)) : cons_in;
         const int e0 = e0_, e1 = valid ? e1_ : e0_;
         const uint8_t fl = valid ? fl_ : 0;
         const int r = valid ? r_ : -1;
@@ -912,7 +1048,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             vo.canopy = d.canopy[q_];
             const double P_ = d.P[q_], aet_ = d.aet[q_], leak_ = d.leakage[q_], sp_ = d.store_prev[q_];
             const double ss_ = d.snow_s[q_], sw_ = d.snow_w[q_];
-            vo.P = d.balance ? P_ : 0.0;
+            vo.P = d.balance ? ((RES && (fl_ & 8)) ? 0.0 : P_) : 0.0;
             vo.aet = d.balance ? aet_ : 0.0;
             vo.leak = d.balance ? leak_ : 0.0;
             vo.store_prev = d.balance ? sp_ : 0.0;
@@ -936,6 +1072,8 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         const int rr = riv ? r : 0;
         const int re0 = d.rup_ptr[rr], re1_ = d.rup_ptr[riv ? rr + 1 : rr], re1 = riv ? re1_ : re0;
         const double rfl = d.r_fl[rr], rfw = d.r_fw[rr], ralpha = d.r_alpha[rr], rarea = d.r_area[rr];
+        const int rres_ = RES ? d.res_of_r[rr] : -1;
+        const int rres = riv ? rres_ : -1;
         const double rq0 = d.r_q[rr], rh0 = d.r_h[rr], r_store_prev = d.r_storage[rr];
         Column<N> col;
         soil_layers<N>(dsoil, d.cfg_cum, d.cfg_thick, col.act, col.cum);
@@ -978,7 +1116,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             return qn;
         };
         if (!run_family<FOLD_SSF>(a, lane, c, valid, a.S_s, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ssf,
-                                  (size_t)n, p, (fl & 1) && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q, ssf_compute))
+                                  (size_t)n, p, (fl & 1) && !lcons_in, lcons_in ? (dn & 63) : -1, lds_k, lds_q, ssf_compute))
             return;
         if (col_upd) update_ustorelayerdepth<N>(col, col_z0, col_z1);
 
@@ -1013,7 +1151,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             return qn;
         };
         if (!run_family<FOLD_OL>(a, lane, c, valid, a.S_l, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ol, (size_t)n,
-                                 p, (fl & 1) && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q, ol_compute))
+                                 p, (fl & 1) && !lcons_in, lcons_in ? (dn & 63) : -1, lds_k, lds_q, ol_compute))
             return;
         double r_inwater = 0.0;
         if (valid)
@@ -1029,10 +1167,10 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         inflow -= 0.0 / rfl;
         const double rqlat_all = rqlat + inflow;
         KwPrep rp = kw_prepare(rq, rqlat_all, ralpha, a.S_r == 1 ? a.dts_r[1] : a.dts_r[0], rfl);
-        auto riv_compute = [&](int k, double sumq, double) -> double {
+        auto riv_compute = [&](int k, double sumq, double res_out) -> double {
             const double dts = (k == a.S_r - 1) ? a.dts_r[1] : a.dts_r[0];
             if (k > 0) rp = kw_prepare(rq, rqlat_all, ralpha, dts, rfl);
-            const double qin = 0.0 + sumq;
+            const double qin = (RES ? res_out : 0.0) + sumq;
             double qn, ar;
             kw_solve(qin, rp, ralpha, qn, ar);
             rq = qn;
@@ -1041,9 +1179,11 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             r_q_cum += qn * dts;
             r_qin_cum += qin * dts;
             rqin = qin;
+            if (RES && rres >= 0) return reservoir_step(d, rres, p, qn, dts, k == 0, k == a.S_r - 1);
             return qn;
         };
-        if (!run_family<FOLD_RIV>(a, lane, c, riv, a.S_r, d.rup_node, d.rup_idx, nullptr, re0, re1, a.slot_r,
+        if (!run_family<RES ? FOLD_RIV_RES : FOLD_RIV>(a, lane, c, riv, a.S_r, d.rup_node, d.rup_idx, d.rup_res, re0,
+                                                      re1, a.slot_r,
                                   (size_t)d.nriv, rr, riv && (fl & 4) && !rcons_in, rcons_in ? (dn & 63) : -1,
                                   lds_k, lds_q, riv_compute))
             return;
@@ -1088,9 +1228,9 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_ssf(Dev d, RouteArgs a, AdAr
         const int dn_ = d.down[q_], e0 = d.up_ptr[q_], e1_ = d.up_ptr[q_ + 1];
         const uint8_t fl_ = d.flags[q_];
         const int dn = valid ? dn_ : -1;
-        const bool cons_in = dn >= 0 && (dn >> 6) == c;
-        const int e1 = valid ? e1_ : e0;
         const uint8_t fl = valid ? fl_ : 0;
+        const bool cons_in = dn >= 0 && (dn >> 6) == c && (fl & 1);  // a reservoir outlet feeds no land cell
+        const int e1 = valid ? e1_ : e0;
         const double area = ld(d.area, q_);
         const double dsoil = ld(d.d, q_), ts = ld(d.ts, q_), tr = ld(d.tr, q_), tfc = ld(d.tfc, q_);
         const double sy = ld(d.sy, q_), slope = ld(d.slope, q_), flen = ld(d.fl, q_), fw = ld(d.fw, q_);
@@ -1156,7 +1296,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_ssf(Dev d, RouteArgs a, AdAr
             x.scum[4 * (size_t)n + p] = cum.tor;
             x.scum[5 * (size_t)n + p] = cum.rech;
         } else if (valid) {
-            const double P_in = d.balance ? d.P[p] : 0.0, aet = d.balance ? d.aet[p] : 0.0;
+            const double P_in = d.balance && !(d.flags[p] & 8) ? d.P[p] : 0.0, aet = d.balance ? d.aet[p] : 0.0;
             const double leak = d.balance ? d.leakage[p] : 0.0, sp = d.balance ? d.store_prev[p] : 0.0;
             const double ss = d.snow ? d.snow_s[p] : 0.0, sw = d.snow ? d.snow_w[p] : 0.0;
             ssf_finish<N>(d, p, col,
@@ -1186,9 +1326,9 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_ol(Dev d, RouteArgs a, AdArg
         const int dn_ = d.down[q_], e0 = d.up_ptr[q_], e1_ = d.up_ptr[q_ + 1];
         const uint8_t fl_ = d.flags[q_];
         const int dn = valid ? dn_ : -1;
-        const bool cons_in = dn >= 0 && (dn >> 6) == c;
-        const int e1 = valid ? e1_ : e0;
         const uint8_t fl = valid ? fl_ : 0;
+        const bool cons_in = dn >= 0 && (dn >> 6) == c && (fl & 1);  // a reservoir outlet feeds no land cell
+        const int e1 = valid ? e1_ : e0;
         const double alpha = ld(d.alpha, q_), sfw = ld(d.sfw, q_), flen = ld(d.fl, q_);
         double oq = d.ol_q[q_], oh = d.ol_h[q_];
         const double o_inwater = d.ol_inwater[q_];
@@ -1235,6 +1375,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_ol(Dev d, RouteArgs a, AdArg
 
 // One river sub-step of length dts, over the river cells in river-position order (river cells
 // sorted by node, a topological order of the river graph).
+template <bool RES>
 __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_riv(Dev d, RouteArgs a, AdArgs x, double dts, int first,
                                                         int last) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -1268,8 +1409,10 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_riv(Dev d, RouteArgs a, AdAr
         const double rqlat_all = rqlat + inflow;
         double r_storage = 0.0, rqin = 0.0;
         const KwPrep rpp = kw_prepare(rq, rqlat_all, ralpha, dts, rfl);
-        auto riv_compute = [&](int, double sumq, double) -> double {
-            const double qin = 0.0 + sumq;
+        const int rres_ = RES ? d.res_of_r[r] : -1, pnode = RES ? x.rp_node[q_] : 0;
+        const int rres = valid ? rres_ : -1;
+        auto riv_compute = [&](int, double sumq, double res_out) -> double {
+            const double qin = (RES ? res_out : 0.0) + sumq;
             double qn, ar;
             kw_solve(qin, rpp, ralpha, qn, ar);
             rq = qn;
@@ -1278,9 +1421,11 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_riv(Dev d, RouteArgs a, AdAr
             r_q_cum += qn * dts;
             r_qin_cum += qin * dts;
             rqin = qin;
+            if (RES && rres >= 0) return reservoir_step(d, rres, pnode, qn, dts, first != 0, last != 0);
             return qn;
         };
-        if (!run_family<FOLD_RIV>(a, lane, c, valid, 1, x.rp_up_idx, x.rp_up_idx, nullptr, e0, e1, a.slot_r,
+        if (!run_family<RES ? FOLD_RIV_RES : FOLD_RIV>(a, lane, c, valid, 1, x.rp_up_idx, x.rp_up_idx, x.rp_up_res, e0,
+                                                      e1, a.slot_r,
                                   (size_t)nr, rp, rdn >= 0 && !cons_in, cons_in ? (rdn & 63) : -1, lds_k, lds_q,
                                   riv_compute))
             return;
@@ -1490,6 +1635,8 @@ struct wfh_handle {
     SelCtl* sel_host = nullptr;  // pinned read-back
     uint64_t* keys = nullptr;    // [max(n, nriv)]
     int32_t nsub[3]{};           // sub-steps of the last step
+    int64_t nres = 0;
+    std::vector<double> fftr_land;  // flow fraction to river, land index order
 };
 
 namespace {
@@ -1814,6 +1961,53 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
             return WFH_ERR_CYCLE;
         }
     }
+    // reservoirs (NetworkReservoir, network.jl:296-360): outlets on river cells, in ascending
+    // land (= river) index; the reservoir's area from reservoir_coverage (the outlet alone without)
+    const int64_t nres = st->nres;
+    std::vector<int> res_of_land(n, -1), res_of_river(nriv, -1), res_cov_id(n, 0);
+    if (nres < 0 || (nres > 0 && (!st->reservoir_outlet || !st->reservoir_outflow_type || !st->reservoir_area ||
+                                  !st->reservoir_waterlevel))) {
+        h->err = "nres < 0, or reservoir_outlet / outflow_type / area / waterlevel missing";
+        wfh_destroy_keep_error(h, out);
+        return WFH_ERR_ARG;
+    }
+    {
+        std::vector<char> fed(nriv, 0);
+        for (int64_t k = 0; k < nres; ++k) {
+            const int64_t i = st->reservoir_outlet[k];
+            const bool ok_i = i >= 0 && i < n && (k == 0 || i > st->reservoir_outlet[k - 1]) && land_river[i] >= 0;
+            const int64_t rd = ok_i ? h->river_down[land_river[i]] : -1;
+            const int32_t t = st->reservoir_outflow_type[k];
+            if (!ok_i || rd < 0 || fed[rd] || (t != 3 && t != 4)) {
+                h->err = "reservoir " + std::to_string(k) +
+                         ": outlet must be a river cell with a river cell downstream (outlets ascending, one "
+                         "reservoir per downstream cell) and outflow type 3 or 4";
+                wfh_destroy_keep_error(h, out);
+                return WFH_ERR_ARG;
+            }
+            fed[rd] = 1;
+            res_of_land[i] = (int)k;
+            res_of_river[land_river[i]] = (int)k;
+        }
+        std::vector<int64_t> cnt(nres, 0);
+        for (int64_t i = 0; i < n && nres > 0; ++i) {
+            const int64_t id = st->reservoir_coverage ? st->reservoir_coverage[i] : res_of_land[i] + 1;
+            if (id < 0 || id > nres) {
+                h->err = "reservoir_coverage out of range";
+                wfh_destroy_keep_error(h, out);
+                return WFH_ERR_ARG;
+            }
+            res_cov_id[i] = (int)id;
+            if (id > 0) ++cnt[id - 1];
+        }
+        for (int64_t k = 0; k < nres; ++k)
+            if (cnt[k] == 0) {
+                h->err = "reservoir " + std::to_string(k) + " has an empty area";
+                wfh_destroy_keep_error(h, out);
+                return WFH_ERR_ARG;
+            }
+    }
+    h->nres = nres;
     // levels (longest upstream path) in DFS topological order, then the level-major node order
     std::vector<int32_t> level(n, 0);
     for (int64_t k = 0; k < n; ++k) {
@@ -1902,14 +2096,17 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
     std::vector<uint8_t> flags(n, 0);
     {
         std::vector<std::vector<int>> ups(n);
+        // a reservoir outlet's overland and subsurface flow go to the reservoir: it is left out
+        // of its downstream cell's upstream list (filter_upstream_nodes  utils.jl:61-71)
         for (int64_t i = 0; i < n; ++i)
-            if (h->land_down[i] >= 0) ups[h->land_down[i]].push_back((int)i);  // ascending i
+            if (h->land_down[i] >= 0 && res_of_land[i] < 0) ups[h->land_down[i]].push_back((int)i);  // ascending i
         up_idx.reserve(n);
         for (int64_t q = 0; q < n; ++q) {
             const int v = h->land_of[q];
             for (int u : ups[v]) up_idx.push_back(h->node_of[u]);
             up_ptr[q + 1] = (int)up_idx.size();
-            if (h->land_down[v] >= 0) flags[q] |= 1;
+            if (h->land_down[v] >= 0 && res_of_land[v] < 0) flags[q] |= 1;
+            if (res_cov_id[v] > 0) flags[q] |= 8;
             if (land_river[v] >= 0) {
                 flags[q] |= 2;
                 river_of[q] = (int)land_river[v];
@@ -1918,6 +2115,7 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         }
     }
     std::vector<int> rup_ptr(nriv + 1, 0), rup_idx, rup_node, down_node(n, -1);
+    std::vector<double> rup_res;
     {
         std::vector<std::vector<int>> ups(nriv);
         for (int64_t r = 0; r < nriv; ++r)
@@ -1926,6 +2124,7 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
             for (int u : ups[r]) {
                 rup_idx.push_back(u);
                 rup_node.push_back(h->node_of[h->river_land[u]]);
+                rup_res.push_back(res_of_river[u] >= 0 ? 1.0 : 0.0);
             }
             rup_ptr[r + 1] = (int)rup_idx.size();
         }
@@ -2125,6 +2324,9 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         if ((rc = put_raw<int>(h, rup_ptr, D.rup_ptr))) goto done;
         if ((rc = put_raw<int>(h, rup_idx, D.rup_idx))) goto done;
         if ((rc = put_raw<int>(h, rup_node, D.rup_node))) goto done;
+        if ((rc = put_raw<double>(h, rup_res, D.rup_res))) goto done;
+        if ((rc = put_raw<int>(h, res_of_river, D.res_of_r))) goto done;
+        h->fftr_land = fftr;
         if ((rc = put_raw<int>(h, down_node, D.down))) goto done;
         if ((rc = put_raw<double>(h, r_alpha, D.r_alpha))) goto done;
         if ((rc = put_raw<double>(h, r_fw, D.r_fw))) goto done;
@@ -2166,6 +2368,61 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         double** routs[] = {&D.r_qin, &D.r_q_avg, &D.r_qin_avg, &D.r_inwater, &D.bal_r};
         for (double** o : routs)
             if ((rc = put_river(h, {}, *o))) goto done;
+        // reservoirs (reservoir.jl:47-300): parameters, the linear storage curve's initial
+        // storage (initialize_storage  :343-362) and each area's nodes in ascending land index
+        {
+            const size_t nr = (size_t)std::max<int64_t>(nres, 1);
+            D.nres = (int)nres;
+            auto res_vec = [&](const double* src, double dflt) {
+                std::vector<double> v(nr, dflt);
+                for (int64_t k = 0; k < nres; ++k) v[k] = src ? src[k] : dflt;
+                return v;
+            };
+            const double nan = std::nan("");
+            std::vector<int> rtype(nr, 4), cov_ptr(nres + 1, 0), cov_idx;
+            for (int64_t k = 0; k < nres; ++k) rtype[k] = st->reservoir_outflow_type[k];
+            for (int64_t i = 0; i < n; ++i)
+                if (res_cov_id[i] > 0) ++cov_ptr[res_cov_id[i]];
+            for (int64_t k = 0; k < nres; ++k) cov_ptr[k + 1] += cov_ptr[k];
+            cov_idx.resize(nres > 0 ? cov_ptr[nres] : 0);
+            {
+                std::vector<int> fill(nr, 0);
+                for (int64_t i = 0; i < n; ++i)
+                    if (res_cov_id[i] > 0) {
+                        const int k = res_cov_id[i] - 1;
+                        cov_idx[cov_ptr[k] + fill[k]++] = h->node_of[i];
+                    }
+            }
+            std::vector<double> area_r = res_vec(st->reservoir_area, 1.0), wl = res_vec(st->reservoir_waterlevel, 0.0);
+            std::vector<double> sto(nr, 0.0);
+            for (int64_t k = 0; k < nres; ++k) sto[k] = area_r[k] * wl[k];
+            std::vector<double> zeros(nr, 0.0), zeros4(4 * nr, 0.0), nans(nr, nan);
+            if ((rc = put_raw<int>(h, rtype, D.res_type))) goto done;
+            if ((rc = put_raw<int>(h, cov_ptr, D.res_cov_ptr))) goto done;
+            if ((rc = put_raw<int>(h, cov_idx, D.res_cov_idx))) goto done;
+            if ((rc = put_raw<double>(h, area_r, D.res_area))) goto done;
+            if ((rc = put_raw<double>(h, res_vec(st->reservoir_demand, nan), D.res_demand))) goto done;
+            if ((rc = put_raw<double>(h, res_vec(st->reservoir_maximum_release, nan), D.res_maxrel))) goto done;
+            if ((rc = put_raw<double>(h, res_vec(st->reservoir_maximum_storage, nan), D.res_maxsto))) goto done;
+            if ((rc = put_raw<double>(h, res_vec(st->reservoir_target_full_fraction, nan), D.res_tff))) goto done;
+            if ((rc = put_raw<double>(h, res_vec(st->reservoir_target_minimum_fraction, nan), D.res_tmf))) goto done;
+            if ((rc = put_raw<double>(h, res_vec(st->reservoir_threshold, nan), D.res_thr))) goto done;
+            if ((rc = put_raw<double>(h, res_vec(st->reservoir_rating_curve_coefficient, nan), D.res_rcc))) goto done;
+            if ((rc = put_raw<double>(h, res_vec(st->reservoir_external_inflow, 0.0), D.res_ext))) goto done;
+            if ((rc = put_raw<double>(h, res_vec(st->reservoir_outflow_obs, nan), D.res_obs))) goto done;
+            double** rs[] = {&D.res_storage, &D.res_wl, &D.res_outflow, &D.res_P, &D.res_E, &D.res_outflow_avg,
+                             &D.res_inflow_avg, &D.res_actevap_avg, &D.res_abs_avg, &D.res_store_prev, &D.bal_res};
+            const std::vector<double>* init[] = {&sto, &wl, &nans, &zeros, &zeros, &zeros,
+                                                 &zeros, &zeros, &zeros, &zeros, &zeros};
+            for (size_t j = 0; j < sizeof(rs) / sizeof(rs[0]); ++j) {
+                const double* tmp = nullptr;
+                if ((rc = put_raw<double>(h, *init[j], tmp))) goto done;
+                *rs[j] = const_cast<double*>(tmp);
+            }
+            const double* cum = nullptr;
+            if ((rc = put_raw<double>(h, zeros4, cum))) goto done;
+            D.res_cum = const_cast<double*>(cum);
+        }
         hipError_t e;
         h->fP = dalloc<double>(h, n, e);
         h->fE = dalloc<double>(h, n, e);
@@ -2221,11 +2478,11 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         hipError_t oe = hipSuccess;
         switch (N) {
-            case 2: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_route<2>, ROUTE_BLOCK, 0); break;
-            case 3: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_route<3>, ROUTE_BLOCK, 0); break;
-            case 4: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_route<4>, ROUTE_BLOCK, 0); break;
-            case 5: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_route<5>, ROUTE_BLOCK, 0); break;
-            default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_route<6>, ROUTE_BLOCK, 0); break;
+            case 2: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_route<2, false>, ROUTE_BLOCK, 0); break;
+            case 3: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_route<3, false>, ROUTE_BLOCK, 0); break;
+            case 4: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_route<4, false>, ROUTE_BLOCK, 0); break;
+            case 5: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_route<5, false>, ROUTE_BLOCK, 0); break;
+            default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_route<6, false>, ROUTE_BLOCK, 0); break;
         }
         if (oe != hipSuccess || per_cu < 1) per_cu = 1;
         if (cus < 1) cus = 1;
@@ -2238,7 +2495,8 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
             std::vector<std::pair<int, int>> nr(nriv);
             for (int64_t r = 0; r < nriv; ++r) nr[r] = {h->node_of[h->river_land[r]], (int)r};
             std::sort(nr.begin(), nr.end());
-            std::vector<int> rp_r(nriv), rpos(nriv), rp_ptr(nriv + 1, 0), rp_idx, rp_down(nriv, -1);
+            std::vector<int> rp_r(nriv), rpos(nriv), rp_ptr(nriv + 1, 0), rp_idx, rp_down(nriv, -1), rp_node(nriv);
+            std::vector<double> rp_res;
             for (int64_t k = 0; k < nriv; ++k) {
                 rp_r[k] = nr[k].second;
                 rpos[nr[k].second] = (int)k;
@@ -2254,7 +2512,9 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
                         goto done;
                     }
                     rp_idx.push_back(rpos[u]);
+                    rp_res.push_back(res_of_river[u] >= 0 ? 1.0 : 0.0);
                 }
+                rp_node[k] = h->node_of[h->river_land[r]];
                 rp_ptr[k + 1] = (int)rp_idx.size();
                 rp_down[k] = h->river_down[r] >= 0 ? rpos[h->river_down[r]] : -1;
             }
@@ -2262,6 +2522,8 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
             if ((rc = put_raw<int>(h, rp_ptr, X.rp_up_ptr))) goto done;
             if ((rc = put_raw<int>(h, rp_idx, X.rp_up_idx))) goto done;
             if ((rc = put_raw<int>(h, rp_down, X.rp_down))) goto done;
+            if ((rc = put_raw<double>(h, rp_res, X.rp_up_res))) goto done;
+            if ((rc = put_raw<int>(h, rp_node, X.rp_node))) goto done;
             X.nchunks_r = (int)((nriv + WAVE - 1) / WAVE);
             hipError_t ea = hipSuccess, eb;
             X.szi = dalloc<double>(h, n, eb); if (eb != hipSuccess) ea = eb;
@@ -2306,12 +2568,17 @@ static void launch_kernels(wfh_handle* h) {
     const int n = (int)h->n;
     hipEvent_t* ev = timing_events(h);
     if (ev) hipEventRecord(ev[0], h->stream);
+    if (h->nres > 0)
+        hipLaunchKernelGGL(k_res_forcing, dim3((unsigned)((h->nres + 63) / 64)), dim3(64), 0, h->stream, h->d);
     if (!FUSE_VERTICAL) hipLaunchKernelGGL(k_vertical<N>, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d);
     if (ev) hipEventRecord(ev[1], h->stream);
     RouteArgs a = h->ra;
     a.parity = h->parity;
     h->parity ^= 1;
-    hipLaunchKernelGGL(k_route<N>, dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, a);
+    if (h->nres > 0)
+        hipLaunchKernelGGL((k_route<N, true>), dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, a);
+    else
+        hipLaunchKernelGGL((k_route<N, false>), dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, a);
     if (ev) hipEventRecord(ev[2], h->stream);
 }
 
@@ -2386,6 +2653,8 @@ static int32_t launch_adaptive(wfh_handle* h) {
     const double dt = h->cfg.dt;
     hipEvent_t* ev = timing_events(h);
     if (ev) hipEventRecord(ev[0], h->stream);
+    if (h->nres > 0)
+        hipLaunchKernelGGL(k_res_forcing, dim3((unsigned)((h->nres + 63) / 64)), dim3(64), 0, h->stream, h->d);
     hipLaunchKernelGGL(k_vertical<N>, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d);
     if (ev) hipEventRecord(ev[1], h->stream);
     const int nch_r = h->ad.nchunks_r;
@@ -2423,9 +2692,12 @@ static int32_t launch_adaptive(wfh_handle* h) {
             else if (fam == 1)
                 hipLaunchKernelGGL(k_ad_ol, dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(), h->ad,
                                    dts, first, last);
+            else if (nriv > 0 && h->nres > 0)
+                hipLaunchKernelGGL(k_ad_riv<true>, dim3(grid_r), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(), h->ad,
+                                   dts, first, last);
             else if (nriv > 0)
-                hipLaunchKernelGGL(k_ad_riv, dim3(grid_r), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(), h->ad, dts,
-                                   first, last);
+                hipLaunchKernelGGL(k_ad_riv<false>, dim3(grid_r), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(), h->ad,
+                                   dts, first, last);
             HIPCHK(h, hipGetLastError());
             t = tn;
         }
@@ -2532,6 +2804,7 @@ struct VarInfo {
     int layers;
     bool river;
     bool settable;
+    bool res = false;  // one value per reservoir
 };
 
 VarInfo var_info(wfh_handle* h, int32_t v) {
@@ -2592,6 +2865,16 @@ VarInfo var_info(wfh_handle* h, int32_t v) {
         case WFH_BALANCE_OVERLAND_ERROR: return {D.bal_ol, 1, false, false};
         case WFH_BALANCE_RIVER_ERROR: return {D.bal_r, 1, true, false};
         case WFH_RIVER_ALPHA: return {const_cast<double*>(D.r_alpha), 1, true, false};
+        case WFH_RESERVOIR_STORAGE: return {D.res_storage, 1, false, true, true};
+        case WFH_RESERVOIR_WATERLEVEL: return {D.res_wl, 1, false, true, true};
+        case WFH_RESERVOIR_OUTFLOW: return {D.res_outflow, 1, false, false, true};
+        case WFH_RESERVOIR_OUTFLOW_AVERAGE: return {D.res_outflow_avg, 1, false, false, true};
+        case WFH_RESERVOIR_INFLOW_AVERAGE: return {D.res_inflow_avg, 1, false, false, true};
+        case WFH_RESERVOIR_ACTEVAP_AVERAGE: return {D.res_actevap_avg, 1, false, false, true};
+        case WFH_RESERVOIR_ABSTRACTION_AVERAGE: return {D.res_abs_avg, 1, false, false, true};
+        case WFH_RESERVOIR_PRECIPITATION: return {D.res_P, 1, false, false, true};
+        case WFH_RESERVOIR_EVAPORATION: return {D.res_E, 1, false, false, true};
+        case WFH_BALANCE_RESERVOIR_ERROR: return {D.bal_res, 1, false, false, true};
         default: return {nullptr, 0, false, false};
     }
 }
@@ -2624,19 +2907,16 @@ extern "C" int32_t wfh_get(wfh_handle* h, int32_t var, double* dst, int64_t coun
     }
     if (var == WFH_FLOW_FRACTION_TO_RIVER) {
         if (count != n) return fail(h, WFH_ERR_ARG, "count mismatch");
-        std::vector<int> ptr(n + 1), idx;
-        std::vector<double> f;
-        HIPCHK(h, hipMemcpy(ptr.data(), h->d.up_ptr, (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
-        idx.resize(ptr[n]);
-        f.resize(ptr[n]);
-        HIPCHK(h, hipMemcpy(idx.data(), h->d.up_idx, idx.size() * sizeof(int), hipMemcpyDeviceToHost));
-        HIPCHK(h, hipMemcpy(f.data(), h->d.up_f, f.size() * sizeof(double), hipMemcpyDeviceToHost));
-        for (int64_t i = 0; i < n; ++i) dst[i] = 0.0;
-        for (size_t e = 0; e < idx.size(); ++e) dst[h->land_of[idx[e]]] = f[e];
+        std::copy(h->fftr_land.begin(), h->fftr_land.end(), dst);
         return WFH_OK;
     }
     VarInfo vi = var_info(h, var);
     if (!vi.p) return fail(h, WFH_ERR_ARG, "unknown variable");
+    if (vi.res) {
+        if (count != h->nres) return fail(h, WFH_ERR_ARG, "count mismatch");
+        if (h->nres) HIPCHK(h, hipMemcpy(dst, vi.p, h->nres * sizeof(double), hipMemcpyDeviceToHost));
+        return WFH_OK;
+    }
     if (vi.river) {
         if (count != h->nriv) return fail(h, WFH_ERR_ARG, "count mismatch");
         if (h->nriv) HIPCHK(h, hipMemcpy(dst, vi.p, h->nriv * sizeof(double), hipMemcpyDeviceToHost));
@@ -2657,6 +2937,11 @@ extern "C" int32_t wfh_set(wfh_handle* h, int32_t var, const double* src, int64_
     if (!vi.p || !vi.settable) return fail(h, WFH_ERR_ARG, "variable is not settable");
     HIPCHK(h, hipStreamSynchronize(h->stream));
     const int64_t n = h->n;
+    if (vi.res) {
+        if (count != h->nres) return fail(h, WFH_ERR_ARG, "count mismatch");
+        if (h->nres) HIPCHK(h, hipMemcpy(vi.p, src, h->nres * sizeof(double), hipMemcpyHostToDevice));
+        return WFH_OK;
+    }
     if (vi.river) {
         if (count != h->nriv) return fail(h, WFH_ERR_ARG, "count mismatch");
         if (h->nriv) HIPCHK(h, hipMemcpy(vi.p, src, h->nriv * sizeof(double), hipMemcpyHostToDevice));
@@ -2700,9 +2985,11 @@ extern "C" int32_t wfh_get_node_order(wfh_handle* h, int64_t* land_index_of_node
 static std::vector<std::pair<double*, size_t>> state_blocks(wfh_handle* h) {
     Dev& D = h->d;
     const size_t n = (size_t)h->n, r = (size_t)h->nriv;
+    const size_t nr = (size_t)h->nres;
     return {{D.canopy, n},     {D.snow_s, n}, {D.snow_w, n}, {D.tsoil, n},          {D.swd, n},
             {D.ustore, n * h->N}, {D.tsws, n}, {D.ssf_q, n}, {D.ol_q, n},           {D.ol_h, n},
-            {D.ol_storage, n}, {D.r_q, r},     {D.r_h, r},      {D.r_storage, r}};
+            {D.ol_storage, n}, {D.r_q, r},     {D.r_h, r},      {D.r_storage, r},
+            {D.res_storage, nr}, {D.res_wl, nr}, {D.res_outflow, nr}};
 }
 
 extern "C" int64_t wfh_state_size(const wfh_handle* h) {

@@ -25,6 +25,7 @@ WFH_MEM_DEVICE_NODE_ORDER = 2
 _D = ctypes.POINTER(ctypes.c_double)
 _U8 = ctypes.POINTER(ctypes.c_uint8)
 _I64 = ctypes.POINTER(ctypes.c_int64)
+_I32 = ctypes.POINTER(ctypes.c_int32)
 
 
 class WfhConfig(ctypes.Structure):
@@ -62,6 +63,11 @@ STATIC_FIELDS = [
     ("khfrac", _D),
     ("river_slope", _D), ("river_manning_n", _D), ("river_bankfull_depth", _D), ("river_length", _D),
     ("river_width", _D),
+    ("nres", ctypes.c_int64), ("reservoir_outlet", _I64), ("reservoir_coverage", _I32),
+    ("reservoir_outflow_type", _I32), ("reservoir_area", _D), ("reservoir_waterlevel", _D),
+    ("reservoir_demand", _D), ("reservoir_maximum_release", _D), ("reservoir_maximum_storage", _D),
+    ("reservoir_target_full_fraction", _D), ("reservoir_target_minimum_fraction", _D), ("reservoir_threshold", _D),
+    ("reservoir_rating_curve_coefficient", _D), ("reservoir_external_inflow", _D), ("reservoir_outflow_obs", _D),
 ]
 
 
@@ -129,14 +135,29 @@ class Var(enum.IntEnum):
     THETA_FC = enum.auto()
     SURFACE_FLOW_WIDTH = enum.auto()
     FLOW_FRACTION_TO_RIVER = enum.auto()
+    RESERVOIR_STORAGE = enum.auto()
+    RESERVOIR_WATERLEVEL = enum.auto()
+    RESERVOIR_OUTFLOW = enum.auto()
+    RESERVOIR_OUTFLOW_AVERAGE = enum.auto()
+    RESERVOIR_INFLOW_AVERAGE = enum.auto()
+    RESERVOIR_ACTEVAP_AVERAGE = enum.auto()
+    RESERVOIR_ABSTRACTION_AVERAGE = enum.auto()
+    RESERVOIR_PRECIPITATION = enum.auto()
+    RESERVOIR_EVAPORATION = enum.auto()
+    BALANCE_RESERVOIR_ERROR = enum.auto()
     VAR_COUNT = enum.auto()
 
 
 LAYERED_VARS = {Var.USTORELAYERDEPTH, Var.VWC}
 RIVER_VARS = {Var.RIVER_Q, Var.RIVER_QIN, Var.RIVER_H, Var.RIVER_STORAGE, Var.RIVER_Q_AVERAGE,
               Var.RIVER_QIN_AVERAGE, Var.RIVER_INWATER, Var.BALANCE_RIVER_ERROR, Var.RIVER_ALPHA}
+RESERVOIR_VARS = {Var.RESERVOIR_STORAGE, Var.RESERVOIR_WATERLEVEL, Var.RESERVOIR_OUTFLOW,
+                  Var.RESERVOIR_OUTFLOW_AVERAGE, Var.RESERVOIR_INFLOW_AVERAGE, Var.RESERVOIR_ACTEVAP_AVERAGE,
+                  Var.RESERVOIR_ABSTRACTION_AVERAGE, Var.RESERVOIR_PRECIPITATION, Var.RESERVOIR_EVAPORATION,
+                  Var.BALANCE_RESERVOIR_ERROR}
 SETTABLE_VARS = {Var.CANOPY_STORAGE, Var.SNOW_STORAGE, Var.SNOW_WATER, Var.SOIL_TEMPERATURE, Var.SATWATERDEPTH,
-                 Var.USTORELAYERDEPTH, Var.SSF_Q, Var.LAND_Q, Var.LAND_H, Var.RIVER_Q, Var.RIVER_H}
+                 Var.USTORELAYERDEPTH, Var.SSF_Q, Var.LAND_Q, Var.LAND_H, Var.RIVER_Q, Var.RIVER_H,
+                 Var.RESERVOIR_STORAGE, Var.RESERVOIR_WATERLEVEL}
 
 # entry points declared in include/wflow_hip.h (checked against the built library)
 EXPORTS = ["wfh_create", "wfh_set_forcing", "wfh_step", "wfh_get", "wfh_set", "wfh_get_topology",
@@ -145,8 +166,10 @@ EXPORTS = ["wfh_create", "wfh_set_forcing", "wfh_step", "wfh_get", "wfh_set", "w
            "wfh_kernel_bytes", "wfh_last_substeps"]
 
 
-def var_count(var, n, nriv, nlayers):
+def var_count(var, n, nriv, nlayers, nres=0):
     var = Var(var)
+    if var in RESERVOIR_VARS:
+        return nres
     if var in LAYERED_VARS:
         return nlayers * n
     if var in RIVER_VARS:

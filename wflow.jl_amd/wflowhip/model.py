@@ -92,6 +92,7 @@ class Model:
         self.h = h
         self.n = catchment.n
         self.nriv = catchment.nriv
+        self.nres = catchment.nres
         self.nlayers = catchment.nlayers
         self._forcing_keep = None
 
@@ -142,7 +143,7 @@ class Model:
 
     # ---- BMI-style access --------------------------------------------------------------------
     def get_value(self, var):
-        cnt = _abi.var_count(var, self.n, self.nriv, self.nlayers)
+        cnt = _abi.var_count(var, self.n, self.nriv, self.nlayers, self.nres)
         out = np.empty(cnt, np.float64)
         self._check(self.lib.wfh_get(self.h, int(var), out.ctypes.data, cnt, _abi.WFH_MEM_HOST))
         return out

@@ -97,6 +97,11 @@ class Catchment:
     def nlayers(self):
         return self.nlayers_cfg + 1
 
+    @property
+    def nres(self):
+        a = self.arrays.get("reservoir_outlet")
+        return 0 if a is None else int(a.shape[0])
+
     def config(self, water_mass_balance=True, device=0, land_dt=3600.0, river_dt=900.0, subsurface_dt=86400.0,
                adaptive=False, subsurface_alpha=1.0):
         c = _abi.WfhConfig()
@@ -120,7 +125,10 @@ class Catchment:
         """wfh_static view of the arrays (the arrays stay owned by this object)."""
         s = _abi.WfhStatic()
         s.nx, s.ny, s.n, s.nriv = self.nx, self.ny, self.n, self.nriv
+        s.nres = self.nres
         for name, ctype in _abi.STATIC_FIELDS[4:]:
+            if name == "nres":
+                continue
             a = self.arrays.get(name)
             if a is None:
                 setattr(s, name, ctype())
@@ -313,6 +321,79 @@ def basin_ids(ldd, nx, ny, cell_index=None):
             break
         lab = nxt
     return lab
+
+
+def land_downstream(catchment):
+    """Land index of each cell's D8 downstream cell (-1 for pits)."""
+    c = catchment
+    ldd = c.arrays["ldd"]
+    n = c.n
+    pos = c.arrays["cell_index"]
+    dxs = np.array([0, -1, 0, 1, -1, 0, 1, -1, 0, 1])
+    dys = np.array([0, -1, -1, -1, 0, 0, 0, 1, 1, 1])
+    tgt = (pos % c.nx + dxs[ldd]) + (pos // c.nx + dys[ldd]) * c.nx
+    k = np.searchsorted(pos, tgt).clip(0, n - 1)
+    ok = (ldd != 5) & (pos[k] == tgt)
+    return np.where(ok, k, -1)
+
+
+def add_reservoirs(catchment, count, seed=0, radius=1):
+    """Place `count` reservoirs on river cells (in place): outlets on river cells whose river
+    continues downstream, at most one reservoir feeding any river cell, each with the cells
+    within `radius` raster steps of its outlet as its area (cells of an earlier reservoir
+    excluded). Types alternate simple (4) and modified Puls (3); some abstract water (negative
+    external inflow) and one uses observed outflow. Sizes are scaled to the cell size."""
+    c = catchment
+    rng = np.random.default_rng(seed + 4242)
+    down = land_downstream(c)
+    riv = c.arrays["river_location"].astype(bool)
+    cand = [i for i in np.nonzero(riv)[0] if down[i] >= 0 and riv[down[i]]]
+    rng.shuffle(cand)
+    outlets, fed = [], set()
+    for i in cand:
+        if len(outlets) == count:
+            break
+        if down[i] in fed or i in fed or any(down[o] == i for o in outlets):
+            continue
+        outlets.append(int(i))
+        fed.add(int(down[i]))
+    if len(outlets) < count:
+        raise ValueError("not enough river cells for the reservoirs")
+    outlets = np.array(sorted(outlets), np.int64)
+    n = c.n
+    pos = c.arrays["cell_index"]
+    x, y = pos % c.nx, pos // c.nx
+    cov = np.zeros(n, np.int32)
+    for k, o in enumerate(outlets):
+        near = (np.abs(x - x[o]) <= radius) & (np.abs(y - y[o]) <= radius) & (cov == 0)
+        near[o] = True
+        cov[near] = k + 1
+    m = outlets.size
+    cell_area = c.cell * c.cell
+    A = c.arrays
+    A["reservoir_outlet"] = outlets
+    A["reservoir_coverage"] = cov
+    A["reservoir_outflow_type"] = np.where(np.arange(m) % 2 == 0, 4, 3).astype(np.int32)
+    ncov = np.bincount(cov, minlength=m + 1)[1:].astype(float)
+    area = ncov * cell_area * rng.uniform(0.6, 1.0, m)
+    A["reservoir_area"] = area
+    A["reservoir_waterlevel"] = rng.uniform(2.0, 6.0, m)
+    maxsto = area * rng.uniform(6.0, 9.0, m)
+    A["reservoir_maximum_storage"] = maxsto
+    A["reservoir_demand"] = maxsto * rng.uniform(2e-8, 8e-8, m)          # m3/s
+    A["reservoir_maximum_release"] = maxsto * rng.uniform(2e-7, 6e-7, m)
+    A["reservoir_target_full_fraction"] = rng.uniform(0.6, 0.9, m)
+    A["reservoir_target_minimum_fraction"] = rng.uniform(0.2, 0.4, m)
+    A["reservoir_threshold"] = rng.uniform(0.0, 3.0, m)
+    A["reservoir_rating_curve_coefficient"] = rng.uniform(0.05, 0.5, m)
+    ext = np.zeros(m)
+    ext[1::3] = -maxsto[1::3] * 1e-8
+    A["reservoir_external_inflow"] = ext
+    obs = np.full(m, np.nan)
+    if m >= 3:
+        obs[2] = maxsto[2] * 3e-8
+    A["reservoir_outflow_obs"] = obs
+    return c
 
 
 # ------------------------------------------------------------------------------------------
