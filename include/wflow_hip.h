@@ -71,6 +71,8 @@ typedef struct wfh_config {
      * lateral_subsurface_flow.jl:314-344), the *_dt fields are then ignored.            */
     int32_t kinematic_wave_adaptive;
     double subsurface_alpha_coefficient; /* subsurface_kinematic_wave__alpha_coefficient (:88), 1.0 */
+    int32_t snow_gravitational_transport; /* snow_gravitational_transport__flag (:70): lateral snow
+                                             transport downhill (surface_process.jl:9-19), needs snow */
 } wfh_config;
 
 /* Static maps as the reference reads them (standard names in brackets), already in SI.
@@ -230,6 +232,9 @@ typedef enum wfh_var {
     WFH_RESERVOIR_PRECIPITATION,         /* mean over the reservoir area [m s-1] */
     WFH_RESERVOIR_EVAPORATION,           /* mean potential evaporation [m s-1] */
     WFH_BALANCE_RESERVOIR_ERROR,
+    /* lateral snow transport [n] (snow_in / snow_out of the snow model) [m s-1] */
+    WFH_SNOW_IN,
+    WFH_SNOW_OUT,
     WFH_VAR_COUNT
 } wfh_var;
 

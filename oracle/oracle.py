@@ -107,6 +107,7 @@ def lib():
         L.wfo_kw_stable_timestep.argtypes = [_D, _D, _D, ctypes.c_int64, _c, _D]
         L.wfo_last_substeps.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
         L.wfo_update_reservoir.argtypes = [ctypes.c_void_p, _c, _c]
+        L.wfo_accucapacityflux.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, _D, _D, _D, _c]
         L.wfo_update_reservoir_at_river.restype = _c
         L.wfo_update_reservoir_at_river.argtypes = [ctypes.c_void_p, _c, _c]
         _lib = L
@@ -156,6 +157,18 @@ def update_reservoir(res, inflow, dt):
 def update_reservoir_at_river(res, q_v, dt):
     """update_reservoir_model! at a river cell (surface_kinwave.jl:441-489); returns the outflow."""
     return lib().wfo_update_reservoir_at_river(ctypes.byref(res), q_v, dt)
+
+
+def accucapacityflux(order, down, material, capacity, dt):
+    """accucapacityflux! (routing/utils.jl:82-108): returns (flux, new material)."""
+    order = np.ascontiguousarray(order, np.int64)
+    down = np.ascontiguousarray(down, np.int64)
+    mat = np.array(material, dtype=np.float64)
+    cap = np.ascontiguousarray(capacity, dtype=np.float64)
+    flux = np.zeros_like(mat)
+    lib().wfo_accucapacityflux(mat.size, order.ctypes.data, down.ctypes.data, mat.ctypes.data_as(_D),
+                               cap.ctypes.data_as(_D), flux.ctypes.data_as(_D), dt)
+    return flux, mat
 
 
 def quantile(v, p):

@@ -929,6 +929,8 @@ struct wfo_model {
     /* adaptive sub-steps: stable time step buffer, sub-step counts of the last step */
     double* stbuf;
     int32_t nsub[3];
+    /* lateral snow transport */
+    double *snow_in, *snow_out, *snow_flux_w, *snow_cap_s, *snow_cap_w;
     /* upstream lists without reservoir outlets (filter_upstream_nodes, utils.jl:61-71) */
     int64_t *fup_ptr, *fup_idx, *rfup_ptr, *rfup_idx;
     /* reservoirs */
@@ -1362,6 +1364,8 @@ wfo_model* wfo_create(const wfh_config* cfg, const wfh_static* st, char* err, in
     m->r_store_prev = dalloc(nriv); m->bal_land = dalloc(n); m->bal_land_rel = dalloc(n); m->bal_ssf = dalloc(n);
     m->bal_ol = dalloc(n); m->bal_r = dalloc(nriv);
     m->stbuf = dalloc(n > nriv ? n : nriv);
+    m->snow_in = dalloc(n); m->snow_out = dalloc(n); m->snow_flux_w = dalloc(n);
+    m->snow_cap_s = dalloc(n); m->snow_cap_w = dalloc(n);
     free(ldd2);
     return m;
 fail:
@@ -1401,7 +1405,8 @@ void wfo_destroy(wfo_model* m) {
         (void**)&m->res_land, (void**)&m->res_river, (void**)&m->land_res, (void**)&m->river_res,
         (void**)&m->cov_ptr, (void**)&m->cov_idx, (void**)&m->land_cov, (void**)&m->res_store_prev,
         (void**)&m->res_outflow_avg, (void**)&m->res_inflow_avg, (void**)&m->res_actevap_avg, (void**)&m->res_abs_avg,
-        (void**)&m->bal_res};
+        (void**)&m->bal_res, (void**)&m->snow_in, (void**)&m->snow_out, (void**)&m->snow_flux_w,
+        (void**)&m->snow_cap_s, (void**)&m->snow_cap_w};
     for (size_t k = 0; k < sizeof p / sizeof p[0]; ++k) free(*p[k]);
     free(m);
 }
@@ -1455,6 +1460,44 @@ static double ssf_stable_timestep(const wfo_model* m) {
             found = 1;
         }
     return (found ? mn : 0.5) * m->cfg.subsurface_alpha_coefficient;
+}
+
+/* accucapacityflux!(flux, material, network, capacity, dt)  Wflow/src/routing/utils.jl:82-108:
+ * in the reference's topological order each cell passes min(material / dt, capacity) on to its
+ * downstream cell (the unfiltered flow graph), which adds it to its own material. */
+void wfo_accucapacityflux(int64_t n, const int64_t* order, const int64_t* down, double* material,
+                          const double* capacity, double* flux, double dt) {
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t v = order[k];
+        const double flux_val = wfh_min(material[v] / dt, capacity[v]);
+        const double material_update = flux_val * dt;
+        material[v] -= material_update;
+        flux[v] = flux_val;
+        if (down[v] >= 0) material[down[v]] += material_update;
+    }
+}
+static void accucapacityflux(const wfo_model* m, double* material, const double* capacity, double* flux, double dt) {
+    wfo_accucapacityflux(m->n, m->order, m->down, material, capacity, flux, dt);
+}
+
+/* lateral_snow_transport!  Wflow/src/routing/surface/surface_process.jl:9-19 (TAN80 = 5.67,
+ * snow_storage_max = 10 m); flux_in! (routing/utils.jl) sums snow_out over the upstream lists
+ * (without reservoir outlets) in ascending id. */
+static void lateral_snow_transport(wfo_model* m, double dt) {
+    const int64_t n = m->n;
+    for (int64_t i = 0; i < n; ++i) {
+        const double frac = wfh_min(0.5, m->slope[i] / 5.67) * wfh_min(1.0, m->snow_s[i] / 10.0);
+        m->snow_cap_s[i] = frac * m->snow_s[i] / dt;
+        m->snow_cap_w[i] = m->snow_w[i] * frac / dt;
+    }
+    accucapacityflux(m, m->snow_s, m->snow_cap_s, m->snow_out, dt);
+    accucapacityflux(m, m->snow_w, m->snow_cap_w, m->snow_flux_w, dt);
+    for (int64_t i = 0; i < n; ++i) m->snow_out[i] += m->snow_flux_w[i];
+    for (int64_t v = 0; v < n; ++v) {
+        double s = 0.0;
+        for (int64_t e = m->fup_ptr[v]; e < m->fup_ptr[v + 1]; ++e) s = s + m->snow_out[m->fup_idx[e]];
+        m->snow_in[v] = s;
+    }
 }
 
 /* compute_total_storage  mass_balance.jl:96-115 */
@@ -1525,6 +1568,7 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
             m->swe[i] = o[2];
             m->snow_runoff[i] = o[4];
         }
+        if (m->cfg.snow_gravitational_transport) lateral_snow_transport(m, dt);
     }
     /* update_bc_open_water_runoff_model! / update_open_water_runoff_model!  runoff.jl:37-111 */
     for (int64_t i = 0; i < n; ++i) {
@@ -1735,9 +1779,9 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
     if (m->cfg.water_mass_balance) { /* compute_mass_balance!  mass_balance.jl:215-549 */
         for (int64_t i = 0; i < n; ++i) {
             double a = m->area[i];
-            double tin = m->ssf_qin_avg[i] / a + (m->precip[i] + 0.0) + 0.0 + 0.0 / m->area[i];
+            double tin = m->ssf_qin_avg[i] / a + (m->precip[i] + 0.0) + m->snow_in[i] + 0.0 / m->area[i];
             double vout = m->soil[i].net_runoff + m->soil[i].aet + m->net_runoff_river[i] + m->soil[i].leakage + 0.0;
-            double tout = m->ssf_q_avg[i] / a + vout + 0.0 + 0.0 / m->area[i];
+            double tout = m->ssf_q_avg[i] / a + vout + m->snow_out[i] + 0.0 / m->area[i];
             double rate = (land_total_storage(m, i) - m->land_store_prev[i]) / dt;
             double o[2];
             wfo_compute_mass_balance_error(tin, tout, rate, o);
@@ -1901,6 +1945,8 @@ int32_t wfo_get(const wfo_model* m, int32_t var, double* dst, int64_t count) {
     case WFH_THETA_FC: SOILF(theta_fc);
     case WFH_SURFACE_FLOW_WIDTH: ARR(m->sfw, n);
     case WFH_FLOW_FRACTION_TO_RIVER: ARR(m->fftr, n);
+    case WFH_SNOW_IN: ARR(m->snow_in, n);
+    case WFH_SNOW_OUT: ARR(m->snow_out, n);
     default: return WFH_ERR_ARG;
     }
 }

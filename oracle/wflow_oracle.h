@@ -82,6 +82,10 @@ void wfo_last_substeps(const wfo_model* m, int32_t* out3);
 /* Statistics.quantile!(v, p) of k >= 2 values (sorts v); the kinematic-wave stable time step
  * (surface_kinwave.jl:674-704) over n cells, buf holding n values. */
 double wfo_quantile(double* v, int64_t k, double p);
+/* accucapacityflux! (routing/utils.jl:82-108) over n nodes in topological order `order` with
+ * downstream `down` (-1 = pit); mutates material, writes flux. */
+void wfo_accucapacityflux(int64_t n, const int64_t* order, const int64_t* down, double* material,
+                          const double* capacity, double* flux, double dt);
 double wfo_kw_stable_timestep(const double* q, const double* alpha, const double* flow_length, int64_t n,
                               double p, double* buf);
 

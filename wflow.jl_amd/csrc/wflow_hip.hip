@@ -117,6 +117,10 @@ struct Dev {
     double *res_storage, *res_wl, *res_outflow, *res_P, *res_E;
     double* res_cum;           // [4][nres] sums of the step: inflow, outflow, actual evaporation, abstraction
     double *res_outflow_avg, *res_inflow_avg, *res_actevap_avg, *res_abs_avg, *res_store_prev, *bal_res;
+    // ---- lateral snow transport ----
+    int snow_lat;
+    const int *sup_ptr, *sup_idx;  // [n+1], [E]: all upstream cells, in the reference's topological order
+    double *snow_in, *snow_out;    // [n] (zeros without transport)
 };
 
 struct RouteArgs {
@@ -501,9 +505,11 @@ __device__ __forceinline__ bool slot_take(uint64_t* s, double& v) {
 //   FOLD_SSF: acc1 = sum q (1 - f)  (q_in),   acc2 = sum q f  (to_river)   lateral_subsurface_flow.jl:228-240
 //   FOLD_OL:  acc1 = sum q f (to_river),      acc2 = sum q (1 - f) (qin)    surface_kinwave.jl:317-326
 //   FOLD_RIV: acc1 = sum q                                                  surface_kinwave.jl:520
+//   FOLD_SNOW: acc1 = own material + material passed on by each upstream cell, in the reference's
+//              topological order of the upstream cells (accucapacityflux!  routing/utils.jl:82-108)
 //   FOLD_RIV_RES (a model with reservoirs): acc1 = sum q over the non-reservoir upstream cells,
 //             acc2 = outflow of an upstream reservoir (f = 1 on that edge)  surface_kinwave.jl:478,520
-enum { FOLD_SSF = 0, FOLD_OL = 1, FOLD_RIV = 2, FOLD_RIV_RES = 3 };
+enum { FOLD_SSF = 0, FOLD_OL = 1, FOLD_RIV = 2, FOLD_RIV_RES = 3, FOLD_SNOW = 4 };
 
 template <int FOLD>
 __device__ __forceinline__ void fold_in(double& acc1, double& acc2, double q, double f) {
@@ -540,13 +546,13 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
                                            const int* __restrict__ up_slot, const double* __restrict__ up_f,
                                            const int e0, const int e1, uint64_t* slots, const size_t stride,
                                            const int my_slot, const bool pub_cross, const int cons_lane,
-                                           int* lds_k, double* lds_q, Compute& compute) {
+                                           int* lds_k, double* lds_q, Compute& compute, const double acc0 = 0.0) {
     // The first PF upstream edges are buffered in registers; further edges (a cell with more
     // than PF upstream cells) are taken one per round in order, after the buffered ones.
     constexpr int PF = 4;
     int k = part ? 0 : S;
     int e = e0;
-    double acc1 = 0.0, acc2 = 0.0, qv = 0.0;
+    double acc1 = acc0, acc2 = 0.0, qv = 0.0;  // acc0: start of the first sub-step's fold
     const int npf = (e1 - e0) < PF ? (e1 - e0) : PF;
     int en[PF], es[PF];
     double ef[PF], buf[PF];
@@ -562,7 +568,7 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
         const int ej = j < npf ? e0 + j : (npf > 0 ? e0 : 0);
         un[j] = up_node[ej];
         us[j] = up_slot[ej];
-        uf[j] = (FOLD != FOLD_RIV) ? up_f[ej] : 0.0;
+        uf[j] = (FOLD != FOLD_RIV && FOLD != FOLD_SNOW) ? up_f[ej] : 0.0;
     }
     static_assert(PF == 4, "the operand list below names four edges");
     asm volatile("" : "+v"(un[0]), "+v"(un[1]), "+v"(un[2]), "+v"(un[3]), "+v"(us[0]), "+v"(us[1]), "+v"(us[2]),
@@ -625,7 +631,7 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
                 }
             if (e >= e0 + PF && e < e1) {
                 const int u = up_node[e];
-                const double f = (FOLD != FOLD_RIV) ? up_f[e] : 0.0;
+                const double f = (FOLD != FOLD_RIV && FOLD != FOLD_SNOW) ? up_f[e] : 0.0;
                 double q = 0.0;
                 bool took = false;
                 if ((u >> 6) == c) {
@@ -822,6 +828,7 @@ struct SsfCellIn {
     double ts, tr, rd, dsoil, area, sy;
     double sat_exc, runoff_l, inf_exc, aeow, nrr;
     double P_in, aet, leak, canopy, snow_s, snow_w, store_prev;
+    double snow_in, snow_out;  // lateral snow transport (0 without it)
 };
 // Sums over the subsurface sub-steps of flux x sub-step length.
 struct SsfCum {
@@ -904,9 +911,9 @@ __device__ __forceinline__ SsfEnd ssf_finish(const Dev& d, const int p, const Co
             fout -= rech_avg;
         d.bal_ssf[p] = (s_qin_avg + fin) - (s_q_avg + fout) - s_qnet_avg;
         // land balance  mass_balance.jl:215-253
-        const double t_in = s_qin_avg / area + (c.P_in + 0.0) + 0.0 + 0.0 / area;
+        const double t_in = s_qin_avg / area + (c.P_in + 0.0) + c.snow_in + 0.0 / area;
         const double vout = net_runoff + c.aet + c.nrr + c.leak + 0.0;
-        const double t_out = s_q_avg / area + vout + 0.0 + 0.0 / area;
+        const double t_out = s_q_avg / area + vout + c.snow_out + 0.0 / area;
         const double snow = d.snow ? c.snow_s + c.snow_w : 0.0 + 0.0;
         const double storage = (swd + usd) + c.canopy + snow + 0.0 * 0.0 + 0.0;
         const double rate = (storage - c.store_prev) / dt;
@@ -984,9 +991,9 @@ __device__ __forceinline__ void total_storage_finish(const Dev& d, const int p, 
     d.total_storage[p] = tot;
 }
 
-// RES: the model has reservoirs (a separate instantiation keeps their code, and its registers,
-// out of the kernel that runs without them).
-template <int N, bool RES>
+// EXT: the model has reservoirs or lateral snow transport (a separate instantiation keeps their
+// code, and its registers, out of the kernel that runs without them).
+template <int N, bool EXT>
 __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
     __shared__ int x_k[ROUTE_BLOCK];     // per-wave exchange tables of the routing families
@@ -1013,7 +1020,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         const int dn = valid ? dn_ : -1;
         const bool cons_in = dn >= 0 && (dn >> 6) == c;
         // in-wave land consumer: a reservoir outlet (flag bit 0 clear) feeds no land cell
-        const bool lcons_in = RES ? (cons_in && (fl_ & 1)) : cons_in;
+        const bool lcons_in = EXT ? (cons_in && (fl_ & 1)) : cons_in;
         const int e0 = e0_, e1 = valid ? e1_ : e0_;
         const uint8_t fl = valid ? fl_ : 0;
         const int r = valid ? r_ : -1;
@@ -1048,7 +1055,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             vo.canopy = d.canopy[q_];
             const double P_ = d.P[q_], aet_ = d.aet[q_], leak_ = d.leakage[q_], sp_ = d.store_prev[q_];
             const double ss_ = d.snow_s[q_], sw_ = d.snow_w[q_];
-            vo.P = d.balance ? ((RES && (fl_ & 8)) ? 0.0 : P_) : 0.0;
+            vo.P = d.balance ? ((EXT && (fl_ & 8)) ? 0.0 : P_) : 0.0;
             vo.aet = d.balance ? aet_ : 0.0;
             vo.leak = d.balance ? leak_ : 0.0;
             vo.store_prev = d.balance ? sp_ : 0.0;
@@ -1061,6 +1068,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         // Every one of these arrays is written, in this kernel, only by the lane that reads it,
         // after the read.
         const double rd = ld(d.rd, q_);
+        const double snow_in = EXT ? d.snow_in[q_] : 0.0, snow_out = EXT ? d.snow_out[q_] : 0.0;
         const double sat_exc = vo.sat_exc, runoff_l = vo.runoff_l, inf_exc = vo.inf_exc;
         const double aeow = vo.aeow_l, nrr = vo.nrr, canopy = vo.canopy;
         const double P_in = vo.P, aet = vo.aet;
@@ -1072,7 +1080,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         const int rr = riv ? r : 0;
         const int re0 = d.rup_ptr[rr], re1_ = d.rup_ptr[riv ? rr + 1 : rr], re1 = riv ? re1_ : re0;
         const double rfl = d.r_fl[rr], rfw = d.r_fw[rr], ralpha = d.r_alpha[rr], rarea = d.r_area[rr];
-        const int rres_ = RES ? d.res_of_r[rr] : -1;
+        const int rres_ = EXT ? d.res_of_r[rr] : -1;
         const int rres = riv ? rres_ : -1;
         const double rq0 = d.r_q[rr], rh0 = d.r_h[rr], r_store_prev = d.r_storage[rr];
         Column<N> col;
@@ -1125,7 +1133,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         if (valid)
             se = ssf_finish<N>(d, p, col,
                                SsfCellIn{ts, tr, rd, dsoil, area, sy, sat_exc, runoff_l, inf_exc, aeow, nrr, P_in, aet,
-                                         leak, canopy, snow_s, snow_w, store_prev},
+                                         leak, canopy, snow_s, snow_w, store_prev, snow_in, snow_out},
                                cum, sq, szi);
         const double o_inwater = se.o_inwater, s_tor_avg = se.s_tor_avg, usd = se.usd, swd = se.swd;
 
@@ -1170,7 +1178,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         auto riv_compute = [&](int k, double sumq, double res_out) -> double {
             const double dts = (k == a.S_r - 1) ? a.dts_r[1] : a.dts_r[0];
             if (k > 0) rp = kw_prepare(rq, rqlat_all, ralpha, dts, rfl);
-            const double qin = (RES ? res_out : 0.0) + sumq;
+            const double qin = (EXT ? res_out : 0.0) + sumq;
             double qn, ar;
             kw_solve(qin, rp, ralpha, qn, ar);
             rq = qn;
@@ -1179,10 +1187,10 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             r_q_cum += qn * dts;
             r_qin_cum += qin * dts;
             rqin = qin;
-            if (RES && rres >= 0) return reservoir_step(d, rres, p, qn, dts, k == 0, k == a.S_r - 1);
+            if (EXT && rres >= 0) return reservoir_step(d, rres, p, qn, dts, k == 0, k == a.S_r - 1);
             return qn;
         };
-        if (!run_family<RES ? FOLD_RIV_RES : FOLD_RIV>(a, lane, c, riv, a.S_r, d.rup_node, d.rup_idx, d.rup_res, re0,
+        if (!run_family<EXT ? FOLD_RIV_RES : FOLD_RIV>(a, lane, c, riv, a.S_r, d.rup_node, d.rup_idx, d.rup_res, re0,
                                                       re1, a.slot_r,
                                   (size_t)d.nriv, rr, riv && (fl & 4) && !rcons_in, rcons_in ? (dn & 63) : -1,
                                   lds_k, lds_q, riv_compute))
@@ -1302,7 +1310,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_ssf(Dev d, RouteArgs a, AdAr
             ssf_finish<N>(d, p, col,
                           SsfCellIn{ts, tr, ld(d.rd, p), dsoil, area, sy, d.sat_excess[p], d.runoff_land[p],
                                     d.inf_excess[p], d.aeow_l[p], d.net_runoff_river[p], P_in, aet, leak, d.canopy[p],
-                                    ss, sw, sp},
+                                    ss, sw, sp, d.snow_in[p], d.snow_out[p]},
                           cum, sq, szi);
         }
     }
@@ -1460,6 +1468,74 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
         v = w < v ? w : v;
     }
     return v;
+}
+
+// ============================================================================================
+// Lateral snow transport (snow_gravitational_transport__flag)  surface_process.jl:9-19
+// ============================================================================================
+// One dataflow launch after the vertical kernel: snow storage, then snow water, each passed down
+// the unfiltered flow graph with accucapacityflux! (routing/utils.jl:82-108); a cell adds what its
+// upstream cells pass on, in the reference's topological order, to its own material, passes on
+// min(material / dt, capacity) and keeps the rest. The capacities come from the state before
+// the transport (TAN80 = 5.67, snow_storage_max = 10 m).
+__global__ void __launch_bounds__(ROUTE_BLOCK) k_snow_transport(Dev d, RouteArgs a) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    __shared__ int x_k[ROUTE_BLOCK];
+    __shared__ double x_q[ROUTE_BLOCK];
+    int* const lds_k = x_k + (threadIdx.x & ~(WAVE - 1));
+    double* const lds_q = x_q + (threadIdx.x & ~(WAVE - 1));
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[a.parity ^ 1] = 0ull;
+    const int n = d.n;
+    const double dt = d.dt;
+    for (;;) {
+        AD_TICKET(a.nchunks)
+        const int p = c * WAVE + lane;
+        const bool valid = p < n;
+        const int q_ = valid ? p : 0;
+        const int dn_ = d.down[q_], e0 = d.sup_ptr[q_], e1_ = d.sup_ptr[q_ + 1];
+        const int dn = valid ? dn_ : -1;
+        const bool cons_in = dn >= 0 && (dn >> 6) == c;
+        const int e1 = valid ? e1_ : e0;
+        const double ss = d.snow_s[q_], sw = d.snow_w[q_], slope = ld(d.slope, q_);
+        const double frac = wfh_min(0.5, slope / 5.67) * wfh_min(1.0, ss / 10.0);
+        const double cap_s = frac * ss / dt, cap_w = sw * frac / dt;
+        double ss_new = ss, sw_new = sw, flux_s = 0.0, flux_w = 0.0;
+        auto pass_s = [&](int, double material, double) -> double {
+            const double flux_val = wfh_min(material / dt, cap_s);
+            const double update = flux_val * dt;
+            ss_new = material - update;
+            flux_s = flux_val;
+            return update;
+        };
+        if (!run_family<FOLD_SNOW>(a, lane, c, valid, 1, d.sup_idx, d.sup_idx, nullptr, e0, e1, a.slot_ssf, (size_t)n,
+                                   p, dn >= 0 && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q, pass_s, ss))
+            return;
+        auto pass_w = [&](int, double material, double) -> double {
+            const double flux_val = wfh_min(material / dt, cap_w);
+            const double update = flux_val * dt;
+            sw_new = material - update;
+            flux_w = flux_val;
+            return update;
+        };
+        if (!run_family<FOLD_SNOW>(a, lane, c, valid, 1, d.sup_idx, d.sup_idx, nullptr, e0, e1, a.slot_ol, (size_t)n,
+                                   p, dn >= 0 && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q, pass_w, sw))
+            return;
+        if (valid) {
+            d.snow_s[p] = ss_new;
+            d.snow_w[p] = sw_new;
+            d.snow_out[p] = flux_s + flux_w;
+        }
+    }
+}
+
+// flux_in!  routing/utils.jl:162-167: snow_in = sum of snow_out over the upstream lists (without
+// reservoir outlets), ascending id.
+__global__ void k_snow_in(Dev d) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.n) return;
+    double s = 0.0;
+    for (int e = d.up_ptr[p]; e < d.up_ptr[p + 1]; ++e) s = s + d.snow_out[d.up_idx[e]];
+    d.snow_in[p] = s;
 }
 
 // ---- stable time steps ----
@@ -2327,6 +2403,30 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         if ((rc = put_raw<double>(h, rup_res, D.rup_res))) goto done;
         if ((rc = put_raw<int>(h, res_of_river, D.res_of_r))) goto done;
         h->fftr_land = fftr;
+        // lateral snow transport: every upstream cell (the flow graph itself), ordered by the
+        // reference's topological order (the order accucapacityflux! passes material on in)
+        D.snow_lat = (cfg->snow_gravitational_transport && cfg->snow) ? 1 : 0;
+        {
+            std::vector<int> sup_ptr(n + 1, 0), sup_idx;
+            if (D.snow_lat) {
+                std::vector<int64_t> pos(n);
+                for (int64_t k = 0; k < n; ++k) pos[h->land_order[k]] = k;
+                std::vector<std::vector<int>> ups(n);
+                for (int64_t i = 0; i < n; ++i)
+                    if (h->land_down[i] >= 0) ups[h->land_down[i]].push_back((int)i);
+                sup_idx.reserve(n);
+                for (int64_t q = 0; q < n; ++q) {
+                    std::vector<int>& u = ups[h->land_of[q]];
+                    std::sort(u.begin(), u.end(), [&](int x, int y) { return pos[x] < pos[y]; });
+                    for (int w : u) sup_idx.push_back(h->node_of[w]);
+                    sup_ptr[q + 1] = (int)sup_idx.size();
+                }
+            }
+            if ((rc = put_raw<int>(h, sup_ptr, D.sup_ptr))) goto done;
+            if ((rc = put_raw<int>(h, sup_idx, D.sup_idx))) goto done;
+            if ((rc = put_state(h, {}, D.snow_in))) goto done;
+            if ((rc = put_state(h, {}, D.snow_out))) goto done;
+        }
         if ((rc = put_raw<int>(h, down_node, D.down))) goto done;
         if ((rc = put_raw<double>(h, r_alpha, D.r_alpha))) goto done;
         if ((rc = put_raw<double>(h, r_fw, D.r_fw))) goto done;
@@ -2563,6 +2663,15 @@ static hipEvent_t* timing_events(wfh_handle* h) {
     return &h->ev[3 * h->timed_steps++];
 }
 
+// lateral_snow_transport! and flux_in! (between the vertical update and the routing)
+static void launch_snow_transport(wfh_handle* h) {
+    RouteArgs a = h->ra;
+    a.parity = h->parity;
+    h->parity ^= 1;
+    hipLaunchKernelGGL(k_snow_transport, dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, a);
+    hipLaunchKernelGGL(k_snow_in, dim3((unsigned)((h->n + 255) / 256)), dim3(256), 0, h->stream, h->d);
+}
+
 template <int N>
 static void launch_kernels(wfh_handle* h) {
     const int n = (int)h->n;
@@ -2572,10 +2681,11 @@ static void launch_kernels(wfh_handle* h) {
         hipLaunchKernelGGL(k_res_forcing, dim3((unsigned)((h->nres + 63) / 64)), dim3(64), 0, h->stream, h->d);
     if (!FUSE_VERTICAL) hipLaunchKernelGGL(k_vertical<N>, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d);
     if (ev) hipEventRecord(ev[1], h->stream);
+    if (h->d.snow_lat) launch_snow_transport(h);
     RouteArgs a = h->ra;
     a.parity = h->parity;
     h->parity ^= 1;
-    if (h->nres > 0)
+    if (h->nres > 0 || h->d.snow_lat)
         hipLaunchKernelGGL((k_route<N, true>), dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, a);
     else
         hipLaunchKernelGGL((k_route<N, false>), dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, a);
@@ -2657,6 +2767,7 @@ static int32_t launch_adaptive(wfh_handle* h) {
         hipLaunchKernelGGL(k_res_forcing, dim3((unsigned)((h->nres + 63) / 64)), dim3(64), 0, h->stream, h->d);
     hipLaunchKernelGGL(k_vertical<N>, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d);
     if (ev) hipEventRecord(ev[1], h->stream);
+    if (h->d.snow_lat) launch_snow_transport(h);
     const int nch_r = h->ad.nchunks_r;
     const int grid_r = std::max(1, std::min(h->route_grid, (nch_r + (ROUTE_BLOCK / WAVE) - 1) / (ROUTE_BLOCK / WAVE)));
     auto args = [&]() {
@@ -2875,6 +2986,8 @@ VarInfo var_info(wfh_handle* h, int32_t v) {
         case WFH_RESERVOIR_PRECIPITATION: return {D.res_P, 1, false, false, true};
         case WFH_RESERVOIR_EVAPORATION: return {D.res_E, 1, false, false, true};
         case WFH_BALANCE_RESERVOIR_ERROR: return {D.bal_res, 1, false, false, true};
+        case WFH_SNOW_IN: return {D.snow_in, 1, false, false};
+        case WFH_SNOW_OUT: return {D.snow_out, 1, false, false};
         default: return {nullptr, 0, false, false};
     }
 }

@@ -43,6 +43,7 @@ class WfhConfig(ctypes.Structure):
         ("water_mass_balance", ctypes.c_int32),
         ("kinematic_wave_adaptive", ctypes.c_int32),
         ("subsurface_alpha_coefficient", ctypes.c_double),
+        ("snow_gravitational_transport", ctypes.c_int32),
     ]
 
 
@@ -145,6 +146,8 @@ class Var(enum.IntEnum):
     RESERVOIR_PRECIPITATION = enum.auto()
     RESERVOIR_EVAPORATION = enum.auto()
     BALANCE_RESERVOIR_ERROR = enum.auto()
+    SNOW_IN = enum.auto()
+    SNOW_OUT = enum.auto()
     VAR_COUNT = enum.auto()
 
 

@@ -103,7 +103,7 @@ class Catchment:
         return 0 if a is None else int(a.shape[0])
 
     def config(self, water_mass_balance=True, device=0, land_dt=3600.0, river_dt=900.0, subsurface_dt=86400.0,
-               adaptive=False, subsurface_alpha=1.0):
+               adaptive=False, subsurface_alpha=1.0, snow_transport=False):
         c = _abi.WfhConfig()
         c.abi_version = _abi.WFH_ABI_VERSION
         c.device = device
@@ -119,6 +119,7 @@ class Catchment:
         c.water_mass_balance = int(water_mass_balance)
         c.kinematic_wave_adaptive = int(adaptive)
         c.subsurface_alpha_coefficient = subsurface_alpha
+        c.snow_gravitational_transport = int(snow_transport)
         return c
 
     def static(self):
