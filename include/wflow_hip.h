@@ -264,7 +264,11 @@ int32_t wfh_step(wfh_handle* h);
 int32_t wfh_last_substeps(wfh_handle* h, int32_t* subsurface, int32_t* land, int32_t* river);
 
 /* BMI.get_value / set_value (Wflow/src/bmi.jl:203-262), land-index / river-index order.
- * count = number of doubles the caller provides (n, nriv or N*n). */
+ * count = number of doubles the caller provides (n, nriv, nres or N*n). memkind WFH_MEM_HOST:
+ * host memory; WFH_MEM_DEVICE: device memory in land-index order (permuted by a kernel);
+ * WFH_MEM_DEVICE_NODE_ORDER: device memory in the library's node order (copied). A device source
+ * must be ready when the call is made (synchronise the stream that wrote it); every variant has
+ * completed when it returns. */
 int32_t wfh_get(wfh_handle* h, int32_t var, double* dst, int64_t count, int32_t memkind);
 int32_t wfh_set(wfh_handle* h, int32_t var, const double* src, int64_t count, int32_t memkind);
 

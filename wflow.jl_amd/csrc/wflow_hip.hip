@@ -3004,9 +3004,18 @@ static bool prm_var(wfh_handle* h, int32_t v, Prm& p) {
     }
 }
 
+// Device-memory variants of wfh_get / wfh_set: land variables are permuted between the library's
+// node order and the caller's land-index order by a gather / scatter kernel (or copied as they
+// are for WFH_MEM_DEVICE_NODE_ORDER); river and reservoir variables are copied. Both complete
+// before returning.
+static int32_t get_device(wfh_handle* h, int32_t var, double* dst, int64_t count, int32_t memkind);
+static int32_t set_device(wfh_handle* h, int32_t var, const double* src, int64_t count, int32_t memkind);
+
 extern "C" int32_t wfh_get(wfh_handle* h, int32_t var, double* dst, int64_t count, int32_t memkind) {
     if (!h || !dst) return WFH_ERR_ARG;
-    if (memkind != WFH_MEM_HOST) return fail(h, WFH_ERR_ARG, "wfh_get supports WFH_MEM_HOST");
+    if (memkind == WFH_MEM_DEVICE || memkind == WFH_MEM_DEVICE_NODE_ORDER)
+        return get_device(h, var, dst, count, memkind);
+    if (memkind != WFH_MEM_HOST) return fail(h, WFH_ERR_ARG, "bad memkind");
     int32_t rc = wfh_sync(h);
     if (rc) return rc;
     const int64_t n = h->n;
@@ -3045,7 +3054,9 @@ extern "C" int32_t wfh_get(wfh_handle* h, int32_t var, double* dst, int64_t coun
 
 extern "C" int32_t wfh_set(wfh_handle* h, int32_t var, const double* src, int64_t count, int32_t memkind) {
     if (!h || !src) return WFH_ERR_ARG;
-    if (memkind != WFH_MEM_HOST) return fail(h, WFH_ERR_ARG, "wfh_set supports WFH_MEM_HOST");
+    if (memkind == WFH_MEM_DEVICE || memkind == WFH_MEM_DEVICE_NODE_ORDER)
+        return set_device(h, var, src, count, memkind);
+    if (memkind != WFH_MEM_HOST) return fail(h, WFH_ERR_ARG, "bad memkind");
     VarInfo vi = var_info(h, var);
     if (!vi.p || !vi.settable) return fail(h, WFH_ERR_ARG, "variable is not settable");
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -3072,6 +3083,64 @@ extern "C" int32_t wfh_set(wfh_handle* h, int32_t var, const double* src, int64_
     for (int k = 0; k < vi.layers; ++k)
         for (int64_t q = 0; q < n; ++q) tmp[k * n + q] = src[k * n + h->land_of[q]];
     HIPCHK(h, hipMemcpy(vi.p, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+    return WFH_OK;
+}
+
+static int32_t get_device(wfh_handle* h, int32_t var, double* dst, int64_t count, int32_t memkind) {
+    int32_t rc = wfh_sync(h);
+    if (rc) return rc;
+    const int64_t n = h->n;
+    const bool node = memkind == WFH_MEM_DEVICE_NODE_ORDER;
+    VarInfo vi = var_info(h, var);
+    if (!vi.p) {  // derived parameters: through host memory
+        std::vector<double> tmp((size_t)count);
+        if ((rc = wfh_get(h, var, tmp.data(), count, WFH_MEM_HOST))) return rc;
+        if (node && count == n) {
+            std::vector<double> t2((size_t)n);
+            for (int64_t q = 0; q < n; ++q) t2[q] = tmp[h->land_of[q]];
+            tmp.swap(t2);
+        }
+        HIPCHK(h, hipMemcpy(dst, tmp.data(), (size_t)count * sizeof(double), hipMemcpyHostToDevice));
+        return WFH_OK;
+    }
+    const int64_t want = vi.res ? h->nres : vi.river ? h->nriv : n * vi.layers;
+    if (count != want) return fail(h, WFH_ERR_ARG, "count mismatch");
+    if (count == 0) return WFH_OK;
+    if (vi.res || vi.river || node) {
+        HIPCHK(h, hipMemcpyAsync(dst, vi.p, (size_t)count * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    } else {
+        for (int k = 0; k < vi.layers; ++k)
+            hipLaunchKernelGGL(k_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, dst + k * n,
+                               vi.p + k * n, h->d_land_of, (int)n);
+        HIPCHK(h, hipGetLastError());
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return WFH_OK;
+}
+
+static int32_t set_device(wfh_handle* h, int32_t var, const double* src, int64_t count, int32_t memkind) {
+    VarInfo vi = var_info(h, var);
+    if (!vi.p || !vi.settable) return fail(h, WFH_ERR_ARG, "variable is not settable");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const int64_t n = h->n;
+    const bool node = memkind == WFH_MEM_DEVICE_NODE_ORDER;
+    const int64_t want = vi.res ? h->nres : vi.river ? h->nriv : n * vi.layers;
+    if (count != want) return fail(h, WFH_ERR_ARG, "count mismatch");
+    if (var == WFH_RIVER_H) {  // keeps the river storage consistent: through host memory
+        std::vector<double> tmp((size_t)count);
+        HIPCHK(h, hipMemcpy(tmp.data(), src, (size_t)count * sizeof(double), hipMemcpyDeviceToHost));
+        return wfh_set(h, var, tmp.data(), count, WFH_MEM_HOST);
+    }
+    if (count == 0) return WFH_OK;
+    if (vi.res || vi.river || node) {
+        HIPCHK(h, hipMemcpyAsync(vi.p, src, (size_t)count * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    } else {
+        for (int k = 0; k < vi.layers; ++k)
+            hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, vi.p + k * n,
+                               src + k * n, h->d_land_of, (int)n);
+        HIPCHK(h, hipGetLastError());
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return WFH_OK;
 }
 

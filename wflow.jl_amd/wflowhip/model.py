@@ -156,6 +156,20 @@ class Model:
 
     set = set_value
 
+    def get_value_device(self, var, out, node_order=False):
+        """BMI get_value into a device tensor (land-index order, or node order), no host copy."""
+        kind = _abi.WFH_MEM_DEVICE_NODE_ORDER if node_order else _abi.WFH_MEM_DEVICE
+        cnt = _abi.var_count(var, self.n, self.nriv, self.nlayers, self.nres)
+        assert out.is_cuda and out.dtype.itemsize == 8 and out.is_contiguous() and out.numel() == cnt
+        self._check(self.lib.wfh_get(self.h, int(var), out.data_ptr(), cnt, kind))
+        return out
+
+    def set_value_device(self, var, src, node_order=False):
+        """BMI set_value from a device tensor (the tensor must be ready: synchronise its stream)."""
+        kind = _abi.WFH_MEM_DEVICE_NODE_ORDER if node_order else _abi.WFH_MEM_DEVICE
+        assert src.is_cuda and src.dtype.itemsize == 8 and src.is_contiguous()
+        self._check(self.lib.wfh_set(self.h, int(var), src.data_ptr(), src.numel(), kind))
+
     def topology(self):
         n, nr = self.n, self.nriv
         lo, ld = np.empty(n, np.int64), np.empty(n, np.int64)
