@@ -125,6 +125,10 @@ typedef struct wfh_static {
     const double* rootfraction;   /* [N][n] or NULL: derived from rooting_depth (soil.jl:526-548) */
     /* lateral subsurface flow */
     const double* khfrac;         /* [n] horizontal/vertical Ksat ratio                     */
+    const double* z_exp;          /* [n] or NULL: depth [m] below which the vertical and horizontal
+                                     Ksat stay constant (KvExponentialConstant / KhExponentialConstant,
+                                     soil.jl:222-271, lateral_subsurface_flow.jl:73-116); NULL = the
+                                     exponential profile                                       */
     /* river (nriv entries) */
     const double* river_slope;    /* river__slope (clamped to >= 1e-5)                     */
     const double* river_manning_n;

@@ -43,8 +43,12 @@ class SoilCell(ctypes.Structure):
                              "act_inf_soil act_inf_compacted excess_soil excess_compacted "
                              "capflux leakage recharge aet runoff net_runoff exfiltration").split()]
         + [(k, _c * 8) for k in ("vwc", "rvwc")]
-        + [(k, _c) for k in ("root_zone_storage", "vwc_root", "rvwc_root")]
+        + [(k, _c) for k in ("root_zone_storage", "vwc_root", "rvwc_root", "z_exp")]
     )
+
+    def __init__(self, *a, **kw):
+        super().__init__(*a, **kw)
+        self.z_exp = float("inf")  # exponential profile
 
     def set(self, **kw):
         for k, v in kw.items():

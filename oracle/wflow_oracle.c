@@ -397,9 +397,10 @@ static double sum_first(const double* v, int k) {
     return s;
 }
 
-/* hydraulic_conductivity_at_depth(::KvExponential)  Wflow/src/utils.jl:727-738 (m is 0-based) */
+/* hydraulic_conductivity_at_depth(::KvExponential / ::KvExponentialConstant)  Wflow/src/utils.jl:
+ * 727-762 (m is 0-based; z_exp = INFINITY for the exponential profile) */
 static double kv_at_depth(const wfo_soil* s, double z, int m) {
-    return s->kvfrac[m] * s->kv_0 * wfh_exp(-s->f * z);
+    return s->kvfrac[m] * s->kv_0 * wfh_exp(-s->f * (z < s->z_exp ? z : s->z_exp));
 }
 
 /* water_table_change  Wflow/src/utils.jl:1090-1131 */
@@ -1279,7 +1280,7 @@ wfo_model* wfo_create(const wfh_config* cfg, const wfh_static* st, char* err, in
             wfo_soil* s = &m->soil[i];
             s->N = N;
             s->theta_s = st->theta_s[i]; s->theta_r = st->theta_r[i];
-            s->kv_0 = st->kv_0[i]; s->f = st->f[i]; s->air_entry_pressure = st->air_entry_pressure[i];
+            s->kv_0 = st->kv_0[i]; s->f = st->f[i]; s->z_exp = st->z_exp ? st->z_exp[i] : INFINITY; s->air_entry_pressure = st->air_entry_pressure[i];
             s->h1 = st->h1[i]; s->h2 = st->h2[i]; s->h3_high = st->h3_high[i]; s->h3_low = st->h3_low[i];
             s->h4 = st->h4[i]; s->alpha_h1 = st->alpha_h1[i]; s->soil_thickness = st->soil_thickness[i];
             s->infiltcap_compacted = st->infiltcap_compacted[i]; s->max_leakage = st->max_leakage[i];
@@ -1348,9 +1349,21 @@ wfo_model* wfo_create(const wfh_config* cfg, const wfh_static* st, char* err, in
         m->ssf_storage[i] = m->sy[i] * (m->ssf_d[i] - m->ssf_zi[i]) * m->area[i];
         m->ssf_head[i] = m->elevation[i] - m->ssf_zi[i];
         double f = s->f;
-        m->qmax[i] = ((m->kh0[i] * m->slope[i]) / f) * (1.0 - wfh_exp(-f * m->ssf_d[i]));
-        m->ssf_q[i] = ((m->kh0[i] * m->slope[i]) / f) * (wfh_exp(-f * m->ssf_zi[i]) - wfh_exp(-f * m->ssf_d[i])) *
-                      m->flow_width[i];
+        const double ze = s->z_exp;
+        if (ze < INFINITY) { /* initialize_lateral_ssf_model!(::KhExponentialConstant)  utils.jl:940-975 */
+            const double q_constant = m->kh0[i] * wfh_exp(-f * ze) * m->slope[i] * (m->ssf_d[i] - ze);
+            m->qmax[i] = ((m->kh0[i] * m->slope[i]) / f) * (1.0 - wfh_exp(-f * ze)) + q_constant;
+            if (m->ssf_zi[i] < ze)
+                m->ssf_q[i] = (((m->kh0[i] * m->slope[i]) / f) * (wfh_exp(-f * m->ssf_zi[i]) - wfh_exp(-f * ze)) +
+                               q_constant) * m->flow_width[i];
+            else
+                m->ssf_q[i] = m->kh0[i] * wfh_exp(-f * m->ssf_zi[i]) * m->slope[i] * (m->ssf_d[i] - m->ssf_zi[i]) *
+                              m->flow_width[i];
+        } else { /* initialize_lateral_ssf_model!(::KhExponential)  utils.jl:916-937 */
+            m->qmax[i] = ((m->kh0[i] * m->slope[i]) / f) * (1.0 - wfh_exp(-f * m->ssf_d[i]));
+            m->ssf_q[i] = ((m->kh0[i] * m->slope[i]) / f) *
+                          (wfh_exp(-f * m->ssf_zi[i]) - wfh_exp(-f * m->ssf_d[i])) * m->flow_width[i];
+        }
     }
     /* ---- overland / river state ---- */
     m->ol_q = dalloc(n); m->ol_qin = dalloc(n); m->ol_h = dalloc(n); m->ol_storage = dalloc(n);
@@ -1454,7 +1467,8 @@ static double ssf_stable_timestep(const wfo_model* m) {
     int found = 0;
     for (int64_t i = 0; i < m->n; ++i)
         if (m->ssf_zi[i] > 0.0) {
-            const double c = wfo_ssf_celerity(m->ssf_zi[i], m->slope[i], m->sy[i], m->kh0[i], m->soil[i].f, NaN);
+            const double c = wfo_ssf_celerity(m->ssf_zi[i], m->slope[i], m->sy[i], m->kh0[i], m->soil[i].f,
+                                              m->soil[i].z_exp);
             const double v = m->flow_length[i] / c;
             mn = found ? wfh_min(mn, v) : v;
             found = 1;
@@ -1632,7 +1646,7 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
                 double o[4];
                 wfo_kinematic_wave_ssf(qin, m->ssf_q[v], m->ssf_zi[v], m->qnet_bnds[v], m->slope[v], m->sy[v],
                                        m->ssf_d[v], dts, m->flow_length[v], m->flow_width[v], m->qmax[v], m->kh0[v],
-                                       m->soil[v].f, NaN, &m->soil[v], o);
+                                       m->soil[v].f, m->soil[v].z_exp, &m->soil[v], o);
                 m->ssf_q[v] = o[0];
                 m->ssf_zi[v] = o[1];
                 m->ssf_qin_cum[v] += m->ssf_qin[v] * dts;

@@ -40,6 +40,7 @@ typedef struct wfo_soil {
     double act_inf_soil, act_inf_compacted, excess_soil, excess_compacted;
     double capflux, leakage, recharge, aet, runoff, net_runoff, exfiltration;
     double vwc[WFO_NMAX], rvwc[WFO_NMAX], root_zone_storage, vwc_root, rvwc_root;
+    double z_exp; /* KvExponentialConstant depth (INFINITY: exponential profile) */
 } wfo_soil;
 
 /* One reservoir (ReservoirParameters / ReservoirVariables / ReservoirBC of

@@ -326,8 +326,9 @@ struct SsfPrep {
 };
 
 WFH_HD SsfPrep ssf_prepare(double q_prev, double zi_prev, double slope, double sy, double dt, double dx, double kh0,
-                           double f) {
-    const double cel = (kh0 * wfh_exp(-f * zi_prev) * slope) / sy;
+                           double f, double z_exp) {
+    // ssf_celerity (KhExponentialConstant: the depth is capped at z_exp; INFINITY = KhExponential)
+    const double cel = (kh0 * wfh_exp(-f * (zi_prev < z_exp ? zi_prev : z_exp)) * slope) / sy;
     SsfPrep p;
     p.dt_dx = dt / dx;
     p.cinv = 1.0 / cel;
@@ -416,7 +417,7 @@ WFH_HD void update_ustorelayerdepth(Column<N>& s, double zi_prev, double zi) {
 template <int N>
 WFH_HD bool kinematic_wave_ssf(double q_in, double q_prev, double zi_prev, double q_net, double slope, double sy,
                                double d, double dt, double dx, double dw, double q_max, double kh0, double f,
-                               const SsfPrep& pp, Column<N>& soil, const bool defer_update, double& q_out,
+                               double z_exp, const SsfPrep& pp, Column<N>& soil, const bool defer_update, double& q_out,
                                double& zi_out, double& exf_out, double& net_out) {
     if (q_in + q_prev == 0.0 && q_net <= 0.0) {
         q_out = 0.0;
@@ -454,7 +455,7 @@ WFH_HD bool kinematic_wave_ssf(double q_in, double q_prev, double zi_prev, doubl
         const double dt_s = dt / its_f;
         double q_sum = 0.0, exf_sum = 0.0, net_sum = 0.0;
         for (long i = 0; i < its; ++i) {
-            const double cel = (kh0 * wfh_exp(-f * zi_prev) * slope) / sy;
+            const double cel = (kh0 * wfh_exp(-f * (zi_prev < z_exp ? zi_prev : z_exp)) * slope) / sy;
             C = (dt_s / dx) * q_in + q_prev / cel + q_net * (dt_s / dx);
             q = kw_ssf_newton_raphson(q_prev, C, cel, dt_s, dx);
             q = wfh_min(q, (q_max * dw));

@@ -76,6 +76,7 @@ struct Dev {
         cfsoil, rd;
     PrmL c, kvfrac, rootfrac;
     Prm area, fl, fw, slope, sy, kh0, qmax, alpha, sfw, elev;
+    Prm zexp;  // depth below which Ksat stays constant (INFINITY: exponential profile)
     // topology
     const int* up_ptr;   // [n+1]
     const int* up_idx;   // [E] node ids of upstream cells
@@ -192,7 +193,7 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
     const double r_h0 = d.r_h[r >= 0 ? r : 0];
     const double rf = ld(d.rf, i), wf = ld(d.wf, i), p_sfrac = ld(d.sfrac, i);
     const double ts = ld(d.ts, i), tr = ld(d.tr, i), tfc = ld(d.tfc, i), dsoil = ld(d.d, i);
-    const double kv0 = ld(d.kv0, i), fk = ld(d.f, i);
+    const double kv0 = ld(d.kv0, i), fk = ld(d.f, i), zexp = ld(d.zexp, i);
     const double swd = d.swd[i], tsoil0 = d.tsoil[i];
     const double compf = ld(d.compf, i), icap_s = ld(d.icap_s, i), icap_c = ld(d.icap_c, i);
     const double rd = ld(d.rd, i), p_h3h = ld(d.h3h, i), p_h3l = ld(d.h3l, i);
@@ -303,7 +304,7 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
             if (m >= nu) continue;
             z = (m == 0) ? th[0] : z + th[m];
             const double l_sat = th[m] * (ts - tr);
-            const double kvz = kvf[m] * kv0 * wfh_exp(-fk * z);
+            const double kvz = kvf[m] * kv0 * wfh_exp(-fk * (z < zexp ? z : zexp));
             double u = (m == 0) ? us[m] + infilt * dt : us[m] + flow * dt;
             unsatzone_flow_layer(u, kvz, l_sat, cbc[m], dt, flow);
             us[m] = u;
@@ -405,7 +406,7 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
 #pragma unroll
         for (int k = 0; k < N; ++k)
             if (k == nu - 1) kvf_nu = kvf[k];
-        const double ksat = kvf_nu * kv0 * wfh_exp(-fk * zi);
+        const double ksat = kvf_nu * kv0 * wfh_exp(-fk * (zi < zexp ? zi : zexp));
         const double mx = wfh_max(0.0, wfh_min(wfh_min(wfh_min(ksat, ae_u), ucap / dt), drainable / dt));
         double cap;
         if (zi > rd)
@@ -429,7 +430,7 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
 #pragma unroll
         for (int k = 0; k < N; ++k)
             if (k == nl - 1) kvf_nl = kvf[k];
-        const double deepksat = kvf_nl * kv0 * wfh_exp(-fk * dsoil);
+        const double deepksat = kvf_nl * kv0 * wfh_exp(-fk * (dsoil < zexp ? dsoil : zexp));
         const double deeptransfer = wfh_min(drainable / dt, deepksat);
         leak = wfh_max(0.0, wfh_min(maxleak, deeptransfer));
     }
@@ -1030,6 +1031,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         const double dsoil = ld(d.d, q_), ts = ld(d.ts, q_), tr = ld(d.tr, q_), tfc = ld(d.tfc, q_);
         const double sy = ld(d.sy, q_), slope = ld(d.slope, q_), flen = ld(d.fl, q_), fw = ld(d.fw, q_);
         const double qmax = ld(d.qmax, q_), kh0 = ld(d.kh0, q_), fk = ld(d.f, q_);
+        const double zexp = EXT ? ld(d.zexp, q_) : INFINITY;
         // The cell's vertical update (the first half of the step), either done here at the start
         // of the chunk or read back from the separate vertical kernel's outputs.
         VertOut<N> vo;
@@ -1096,12 +1098,12 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         // Inflow-independent part of the first sub-step's solve, evaluated before the chain
         // reaches this cell; the closing soil-column update of the last sub-step is applied
         // after the family (nothing in the family reads the column after that solve).
-        SsfPrep pp = ssf_prepare(sq, szi, slope, sy, a.S_s == 1 ? a.dts_s[1] : a.dts_s[0], flen, kh0, fk);
+        SsfPrep pp = ssf_prepare(sq, szi, slope, sy, a.S_s == 1 ? a.dts_s[1] : a.dts_s[0], flen, kh0, fk, zexp);
         bool col_upd = false;
         double col_z0 = 0.0, col_z1 = 0.0;
         auto ssf_compute = [&](int k, double qin, double tor) -> double {
             const double dts = (k == a.S_s - 1) ? a.dts_s[1] : a.dts_s[0];
-            if (k > 0) pp = ssf_prepare(sq, szi, slope, sy, dts, flen, kh0, fk);
+            if (k > 0) pp = ssf_prepare(sq, szi, slope, sy, dts, flen, kh0, fk, zexp);
             // flux!(::RechargeModel) with check_flux  boundary_conditions.jl:12-21,219-236
             double flx = rech * area;
             if (szi >= dsoil) flx = wfh_max(0.0, flx);
@@ -1109,8 +1111,8 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             const double qnb = 0.0 + flx;
             cum.tor += tor * dts;
             double qn, zin, exf, net;
-            if (kinematic_wave_ssf<N>(qin, sq, szi, qnb, slope, sy, dsoil, dts, flen, fw, qmax, kh0, fk, pp, col,
-                                      k == a.S_s - 1, qn, zin, exf, net)) {
+            if (kinematic_wave_ssf<N>(qin, sq, szi, qnb, slope, sy, dsoil, dts, flen, fw, qmax, kh0, fk, zexp, pp,
+                                      col, k == a.S_s - 1, qn, zin, exf, net)) {
                 col_upd = true;
                 col_z0 = szi;
                 col_z1 = zin;
@@ -1242,7 +1244,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_ssf(Dev d, RouteArgs a, AdAr
         const double area = ld(d.area, q_);
         const double dsoil = ld(d.d, q_), ts = ld(d.ts, q_), tr = ld(d.tr, q_), tfc = ld(d.tfc, q_);
         const double sy = ld(d.sy, q_), slope = ld(d.slope, q_), flen = ld(d.fl, q_), fw = ld(d.fw, q_);
-        const double qmax = ld(d.qmax, q_), kh0 = ld(d.kh0, q_), fk = ld(d.f, q_);
+        const double qmax = ld(d.qmax, q_), kh0 = ld(d.kh0, q_), fk = ld(d.f, q_), zexp = ld(d.zexp, q_);
         const double rech = d.recharge[q_];
         Column<N> col;
         soil_layers<N>(dsoil, d.cfg_cum, d.cfg_thick, col.act, col.cum);
@@ -1262,7 +1264,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_ssf(Dev d, RouteArgs a, AdAr
             cum.tor = x.scum[4 * (size_t)n + q_];
             cum.rech = x.scum[5 * (size_t)n + q_];
         }
-        const SsfPrep pp = ssf_prepare(sq, szi, slope, sy, dts, flen, kh0, fk);
+        const SsfPrep pp = ssf_prepare(sq, szi, slope, sy, dts, flen, kh0, fk, zexp);
         bool col_upd = false;
         double col_z0 = 0.0, col_z1 = 0.0;
         auto ssf_compute = [&](int, double qin, double tor) -> double {
@@ -1273,7 +1275,8 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_ssf(Dev d, RouteArgs a, AdAr
             const double qnb = 0.0 + flx;
             cum.tor += tor * dts;
             double qn, zin, exf, net;
-            if (kinematic_wave_ssf<N>(qin, sq, szi, qnb, slope, sy, dsoil, dts, flen, fw, qmax, kh0, fk, pp, col, true,
+            if (kinematic_wave_ssf<N>(qin, sq, szi, qnb, slope, sy, dsoil, dts, flen, fw, qmax, kh0, fk, zexp, pp, col,
+                                      true,
                                       qn, zin, exf, net)) {
                 col_upd = true;
                 col_z0 = szi;
@@ -1646,7 +1649,8 @@ __global__ void __launch_bounds__(256) k_ssf_min(Dev d, AdArgs x, SelCtl* s) {
     if (i < d.n) {
         const double z = x.szi[i];
         if (z > 0.0) {
-            const double cel = (ld(d.kh0, i) * wfh_exp(-ld(d.f, i) * z) * ld(d.slope, i)) / ld(d.sy, i);
+            const double ze = ld(d.zexp, i);
+            const double cel = (ld(d.kh0, i) * wfh_exp(-ld(d.f, i) * (z < ze ? z : ze)) * ld(d.slope, i)) / ld(d.sy, i);
             key = wfh_bits(ld(d.fl, i) / cel);
             elig = true;
         }
@@ -1713,6 +1717,7 @@ struct wfh_handle {
     int32_t nsub[3]{};           // sub-steps of the last step
     int64_t nres = 0;
     std::vector<double> fftr_land;  // flow fraction to river, land index order
+    bool ext = false;               // reservoirs, lateral snow transport or an exponential-constant profile
 };
 
 namespace {
@@ -2317,8 +2322,18 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         sy[i] = wfh_max(ts - tfc[i], 0.02);
         kh0[i] = st->khfrac[i] * st->kv_0[i];
         const double fk = st->f[i];
-        qmax[i] = ((kh0[i] * slope[i]) / fk) * (1.0 - wfh_exp(-fk * d));
-        q0[i] = ((kh0[i] * slope[i]) / fk) * (wfh_exp(-fk * zi0[i]) - wfh_exp(-fk * d)) * fw[i];
+        const double ze = st->z_exp ? st->z_exp[i] : INFINITY;
+        if (ze < INFINITY) {  // initialize_lateral_ssf_model!(::KhExponentialConstant)  utils.jl:940-975
+            const double q_constant = kh0[i] * wfh_exp(-fk * ze) * slope[i] * (d - ze);
+            qmax[i] = ((kh0[i] * slope[i]) / fk) * (1.0 - wfh_exp(-fk * ze)) + q_constant;
+            if (zi0[i] < ze)
+                q0[i] = (((kh0[i] * slope[i]) / fk) * (wfh_exp(-fk * zi0[i]) - wfh_exp(-fk * ze)) + q_constant) * fw[i];
+            else
+                q0[i] = kh0[i] * wfh_exp(-fk * zi0[i]) * slope[i] * (d - zi0[i]) * fw[i];
+        } else {
+            qmax[i] = ((kh0[i] * slope[i]) / fk) * (1.0 - wfh_exp(-fk * d));
+            q0[i] = ((kh0[i] * slope[i]) / fk) * (wfh_exp(-fk * zi0[i]) - wfh_exp(-fk * d)) * fw[i];
+        }
     }
 
     // ---------------- node order, uploads ----------------
@@ -2352,6 +2367,7 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
     if ((rc = put_prm(h, wf, D.wf))) goto done;
     if ((rc = put_prm(h, sfrac, D.sfrac))) goto done;
     if ((rc = put_prm(h, V(st->theta_s), D.ts))) goto done;
+    if ((rc = put_prm(h, st->z_exp ? V(st->z_exp) : std::vector<double>(n, INFINITY), D.zexp))) goto done;
     if ((rc = put_prm(h, V(st->theta_r), D.tr))) goto done;
     if ((rc = put_prm(h, tfc, D.tfc))) goto done;
     if ((rc = put_prm(h, V(st->kv_0), D.kv0))) goto done;
@@ -2406,6 +2422,7 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         // lateral snow transport: every upstream cell (the flow graph itself), ordered by the
         // reference's topological order (the order accucapacityflux! passes material on in)
         D.snow_lat = (cfg->snow_gravitational_transport && cfg->snow) ? 1 : 0;
+        h->ext = nres > 0 || D.snow_lat || st->z_exp != nullptr;
         {
             std::vector<int> sup_ptr(n + 1, 0), sup_idx;
             if (D.snow_lat) {
@@ -2685,7 +2702,7 @@ static void launch_kernels(wfh_handle* h) {
     RouteArgs a = h->ra;
     a.parity = h->parity;
     h->parity ^= 1;
-    if (h->nres > 0 || h->d.snow_lat)
+    if (h->ext)
         hipLaunchKernelGGL((k_route<N, true>), dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, a);
     else
         hipLaunchKernelGGL((k_route<N, false>), dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, a);

@@ -61,7 +61,7 @@ STATIC_FIELDS = [
     ("soil_thickness", _D), ("infiltcap_compacted", _D), ("max_leakage", _D), ("compacted_fraction", _D),
     ("rootdistpar", _D), ("cap_hmax", _D), ("cap_n", _D), ("w_soil", _D), ("cf_soil", _D),
     ("brooks_corey_c", _D), ("kvfrac", _D), ("rootfraction", _D),
-    ("khfrac", _D),
+    ("khfrac", _D), ("z_exp", _D),
     ("river_slope", _D), ("river_manning_n", _D), ("river_bankfull_depth", _D), ("river_length", _D),
     ("river_width", _D),
     ("nres", ctypes.c_int64), ("reservoir_outlet", _I64), ("reservoir_coverage", _I32),
