@@ -302,8 +302,10 @@ int32_t wfh_set_timing(wfh_handle* h, int32_t enable);
 int32_t wfh_get_timing(wfh_handle* h, double* vertical_ms, double* routing_ms);
 /* Algorithmic HBM bytes of one launch of each kernel for this model: every input array read
  * once and every output array written once (parameter maps stored once when uniform count
- * 8 bytes), upstream discharge read once per edge and sub-step. No counterpart in the
- * reference; used for the roofline of the measurement harness. */
+ * 8 bytes), upstream discharge read once per edge and sub-step. Counts the fixed sub-steps'
+ * single routing launch (with adaptive sub-steps the routing is one launch per sub-step and
+ * this count does not apply). No counterpart in the reference; used for the roofline of the
+ * measurement harness. */
 int32_t wfh_kernel_bytes(wfh_handle* h, int64_t* vertical_bytes, int64_t* routing_bytes);
 const char* wfh_last_error(const wfh_handle* h);
 void wfh_destroy(wfh_handle* h);
