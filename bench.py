@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16, help="CPU threads of the baseline (at most the cores)")
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--reservoirs", type=int, default=0,
+                    help="place this many reservoirs on the grid's river cells (simple / modified Puls)")
+    ap.add_argument("--snow-transport", action="store_true", help="lateral snow transport (gravitational)")
     ap.add_argument("--adaptive", action="store_true",
                     help="adaptive kinematic-wave sub-steps (kinematic_wave__adaptive_time_step_flag) instead of "
                          "the fixed ones; not the headline line: no roofline, every sub-step waits for the last")
@@ -156,7 +159,12 @@ def main():
           f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     if args.adaptive:
         workload = workload.replace("(land 1, river 4, subsurface 1 sub-steps)", "(adaptive sub-steps)")
-    m = Model(c, water_mass_balance=True, adaptive=args.adaptive)
+    if args.reservoirs:
+        synth.add_reservoirs(c, args.reservoirs, seed=1, radius=2)
+        workload += f", {args.reservoirs} reservoirs"
+    if args.snow_transport:
+        workload += ", lateral snow transport"
+    m = Model(c, water_mass_balance=True, adaptive=args.adaptive, snow_transport=args.snow_transport)
     print(f"# model created ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     node = m.node_order()
     node_t = torch.as_tensor(node, device="cuda")
@@ -207,6 +215,8 @@ def main():
               "cell-timesteps/s FP64 SBM+kinwave (batched GPU step)")
     if args.adaptive:
         metric = metric.replace(" step)", " step, adaptive sub-steps)")
+    if args.reservoirs or args.snow_transport:
+        metric = metric.replace(" step", " step with reservoirs / snow transport", 1)
     result = {
         "metric": metric,
         "value": c.n * args.steps / elapsed,
@@ -255,7 +265,7 @@ def main():
         torch.cuda.synchronize()
         print(f"# calibration kernel (neg): {nb} bytes read, {nb} bytes written", file=sys.stderr, flush=True)
         del src, dst
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.reservoirs:  # basin samples carry no reservoirs
         frames_host = []
         inv = np.empty_like(node)
         inv[node] = np.arange(node.size)
