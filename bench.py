@@ -133,7 +133,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                                   "(sharding design in DESIGN.md)"}))
+                                   "(a model is one GPU's basins; see the sharding notes of the design)"}))
         sys.exit(2)
     import numpy as np
     import torch

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define WFH_ABI_VERSION 2
+#define WFH_ABI_VERSION 3
 #define WFH_MAX_LAYERS 8
 
 enum wfh_status {
@@ -136,25 +136,40 @@ typedef struct wfh_static {
     const double* river_length;
     const double* river_width;
     /* reservoirs (reservoir__flag; Wflow/src/routing/surface/reservoir.jl, network.jl:296-360),
-     * nres = 0 for none. Linear storage curve; outflow types 3 (modified Puls) and 4 (simple),
-     * or the observed outflow where given. */
+     * nres = 0 for none. Outflow types 1 (rating curve, H-Q table per julian day), 2 (free weir,
+     * without a linked lower reservoir), 3 (modified Puls) and 4 (simple), or the observed outflow
+     * where given; linear or interpolated (S-H table) storage curve (fields at the end). */
     int64_t nres;
     const int64_t* reservoir_outlet;       /* [nres] land index of each outlet, ascending; a river
                                               cell whose river flows on to a river cell   */
     const int32_t* reservoir_coverage;     /* [n] 1-based reservoir whose area holds the cell
                                               (reservoir_area__count), 0 = none; NULL = outlets */
-    const int32_t* reservoir_outflow_type; /* [nres] ReservoirOutflowType: 3 or 4            */
+    const int32_t* reservoir_outflow_type; /* [nres] ReservoirOutflowType: 1, 2, 3 or 4      */
     const double* reservoir_area;          /* [nres] [m2]                                      */
     const double* reservoir_waterlevel;    /* [nres] initial water level [m]                   */
     const double* reservoir_demand;        /* [nres] simple: [m3 s-1]                          */
     const double* reservoir_maximum_release;         /* [nres] simple: [m3 s-1]               */
-    const double* reservoir_maximum_storage;         /* [nres] simple (and observed): [m3]    */
+    const double* reservoir_maximum_storage;         /* [nres] simple (and observed): [m3];
+                                                        rating curve: derived from the tables */
     const double* reservoir_target_full_fraction;    /* [nres] simple [-]                     */
     const double* reservoir_target_minimum_fraction; /* [nres] simple [-]                     */
-    const double* reservoir_threshold;               /* [nres] modified Puls [m]              */
-    const double* reservoir_rating_curve_coefficient;/* [nres] modified Puls [-]              */
+    const double* reservoir_threshold;               /* [nres] modified Puls, free weir [m]   */
+    const double* reservoir_rating_curve_coefficient;/* [nres] modified Puls, free weir [-]   */
     const double* reservoir_external_inflow; /* [nres] or NULL (0) [m3 s-1]; negative = abstraction */
     const double* reservoir_outflow_obs;     /* [nres] or NULL (NaN = not observed) [m3 s-1]       */
+    /* Storage and rating curves (ReservoirProfileType, SH and HQ tables of reservoir.jl:5-45 and
+     * io.jl:1129-1151). Tables are in CSR form: reservoir k owns rows ptr[k] .. ptr[k+1]-1. H and S
+     * must be finite and non-decreasing within a reservoir (interpolate_linear assumes it). */
+    const int32_t* reservoir_storage_curve_type;    /* [nres] or NULL (all 1): 1 linear S = A H,
+                                                       2 interpolation of the S-H table         */
+    const double* reservoir_rating_curve_exponent;  /* [nres] free weir [-]                     */
+    const int64_t* reservoir_sh_ptr;  /* [nres+1] or NULL: S-H rows (>= 1 for storage curve 2) */
+    const double* reservoir_sh_h;     /* [rows] water level H [m]                                */
+    const double* reservoir_sh_s;     /* [rows] storage S [m3]                                   */
+    const int64_t* reservoir_hq_ptr;  /* [nres+1] or NULL: H-Q rows (>= 1 for outflow type 1)  */
+    const double* reservoir_hq_h;     /* [rows] water level H [m]                                */
+    const double* reservoir_hq_q;     /* [rows][365] outflow Q [m3 s-1], one column per julian
+                                         day (read_hq_csv's Q matrix, row-major)                */
 } wfh_static;
 
 /* Readable / writable variables (BMI get_value / set_value equivalents, bmi.jl:203-262). */
@@ -262,6 +277,13 @@ int32_t wfh_set_forcing(wfh_handle* h, const double* precipitation, const double
  * previous sub-step left (a global quantile or minimum over the cells), so the host reads each
  * one back before it launches the sub-step: the call returns when the step has completed. */
 int32_t wfh_step(wfh_handle* h);
+
+/* Julian day (1..365; February 29 counts as March 1) that selects the rating-curve column of
+ * reservoirs with outflow type 1 in the following steps; 1 after wfh_create. The reference sets
+ * it from the step's start time before the river update: julian_day(clock.time - clock.dt).
+ * Replaces: update_index_hq!  Wflow/src/routing/surface/reservoir.jl:379-386 (called from
+ * surface_kinwave.jl:629), julian_day  Wflow/src/utils.jl:674-678 */
+int32_t wfh_set_julian_day(wfh_handle* h, int32_t julian_day);
 
 /* Number of subsurface, overland and river sub-steps the last wfh_step() took (fixed or
  * adaptive). No counterpart in the reference (its loop counters are local). */

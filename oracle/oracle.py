@@ -114,6 +114,13 @@ def lib():
         L.wfo_accucapacityflux.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, _D, _D, _D, _c]
         L.wfo_update_reservoir_at_river.restype = _c
         L.wfo_update_reservoir_at_river.argtypes = [ctypes.c_void_p, _c, _c]
+        L.wfo_interpolate_linear.restype = _c
+        L.wfo_interpolate_linear.argtypes = [_c, _D, _D, ctypes.c_int64, ctypes.c_int64]
+        for f in ("wfo_reservoir_maximum_storage", "wfo_reservoir_initial_storage"):
+            getattr(L, f).restype = _c
+            getattr(L, f).argtypes = [ctypes.c_void_p]
+        L.wfo_set_julian_day.restype = ctypes.c_int32
+        L.wfo_set_julian_day.argtypes = [ctypes.c_void_p, ctypes.c_int32]
         _lib = L
     return _lib
 
@@ -137,10 +144,15 @@ class Reservoir(ctypes.Structure):
         "area", "demand", "maximum_release", "maximum_storage", "target_full_fraction", "target_minimum_fraction",
         "threshold", "rating_curve_coefficient", "outflow_obs", "external_inflow", "precipitation", "evaporation",
         "inflow_overland", "inflow_subsurface", "storage", "waterlevel", "outflow", "inflow_cumulative",
-        "outflow_cumulative", "actevap_cumulative", "abstraction_cumulative")]
+        "outflow_cumulative", "actevap_cumulative", "abstraction_cumulative")] + [
+        ("storage_curve_type", ctypes.c_int32), ("rating_curve_exponent", ctypes.c_double),
+        ("sh_n", ctypes.c_int64), ("sh_h", _D), ("sh_s", _D),
+        ("hq_n", ctypes.c_int64), ("hq_h", _D), ("hq_q", _D), ("hq_col", ctypes.c_int32)]
 
     @classmethod
-    def make(cls, **kw):
+    def make(cls, sh=None, hq=None, julian_day=1, **kw):
+        """sh = (H, S) storage curve table (sets storage curve type 2); hq = (H, Q[rows][365])
+        rating curve table. The tables are kept alive by the returned object."""
         r = cls()
         for f, t in cls._fields_:
             if t is ctypes.c_double:
@@ -148,9 +160,39 @@ class Reservoir(ctypes.Structure):
         for f in ("external_inflow", "inflow_overland", "inflow_subsurface", "inflow_cumulative",
                   "outflow_cumulative", "actevap_cumulative", "abstraction_cumulative"):
             setattr(r, f, 0.0)
+        r.storage_curve_type = 1
+        r._keep = []
+        if sh is not None:
+            h, s_ = (np.ascontiguousarray(a, np.float64) for a in sh)
+            r._keep += [h, s_]
+            r.sh_n, r.sh_h, r.sh_s = h.size, h.ctypes.data_as(_D), s_.ctypes.data_as(_D)
+            r.storage_curve_type = 2
+        if hq is not None:
+            h = np.ascontiguousarray(hq[0], np.float64)
+            q = np.ascontiguousarray(hq[1], np.float64).reshape(h.size, 365)
+            r._keep += [h, q]
+            r.hq_n, r.hq_h, r.hq_q = h.size, h.ctypes.data_as(_D), q.ctypes.data_as(_D)
+        r.hq_col = julian_day - 1
         for k, v in kw.items():
             setattr(r, k, float("nan") if v is None else v)
         return r
+
+
+def interpolate_linear(x, xp, fp):
+    """interpolate_linear(x, xp, fp)  reservoir.jl:364-377."""
+    xp = np.ascontiguousarray(xp, np.float64)
+    fp = np.ascontiguousarray(fp, np.float64)
+    return lib().wfo_interpolate_linear(float(x), xp.ctypes.data_as(_D), fp.ctypes.data_as(_D), xp.size, 1)
+
+
+def reservoir_maximum_storage(res):
+    """get_maximum_storage  reservoir.jl:324-340."""
+    return lib().wfo_reservoir_maximum_storage(ctypes.byref(res))
+
+
+def reservoir_initial_storage(res):
+    """initialize_storage  reservoir.jl:342-362."""
+    return lib().wfo_reservoir_initial_storage(ctypes.byref(res))
 
 
 def update_reservoir(res, inflow, dt):
@@ -349,6 +391,11 @@ class OracleModel:
 
     def kinwave_iterations(self):
         return lib().wfo_kinwave_iterations(self.h)
+
+    def set_julian_day(self, day):
+        """update_index_hq!  reservoir.jl:379-386."""
+        if lib().wfo_set_julian_day(self.h, int(day)) != 0:
+            raise ValueError(f"julian day {day} outside 1..365")
 
     def last_substeps(self):
         """(subsurface, overland, river) sub-step counts of the last step."""

@@ -576,6 +576,73 @@ static void reservoir_modified_puls(const wfo_reservoir* r, double inflow, doubl
     *storage_out = (si_factor - outflow) * dt;
 }
 
+/* interpolate_linear  reservoir.jl:364-377, literally: the ends of the table give minimum(fp) /
+ * maximum(fp); otherwise i1 is the last index with xp[i1] <= x (findall), i2 = i1 + 1. A NaN x
+ * (no such index: the reference raises) gives NaN. */
+double wfo_interpolate_linear(double x, const double* xp, const double* fp, int64_t n, int64_t fs) {
+    double xmin = xp[0], xmax = xp[0], fmin = fp[0], fmax = fp[0];
+    for (int64_t i = 1; i < n; ++i) {
+        xmin = wfh_min(xmin, xp[i]);
+        xmax = wfh_max(xmax, xp[i]);
+        fmin = wfh_min(fmin, fp[i * fs]);
+        fmax = wfh_max(fmax, fp[i * fs]);
+    }
+    if (x <= xmin) return fmin;
+    if (x >= xmax) return fmax;
+    int64_t i1 = -1;
+    for (int64_t i = 0; i < n; ++i)
+        if (xp[i] <= x) i1 = i;
+    if (i1 < 0 || i1 + 1 >= n) return NaN;
+    const int64_t i2 = i1 + 1;
+    const double f1 = fp[i1 * fs], f2 = fp[i2 * fs];
+    return f1 * (1.0 - (x - xp[i1]) / (xp[i2] - xp[i1])) + f2 * (x - xp[i1]) / (xp[i2] - xp[i1]);
+}
+
+double wfo_reservoir_maximum_storage(const wfo_reservoir* r) {
+    double hmax = r->hq_h[0];
+    for (int64_t i = 1; i < r->hq_n; ++i) hmax = wfh_max(hmax, r->hq_h[i]);
+    if (r->storage_curve_type == 2) return wfo_interpolate_linear(hmax, r->sh_h, r->sh_s, r->sh_n, 1);
+    return r->area * hmax;
+}
+
+double wfo_reservoir_initial_storage(const wfo_reservoir* r) {
+    if (r->storage_curve_type == 2) return wfo_interpolate_linear(r->waterlevel, r->sh_h, r->sh_s, r->sh_n, 1);
+    return r->area * r->waterlevel;
+}
+
+/* update_reservoir_hq  reservoir.jl:457-483: the outflow of the rating curve column of the day at
+ * the current water level, then the overflow above the maximum storage */
+static void reservoir_hq(const wfo_reservoir* r, double inflow, double precipitation, double evaporation,
+                         double dt, double* outflow_out, double* storage_out) {
+    const double storage_input = r->storage / dt + precipitation - evaporation + inflow;
+    double outflow = wfo_interpolate_linear(r->waterlevel, r->hq_h, r->hq_q + r->hq_col, r->hq_n, 365);
+    outflow = wfh_min(outflow, storage_input);
+    double storage = (storage_input - outflow) * dt;
+    const double overflow = wfh_max(0.0, (storage - r->maximum_storage) / dt);
+    storage = wfh_min(storage, r->maximum_storage);
+    outflow += overflow;
+    *outflow_out = outflow;
+    *storage_out = storage;
+}
+
+/* update_reservoir_free_weir  reservoir.jl:485-553 for a reservoir without a linked lower
+ * reservoir (lower_reservoir_ind = 0, so diff_wl = 0) */
+static void reservoir_free_weir(const wfo_reservoir* r, double inflow, double precipitation, double evaporation,
+                                double dt, double* outflow_out, double* storage_out) {
+    const double storage_input = wfh_max(r->storage / dt + precipitation - evaporation + inflow, 0.0);
+    double outflow;
+    if (r->waterlevel > r->threshold) {
+        const double dh = r->waterlevel - r->threshold;
+        outflow = r->rating_curve_coefficient * wfh_pow(dh, r->rating_curve_exponent);
+        const double maxflow = dh * r->area / dt;
+        outflow = wfh_min(outflow, maxflow);
+    } else {
+        outflow = 0.0;
+    }
+    *outflow_out = outflow;
+    *storage_out = (storage_input - outflow) * dt;
+}
+
 /* update_reservoir_outflow_obs  reservoir.jl:555-575 */
 static void reservoir_outflow_obs(const wfo_reservoir* r, double inflow, double precipitation, double evaporation,
                                   double dt, double* outflow_out, double* storage_out) {
@@ -599,11 +666,17 @@ void wfo_update_reservoir(wfo_reservoir* r, double inflow, double dt) {
     double outflow, storage;
     if (!wfh_isnan(r->outflow_obs))
         reservoir_outflow_obs(r, inflow, precipitation, evaporation, dt, &outflow, &storage);
+    else if (r->outflow_type == 1)
+        reservoir_hq(r, inflow, precipitation, evaporation, dt, &outflow, &storage);
+    else if (r->outflow_type == 2)
+        reservoir_free_weir(r, inflow, precipitation, evaporation, dt, &outflow, &storage);
     else if (r->outflow_type == 3)
         reservoir_modified_puls(r, inflow, precipitation, evaporation, dt, &outflow, &storage);
     else
         reservoir_simple(r, inflow, precipitation, evaporation, dt, &outflow, &storage);
-    double waterlevel = r->waterlevel + (storage - r->storage) / r->area;
+    double waterlevel = r->storage_curve_type == 2
+                            ? wfo_interpolate_linear(storage, r->sh_s, r->sh_h, r->sh_n, 1)
+                            : r->waterlevel + (storage - r->storage) / r->area;
     r->storage = storage;
     r->waterlevel = waterlevel;
     r->outflow = outflow;
@@ -940,6 +1013,7 @@ struct wfo_model {
     int64_t *res_land, *res_river, *land_res, *river_res, *cov_ptr, *cov_idx;
     uint8_t* land_cov;
     double *res_store_prev, *res_outflow_avg, *res_inflow_avg, *res_actevap_avg, *res_abs_avg, *bal_res;
+    double* res_tab; /* copies of the S-H and H-Q tables the reservoirs point into */
 };
 
 static double* dalloc(int64_t n) { return (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
@@ -1182,14 +1256,52 @@ wfo_model* wfo_create(const wfh_config* cfg, const wfh_static* st, char* err, in
             m->land_res[i] = k;
             m->river_res[m->land_river[i]] = k;
             const int32_t t = st->reservoir_outflow_type[k];
-            if (t != 3 && t != 4) {
-                snprintf(err, (size_t)errlen, "reservoir %lld: outflow type %d not supported", (long long)k, t);
+            const int32_t sct = st->reservoir_storage_curve_type ? st->reservoir_storage_curve_type[k] : 1;
+            if (t < 1 || t > 4 || (sct != 1 && sct != 2)) {
+                snprintf(err, (size_t)errlen, "reservoir %lld: outflow type %d / storage curve type %d not supported",
+                         (long long)k, t, sct);
                 free(taken);
                 goto fail;
             }
             wfo_reservoir* R = &m->res[k];
             R->outflow_type = t;
+            R->storage_curve_type = sct;
             R->area = st->reservoir_area[k];
+            /* S-H and H-Q tables (read_sh_csv / read_hq_csv  io.jl:1129-1147) */
+            if (sct == 2 || t == 1) {
+                const int64_t* ptr = sct == 2 ? st->reservoir_sh_ptr : st->reservoir_hq_ptr;
+                int ok = ptr != NULL;
+                if (sct == 2) {
+                    ok = ok && st->reservoir_sh_h && st->reservoir_sh_s && ptr[k + 1] > ptr[k];
+                    if (ok) {
+                        R->sh_n = ptr[k + 1] - ptr[k];
+                        R->sh_h = st->reservoir_sh_h + ptr[k];
+                        R->sh_s = st->reservoir_sh_s + ptr[k];
+                        for (int64_t j = 0; j < R->sh_n; ++j)
+                            ok = ok && isfinite(R->sh_h[j]) && isfinite(R->sh_s[j]) &&
+                                 (j == 0 || (R->sh_h[j] >= R->sh_h[j - 1] && R->sh_s[j] >= R->sh_s[j - 1]));
+                    }
+                }
+                if (t == 1) {
+                    ptr = st->reservoir_hq_ptr;
+                    ok = ok && ptr && st->reservoir_hq_h && st->reservoir_hq_q && ptr[k + 1] > ptr[k];
+                    if (ok) {
+                        R->hq_n = ptr[k + 1] - ptr[k];
+                        R->hq_h = st->reservoir_hq_h + ptr[k];
+                        R->hq_q = st->reservoir_hq_q + ptr[k] * 365;
+                        for (int64_t j = 0; j < R->hq_n; ++j) {
+                            ok = ok && isfinite(R->hq_h[j]) && (j == 0 || R->hq_h[j] >= R->hq_h[j - 1]);
+                            for (int c = 0; c < 365; ++c) ok = ok && isfinite(R->hq_q[j * 365 + c]);
+                        }
+                    }
+                }
+                if (!ok) {
+                    snprintf(err, (size_t)errlen, "reservoir %lld: missing, empty, non-finite or decreasing S-H / "
+                             "H-Q table", (long long)k);
+                    free(taken);
+                    goto fail;
+                }
+            }
 #define RES_PRM(field, name, dflt) R->field = st->name ? st->name[k] : (dflt)
             RES_PRM(demand, reservoir_demand, NaN);
             RES_PRM(maximum_release, reservoir_maximum_release, NaN);
@@ -1200,12 +1312,40 @@ wfo_model* wfo_create(const wfh_config* cfg, const wfh_static* st, char* err, in
             RES_PRM(rating_curve_coefficient, reservoir_rating_curve_coefficient, NaN);
             RES_PRM(external_inflow, reservoir_external_inflow, 0.0);
             RES_PRM(outflow_obs, reservoir_outflow_obs, NaN);
+            RES_PRM(rating_curve_exponent, reservoir_rating_curve_exponent, NaN);
 #undef RES_PRM
             R->waterlevel = st->reservoir_waterlevel[k];
-            R->storage = R->area * R->waterlevel; /* initialize_storage, linear  reservoir.jl:343-362 */
+            R->storage = wfo_reservoir_initial_storage(R);
+            if (t == 1) R->maximum_storage = wfo_reservoir_maximum_storage(R); /* reservoir.jl:170-175 */
+            R->sh_h = R->sh_s = R->hq_h = R->hq_q = NULL; /* re-pointed into res_tab below */
+            R->hq_col = 0;
             R->outflow = NaN;
         }
         free(taken);
+        {
+            int64_t tot = 0;
+            for (int64_t k = 0; k < nr; ++k) tot += 2 * m->res[k].sh_n + 366 * m->res[k].hq_n;
+            m->res_tab = dalloc(tot > 0 ? tot : 1);
+            double* w = m->res_tab;
+            for (int64_t k = 0; k < nr; ++k) {
+                wfo_reservoir* R = &m->res[k];
+                const int64_t s0 = R->sh_n ? st->reservoir_sh_ptr[k] : 0, h0 = R->hq_n ? st->reservoir_hq_ptr[k] : 0;
+                if (R->sh_n) {
+                    memcpy(w, st->reservoir_sh_h + s0, sizeof(double) * (size_t)R->sh_n);
+                    memcpy(w + R->sh_n, st->reservoir_sh_s + s0, sizeof(double) * (size_t)R->sh_n);
+                    R->sh_h = w;
+                    R->sh_s = w + R->sh_n;
+                    w += 2 * R->sh_n;
+                }
+                if (R->hq_n) {
+                    memcpy(w, st->reservoir_hq_h + h0, sizeof(double) * (size_t)R->hq_n);
+                    memcpy(w + R->hq_n, st->reservoir_hq_q + h0 * 365, sizeof(double) * (size_t)(365 * R->hq_n));
+                    R->hq_h = w;
+                    R->hq_q = w + R->hq_n;
+                    w += 366 * R->hq_n;
+                }
+            }
+        }
         /* coverage lists in ascending land index (findall order of reservoir_area__count) */
         m->cov_ptr = ialloc(nr + 1);
         for (int64_t i = 0; i < n; ++i) {
@@ -1418,7 +1558,7 @@ void wfo_destroy(wfo_model* m) {
         (void**)&m->res_land, (void**)&m->res_river, (void**)&m->land_res, (void**)&m->river_res,
         (void**)&m->cov_ptr, (void**)&m->cov_idx, (void**)&m->land_cov, (void**)&m->res_store_prev,
         (void**)&m->res_outflow_avg, (void**)&m->res_inflow_avg, (void**)&m->res_actevap_avg, (void**)&m->res_abs_avg,
-        (void**)&m->bal_res, (void**)&m->snow_in, (void**)&m->snow_out, (void**)&m->snow_flux_w,
+        (void**)&m->bal_res, (void**)&m->res_tab, (void**)&m->snow_in, (void**)&m->snow_out, (void**)&m->snow_flux_w,
         (void**)&m->snow_cap_s, (void**)&m->snow_cap_w};
     for (size_t k = 0; k < sizeof p / sizeof p[0]; ++k) free(*p[k]);
     free(m);
@@ -1839,6 +1979,12 @@ int64_t wfo_n(const wfo_model* m) { return m->n; }
 int64_t wfo_nriv(const wfo_model* m) { return m->nriv; }
 int32_t wfo_nlayers(const wfo_model* m) { return m->N; }
 int64_t wfo_kinwave_iterations(const wfo_model* m) { return m->kw_iters; }
+int32_t wfo_set_julian_day(wfo_model* m, int32_t julian_day) {
+    if (julian_day < 1 || julian_day > 365) return -1;
+    for (int64_t k = 0; k < m->nres; ++k) m->res[k].hq_col = julian_day - 1; /* update_index_hq! */
+    return 0;
+}
+
 void wfo_last_substeps(const wfo_model* m, int32_t* out3) {
     for (int k = 0; k < 3; ++k) out3[k] = m->nsub[k];
 }

@@ -44,16 +44,30 @@ typedef struct wfo_soil {
 } wfo_soil;
 
 /* One reservoir (ReservoirParameters / ReservoirVariables / ReservoirBC of
- * Wflow/src/routing/surface/reservoir.jl, linear storage curve). */
+ * Wflow/src/routing/surface/reservoir.jl). */
 typedef struct wfo_reservoir {
-    int32_t outflow_type; /* 3 modified Puls, 4 simple */
+    int32_t outflow_type; /* 1 rating curve, 2 free weir, 3 modified Puls, 4 simple */
     double area, demand, maximum_release, maximum_storage, target_full_fraction, target_minimum_fraction;
     double threshold, rating_curve_coefficient, outflow_obs, external_inflow;
     double precipitation, evaporation;           /* [m s-1] over the reservoir area */
     double inflow_overland, inflow_subsurface;   /* [m3 s-1] */
     double storage, waterlevel, outflow;
     double inflow_cumulative, outflow_cumulative, actevap_cumulative, abstraction_cumulative;
+    int32_t storage_curve_type;     /* 1 linear S = A H, 2 interpolation of the S-H table */
+    double rating_curve_exponent;   /* free weir */
+    int64_t sh_n;                   /* S-H table rows (storage curve 2) */
+    const double *sh_h, *sh_s;
+    int64_t hq_n;                   /* H-Q table rows (outflow type 1) */
+    const double *hq_h, *hq_q;      /* hq_q: [hq_n][365], one column per julian day */
+    int32_t hq_col;                 /* col_index_hq - 1 */
 } wfo_reservoir;
+
+/* interpolate_linear(x, xp, fp)  reservoir.jl:364-377; fp read with stride fs */
+double wfo_interpolate_linear(double x, const double* xp, const double* fp, int64_t n, int64_t fs);
+/* get_maximum_storage  reservoir.jl:324-340 (a reservoir with a rating curve) */
+double wfo_reservoir_maximum_storage(const wfo_reservoir* r);
+/* initialize_storage  reservoir.jl:342-362 for r->waterlevel */
+double wfo_reservoir_initial_storage(const wfo_reservoir* r);
 
 /* update_reservoir_model!(reservoir_model, i, inflow, dt)  reservoir.jl:583-635 */
 void wfo_update_reservoir(wfo_reservoir* r, double inflow, double dt);
@@ -80,6 +94,8 @@ int32_t wfo_topology(const wfo_model* m, int64_t* land_order, int64_t* land_down
 int64_t wfo_kinwave_iterations(const wfo_model* m);
 /* subsurface, overland and river sub-step counts of the last wfo_step */
 void wfo_last_substeps(const wfo_model* m, int32_t* out3);
+/* update_index_hq!  reservoir.jl:379-386: the rating-curve column (julian day 1..365); 0 or -1 */
+int32_t wfo_set_julian_day(wfo_model* m, int32_t julian_day);
 /* Statistics.quantile!(v, p) of k >= 2 values (sorts v); the kinematic-wave stable time step
  * (surface_kinwave.jl:674-704) over n cells, buf holding n values. */
 double wfo_quantile(double* v, int64_t k, double p);

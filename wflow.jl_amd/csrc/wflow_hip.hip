@@ -112,8 +112,17 @@ struct Dev {
     const int* res_of_r;       // [nriv] reservoir whose outlet is the river cell, or -1
     const double* rup_res;     // [Er] 1 when the edge's upstream river cell is a reservoir outlet
     const int *res_cov_ptr, *res_cov_idx;  // nodes of each reservoir's area, ascending land index
-    const int* res_type;       // [nres] 3 modified Puls, 4 simple
+    const int* res_type;       // [nres] 1 rating curve, 2 free weir, 3 modified Puls, 4 simple
+    const int* res_sct;        // [nres] storage curve: 1 linear, 2 interpolation of the S-H table
+    const int *res_sh_ptr, *res_hq_ptr;  // [nres+1] rows of each reservoir's S-H / H-Q table
+    const double *res_sh_h, *res_sh_s;   // S-H rows
+    const double *res_hq_h, *res_hq_q;   // H-Q rows; q is [rows][365], one column per julian day
+    const double* res_shx;     // [nres][4] minimum / maximum of the S-H table's S and H
+    const double* res_hq_ext;  // [nres][732]: minimum / maximum of H, then of each day's Q column
+                               // (rating-curve reservoirs only)
+    int res_hq_col;            // julian day - 1
     const double *res_area, *res_demand, *res_maxrel, *res_maxsto, *res_tff, *res_tmf, *res_thr, *res_rcc;
+    const double* res_rce;     // [nres] free weir rating curve exponent
     const double *res_ext, *res_obs;
     double *res_storage, *res_wl, *res_outflow, *res_P, *res_E;
     double* res_cum;           // [4][nres] sums of the step: inflow, outflow, actual evaporation, abstraction
@@ -123,6 +132,8 @@ struct Dev {
     const int *sup_ptr, *sup_idx;  // [n+1], [E]: all upstream cells, in the reference's topological order
     double *snow_in, *snow_out;    // [n] (zeros without transport)
 };
+// Dev is passed by value to every kernel: it must fit the kernel argument space
+static_assert(sizeof(Dev) + 512 <= 4096, "Dev too large for a kernel argument");
 
 struct RouteArgs {
     uint64_t* slot_ssf;  // [S_s][n]
@@ -713,10 +724,32 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
     return true;
 }
 
+// interpolate_linear(x, xp, fp)  reservoir.jl:364-377 for a table of n rows whose xp is
+// non-decreasing (checked at create). The minimum and maximum of xp and fp come precomputed; the
+// last row with xp <= x is found by bisection (xp[lo] <= x < xp[hi] throughout), which for a
+// non-decreasing xp is the row the reference's findall takes. The interpolation keeps the
+// reference's operation order.
+__device__ __forceinline__ double interp_table(const double x, const double* xp, const double* fp, const int fs,
+                                               const int n, const double xmin, const double xmax,
+                                               const double fmin, const double fmax) {
+    if (wfh_isnan(x)) return x;  // (the reference raises: findall finds no row)
+    if (x <= xmin) return fmin;
+    if (x >= xmax) return fmax;
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (xp[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    const double x1 = xp[lo], x2 = xp[lo + 1], f1 = fp[lo * fs], f2 = fp[(lo + 1) * fs];
+    return f1 * (1.0 - (x - x1) / (x2 - x1)) + f2 * (x - x1) / (x2 - x1);
+}
+
 // update_reservoir_model! at the river cell that is reservoir ri's outlet (node p), for its
 // river discharge q_v of a sub-step of length dts: external inflow / abstraction
-// (surface_kinwave.jl:441-489), the outflow of the simple, modified Puls or observed-outflow
-// update and the linear storage curve (reservoir.jl:389-456, 555-635). The reservoir's state
+// (surface_kinwave.jl:441-489), the outflow of the rating-curve, free-weir (no linked lower
+// reservoir), modified Puls, simple or observed-outflow update and the linear or interpolated
+// storage curve (reservoir.jl:389-635). The reservoir's state
 // lives in memory and only this lane touches it. first / last: the step's first / last river
 // sub-step (set_reservoir_vars!, average_reservoir_vars!, the reservoir balance). Returns the
 // outflow, which becomes the downstream river cell's inflow.
@@ -755,6 +788,28 @@ __device__ __forceinline__ double reservoir_step(const Dev& d, const int ri, con
             storage -= overflow * dts;
             outflow += overflow;
         }
+    } else if (d.res_type[ri] == 1) {  // update_reservoir_hq  reservoir.jl:457-483
+        const double storage_input = storage0 / dts + precipitation - evaporation + net_inflow;
+        const int r0 = d.res_hq_ptr[ri], col = d.res_hq_col;
+        const double* ext = d.res_hq_ext + (size_t)ri * 732;
+        outflow = interp_table(d.res_wl[ri], d.res_hq_h + r0, d.res_hq_q + (size_t)r0 * 365 + col, 365,
+                               d.res_hq_ptr[ri + 1] - r0, ext[0], ext[1], ext[2 + 2 * col], ext[3 + 2 * col]);
+        outflow = wfh_min(outflow, storage_input);
+        storage = (storage_input - outflow) * dts;
+        const double overflow = wfh_max(0.0, (storage - maxsto) / dts);
+        storage = wfh_min(storage, maxsto);
+        outflow += overflow;
+    } else if (d.res_type[ri] == 2) {  // update_reservoir_free_weir  reservoir.jl:485-553, diff_wl = 0
+        const double storage_input = wfh_max(storage0 / dts + precipitation - evaporation + net_inflow, 0.0);
+        const double wl0 = d.res_wl[ri], thr = d.res_thr[ri];
+        outflow = 0.0;
+        if (wl0 > thr) {
+            const double dh = wl0 - thr;
+            outflow = d.res_rcc[ri] * wfh_pow(dh, d.res_rce[ri]);
+            const double maxflow = dh * area / dts;
+            outflow = wfh_min(outflow, maxflow);
+        }
+        storage = (storage_input - outflow) * dts;
     } else if (d.res_type[ri] == 3) {  // update_reservoir_modified_puls  reservoir.jl:427-456
         const double res_factor = area / (dts * sqrt(d.res_rcc[ri]));
         const double si_factor = storage0 / dts + precipitation - evaporation + net_inflow;
@@ -779,7 +834,14 @@ __device__ __forceinline__ double reservoir_step(const Dev& d, const int ri, con
         storage -= release_realized * dts;
         outflow = release_realized + demand_release;
     }
-    d.res_wl[ri] = d.res_wl[ri] + (storage - storage0) / area;
+    if (d.res_sct[ri] == 2) {  // interpolate_linear(storage, S, H)  reservoir.jl:613-621
+        const int r0 = d.res_sh_ptr[ri];
+        const double* ext = d.res_shx + (size_t)ri * 4;
+        d.res_wl[ri] = interp_table(storage, d.res_sh_s + r0, d.res_sh_h + r0, 1, d.res_sh_ptr[ri + 1] - r0, ext[0],
+                                    ext[1], ext[2], ext[3]);
+    } else {
+        d.res_wl[ri] = d.res_wl[ri] + (storage - storage0) / area;
+    }
     d.res_storage[ri] = storage;
     d.res_outflow[ri] = outflow;
     cin += net_inflow * dts;
@@ -1955,6 +2017,26 @@ static void wfh_destroy_keep_error(wfh_handle* h, wfh_handle** out) {
 template <int N>
 static void launch_kernels(wfh_handle* h);
 
+// interpolate_linear(x, xp, fp)  reservoir.jl:364-377 as the reference writes it, for the
+// storage curve at create (initialize_storage, get_maximum_storage); fp read with stride fs.
+static double interp_host(double x, const double* xp, const double* fp, int64_t n, int64_t fs) {
+    double xmin = xp[0], xmax = xp[0], fmin = fp[0], fmax = fp[0];
+    for (int64_t i = 1; i < n; ++i) {
+        xmin = wfh_min(xmin, xp[i]);
+        xmax = wfh_max(xmax, xp[i]);
+        fmin = wfh_min(fmin, fp[i * fs]);
+        fmax = wfh_max(fmax, fp[i * fs]);
+    }
+    if (x <= xmin) return fmin;
+    if (x >= xmax) return fmax;
+    int64_t i1 = -1;
+    for (int64_t i = 0; i < n; ++i)
+        if (xp[i] <= x) i1 = i;
+    if (i1 < 0 || i1 + 1 >= n) return std::nan("");
+    const int64_t i2 = i1 + 1;
+    return fp[i1 * fs] * (1.0 - (x - xp[i1]) / (xp[i2] - xp[i1])) + fp[i2 * fs] * (x - xp[i1]) / (xp[i2] - xp[i1]);
+}
+
 extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_handle** out) {
     if (!out) return WFH_ERR_ARG;
     *out = nullptr;
@@ -2059,10 +2141,36 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
             const bool ok_i = i >= 0 && i < n && (k == 0 || i > st->reservoir_outlet[k - 1]) && land_river[i] >= 0;
             const int64_t rd = ok_i ? h->river_down[land_river[i]] : -1;
             const int32_t t = st->reservoir_outflow_type[k];
-            if (!ok_i || rd < 0 || fed[rd] || (t != 3 && t != 4)) {
+            const int32_t sct = st->reservoir_storage_curve_type ? st->reservoir_storage_curve_type[k] : 1;
+            if (!ok_i || rd < 0 || fed[rd] || t < 1 || t > 4 || (sct != 1 && sct != 2)) {
                 h->err = "reservoir " + std::to_string(k) +
                          ": outlet must be a river cell with a river cell downstream (outlets ascending, one "
-                         "reservoir per downstream cell) and outflow type 3 or 4";
+                         "reservoir per downstream cell), outflow type 1..4 and storage curve type 1 or 2";
+                wfh_destroy_keep_error(h, out);
+                return WFH_ERR_ARG;
+            }
+            // S-H and H-Q tables: present, non-empty, finite, H and S non-decreasing
+            bool tab_ok = true;
+            if (sct == 2) {
+                const int64_t* sp = st->reservoir_sh_ptr;
+                tab_ok = sp && st->reservoir_sh_h && st->reservoir_sh_s && sp[k + 1] > sp[k];
+                for (int64_t j = tab_ok ? sp[k] : 0; tab_ok && j < sp[k + 1]; ++j)
+                    tab_ok = std::isfinite(st->reservoir_sh_h[j]) && std::isfinite(st->reservoir_sh_s[j]) &&
+                             (j == sp[k] || (st->reservoir_sh_h[j] >= st->reservoir_sh_h[j - 1] &&
+                                             st->reservoir_sh_s[j] >= st->reservoir_sh_s[j - 1]));
+            }
+            if (t == 1) {
+                const int64_t* hp = st->reservoir_hq_ptr;
+                tab_ok = tab_ok && hp && st->reservoir_hq_h && st->reservoir_hq_q && hp[k + 1] > hp[k];
+                for (int64_t j = tab_ok ? hp[k] : 0; tab_ok && j < hp[k + 1]; ++j) {
+                    tab_ok = std::isfinite(st->reservoir_hq_h[j]) &&
+                             (j == hp[k] || st->reservoir_hq_h[j] >= st->reservoir_hq_h[j - 1]);
+                    for (int c = 0; tab_ok && c < 365; ++c) tab_ok = std::isfinite(st->reservoir_hq_q[j * 365 + c]);
+                }
+            }
+            if (!tab_ok) {
+                h->err = "reservoir " + std::to_string(k) +
+                         ": missing, empty, non-finite or decreasing S-H / H-Q table";
                 wfh_destroy_keep_error(h, out);
                 return WFH_ERR_ARG;
             }
@@ -2511,8 +2619,73 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
                     }
             }
             std::vector<double> area_r = res_vec(st->reservoir_area, 1.0), wl = res_vec(st->reservoir_waterlevel, 0.0);
-            std::vector<double> sto(nr, 0.0);
-            for (int64_t k = 0; k < nres; ++k) sto[k] = area_r[k] * wl[k];
+            std::vector<double> sto(nr, 0.0), maxsto = res_vec(st->reservoir_maximum_storage, nan);
+            // storage curves and rating curves (validated above), their extremes, the initial
+            // storage (initialize_storage  reservoir.jl:342-362) and the maximum storage of a
+            // rating-curve reservoir (get_maximum_storage  :324-340, set at :175)
+            std::vector<int> sct(nr, 1), sh_ptr(nres + 1, 0), hq_ptr(nres + 1, 0);
+            std::vector<double> sh_h, sh_s, hq_h, hq_q, shx(4 * nr, 0.0), hq_ext(732 * nr, 0.0);
+            for (int64_t k = 0; k < nres; ++k) {
+                sct[k] = st->reservoir_storage_curve_type ? st->reservoir_storage_curve_type[k] : 1;
+                if (sct[k] == 2) {
+                    const int64_t a = st->reservoir_sh_ptr[k], b = st->reservoir_sh_ptr[k + 1];
+                    sh_h.insert(sh_h.end(), st->reservoir_sh_h + a, st->reservoir_sh_h + b);
+                    sh_s.insert(sh_s.end(), st->reservoir_sh_s + a, st->reservoir_sh_s + b);
+                    double* e = &shx[4 * k];
+                    e[0] = e[1] = st->reservoir_sh_s[a];
+                    e[2] = e[3] = st->reservoir_sh_h[a];
+                    for (int64_t j = a + 1; j < b; ++j) {
+                        e[0] = wfh_min(e[0], st->reservoir_sh_s[j]);
+                        e[1] = wfh_max(e[1], st->reservoir_sh_s[j]);
+                        e[2] = wfh_min(e[2], st->reservoir_sh_h[j]);
+                        e[3] = wfh_max(e[3], st->reservoir_sh_h[j]);
+                    }
+                    sto[k] = interp_host(wl[k], st->reservoir_sh_h + a, st->reservoir_sh_s + a, b - a, 1);
+                } else {
+                    sto[k] = area_r[k] * wl[k];
+                }
+                sh_ptr[k + 1] = (int)sh_h.size();
+                if (rtype[k] == 1) {
+                    const int64_t a = st->reservoir_hq_ptr[k], b = st->reservoir_hq_ptr[k + 1];
+                    hq_h.insert(hq_h.end(), st->reservoir_hq_h + a, st->reservoir_hq_h + b);
+                    hq_q.insert(hq_q.end(), st->reservoir_hq_q + a * 365, st->reservoir_hq_q + b * 365);
+                    double* e = &hq_ext[732 * k];
+                    e[0] = e[1] = st->reservoir_hq_h[a];
+                    for (int64_t j = a + 1; j < b; ++j) {
+                        e[0] = wfh_min(e[0], st->reservoir_hq_h[j]);
+                        e[1] = wfh_max(e[1], st->reservoir_hq_h[j]);
+                    }
+                    for (int c = 0; c < 365; ++c) {
+                        double lo = st->reservoir_hq_q[a * 365 + c], hi = lo;
+                        for (int64_t j = a + 1; j < b; ++j) {
+                            lo = wfh_min(lo, st->reservoir_hq_q[j * 365 + c]);
+                            hi = wfh_max(hi, st->reservoir_hq_q[j * 365 + c]);
+                        }
+                        e[2 + 2 * c] = lo;
+                        e[3 + 2 * c] = hi;
+                    }
+                    maxsto[k] = sct[k] == 2 ? interp_host(e[1], st->reservoir_sh_h + st->reservoir_sh_ptr[k],
+                                                          st->reservoir_sh_s + st->reservoir_sh_ptr[k],
+                                                          st->reservoir_sh_ptr[k + 1] - st->reservoir_sh_ptr[k], 1)
+                                            : area_r[k] * e[1];
+                }
+                hq_ptr[k + 1] = (int)hq_h.size();
+            }
+            if (sh_h.size() > (size_t)INT32_MAX / 2 || hq_h.size() > (size_t)INT32_MAX / 365) {
+                rc = fail(h, WFH_ERR_ARG, "reservoir tables too large");
+                goto done;
+            }
+            D.res_hq_col = 0;
+            if ((rc = put_raw<int>(h, sct, D.res_sct))) goto done;
+            if ((rc = put_raw<int>(h, sh_ptr, D.res_sh_ptr))) goto done;
+            if ((rc = put_raw<int>(h, hq_ptr, D.res_hq_ptr))) goto done;
+            if ((rc = put_raw<double>(h, sh_h, D.res_sh_h))) goto done;
+            if ((rc = put_raw<double>(h, sh_s, D.res_sh_s))) goto done;
+            if ((rc = put_raw<double>(h, hq_h, D.res_hq_h))) goto done;
+            if ((rc = put_raw<double>(h, hq_q, D.res_hq_q))) goto done;
+            if ((rc = put_raw<double>(h, shx, D.res_shx))) goto done;
+            if ((rc = put_raw<double>(h, hq_ext, D.res_hq_ext))) goto done;
+            if ((rc = put_raw<double>(h, res_vec(st->reservoir_rating_curve_exponent, nan), D.res_rce))) goto done;
             std::vector<double> zeros(nr, 0.0), zeros4(4 * nr, 0.0), nans(nr, nan);
             if ((rc = put_raw<int>(h, rtype, D.res_type))) goto done;
             if ((rc = put_raw<int>(h, cov_ptr, D.res_cov_ptr))) goto done;
@@ -2520,7 +2693,7 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
             if ((rc = put_raw<double>(h, area_r, D.res_area))) goto done;
             if ((rc = put_raw<double>(h, res_vec(st->reservoir_demand, nan), D.res_demand))) goto done;
             if ((rc = put_raw<double>(h, res_vec(st->reservoir_maximum_release, nan), D.res_maxrel))) goto done;
-            if ((rc = put_raw<double>(h, res_vec(st->reservoir_maximum_storage, nan), D.res_maxsto))) goto done;
+            if ((rc = put_raw<double>(h, maxsto, D.res_maxsto))) goto done;
             if ((rc = put_raw<double>(h, res_vec(st->reservoir_target_full_fraction, nan), D.res_tff))) goto done;
             if ((rc = put_raw<double>(h, res_vec(st->reservoir_target_minimum_fraction, nan), D.res_tmf))) goto done;
             if ((rc = put_raw<double>(h, res_vec(st->reservoir_threshold, nan), D.res_thr))) goto done;
@@ -2911,6 +3084,14 @@ extern "C" int32_t wfh_last_substeps(wfh_handle* h, int32_t* subsurface, int32_t
     if (subsurface) *subsurface = h->nsub[0];
     if (land) *land = h->nsub[1];
     if (river) *river = h->nsub[2];
+    return WFH_OK;
+}
+
+// The column is a kernel argument (part of Dev), so it applies to steps launched after the call.
+extern "C" int32_t wfh_set_julian_day(wfh_handle* h, int32_t julian_day) {
+    if (!h) return WFH_ERR_ARG;
+    if (julian_day < 1 || julian_day > 365) return fail(h, WFH_ERR_ARG, "julian day outside 1..365");
+    h->d.res_hq_col = julian_day - 1;
     return WFH_OK;
 }
 

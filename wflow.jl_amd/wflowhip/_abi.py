@@ -7,7 +7,7 @@ test-suite checks the sizes against the compiled library's view of them.
 import ctypes
 import enum
 
-WFH_ABI_VERSION = 2
+WFH_ABI_VERSION = 3
 WFH_MAX_LAYERS = 8
 
 WFH_OK = 0
@@ -69,6 +69,9 @@ STATIC_FIELDS = [
     ("reservoir_demand", _D), ("reservoir_maximum_release", _D), ("reservoir_maximum_storage", _D),
     ("reservoir_target_full_fraction", _D), ("reservoir_target_minimum_fraction", _D), ("reservoir_threshold", _D),
     ("reservoir_rating_curve_coefficient", _D), ("reservoir_external_inflow", _D), ("reservoir_outflow_obs", _D),
+    ("reservoir_storage_curve_type", _I32), ("reservoir_rating_curve_exponent", _D),
+    ("reservoir_sh_ptr", _I64), ("reservoir_sh_h", _D), ("reservoir_sh_s", _D),
+    ("reservoir_hq_ptr", _I64), ("reservoir_hq_h", _D), ("reservoir_hq_q", _D),
 ]
 
 
@@ -166,7 +169,7 @@ SETTABLE_VARS = {Var.CANOPY_STORAGE, Var.SNOW_STORAGE, Var.SNOW_WATER, Var.SOIL_
 EXPORTS = ["wfh_create", "wfh_set_forcing", "wfh_step", "wfh_get", "wfh_set", "wfh_get_topology",
            "wfh_state_size", "wfh_save_state", "wfh_load_state", "wfh_sync", "wfh_set_timing",
            "wfh_get_timing", "wfh_last_error", "wfh_destroy", "wfh_get_node_order",
-           "wfh_kernel_bytes", "wfh_last_substeps"]
+           "wfh_kernel_bytes", "wfh_last_substeps", "wfh_set_julian_day"]
 
 
 def var_count(var, n, nriv, nlayers, nres=0):

@@ -67,6 +67,8 @@ def load_library():
     L.wfh_kernel_bytes.restype = i32
     L.wfh_last_substeps.argtypes = [P, P, P, P]
     L.wfh_last_substeps.restype = i32
+    L.wfh_set_julian_day.argtypes = [P, i32]
+    L.wfh_set_julian_day.restype = i32
     _LIB = L
     return L
 
@@ -202,6 +204,11 @@ class Model:
         a, b = ctypes.c_int64(), ctypes.c_int64()
         self._check(self.lib.wfh_kernel_bytes(self.h, ctypes.addressof(a), ctypes.addressof(b)))
         return a.value, b.value
+
+    def set_julian_day(self, day):
+        """Rating-curve column (julian day 1..365) of the following steps (update_index_hq!
+        Wflow/src/routing/surface/reservoir.jl:379-386)."""
+        self._check(self.lib.wfh_set_julian_day(self.h, int(day)))
 
     def last_substeps(self):
         """(subsurface, overland, river) sub-step counts of the last step."""

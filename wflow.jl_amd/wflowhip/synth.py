@@ -338,12 +338,17 @@ def land_downstream(catchment):
     return np.where(ok, k, -1)
 
 
-def add_reservoirs(catchment, count, seed=0, radius=1):
+def add_reservoirs(catchment, count, seed=0, radius=1, curves=False):
     """Place `count` reservoirs on river cells (in place): outlets on river cells whose river
     continues downstream, at most one reservoir feeding any river cell, each with the cells
     within `radius` raster steps of its outlet as its area (cells of an earlier reservoir
     excluded). Types alternate simple (4) and modified Puls (3); some abstract water (negative
-    external inflow) and one uses observed outflow. Sizes are scaled to the cell size."""
+    external inflow) and one uses observed outflow. Sizes are scaled to the cell size.
+
+    curves=True cycles the outflow types 1 (rating curve), 2 (free weir), 3 and 4, gives every
+    third reservoir (k % 3 == 1) an interpolated storage curve (S-H table), and gives the
+    rating-curve reservoirs H-Q tables with a seasonal cycle over the 365 day columns; reservoir 4
+    starts above the top of its H-Q table, so it overflows."""
     c = catchment
     rng = np.random.default_rng(seed + 4242)
     down = land_downstream(c)
@@ -394,7 +399,50 @@ def add_reservoirs(catchment, count, seed=0, radius=1):
     if m >= 3:
         obs[2] = maxsto[2] * 3e-8
     A["reservoir_outflow_obs"] = obs
+    if curves:
+        _add_reservoir_curves(c, rng)
     return c
+
+
+def _add_reservoir_curves(c, rng):
+    A = c.arrays
+    m = A["reservoir_outlet"].size
+    k = np.arange(m)
+    types = np.array([1, 2, 3, 4], np.int32)[k % 4]
+    sct = np.where(k % 3 == 1, 2, 1).astype(np.int32)
+    area = A["reservoir_area"]
+    wl = A["reservoir_waterlevel"]
+    A["reservoir_outflow_type"] = types
+    A["reservoir_storage_curve_type"] = sct
+    A["reservoir_rating_curve_exponent"] = rng.uniform(1.4, 1.6, m)
+    sh_ptr, sh_h, sh_s = [0], [], []
+    hq_ptr, hq_h, hq_q = [0], [], []
+    day = np.arange(1, 366)
+    for r in range(m):
+        if sct[r] == 2:  # convex storage curve on a ragged water-level grid, 0 .. 12 m
+            rows = 10 + r % 7
+            h = np.sort(np.concatenate([[0.0, 12.0], rng.uniform(0.2, 11.8, rows - 2)]))
+            sh_h += list(h)
+            sh_s += list(area[r] * h * (1.0 + 0.05 * h))
+        sh_ptr.append(len(sh_h))
+        if types[r] == 1:  # Q = a (H - H_spill)^1.5, seasonal factor per day column
+            hmax = 4.5 if r == 4 else 12.0
+            if r == 4:
+                wl[r] = 5.0
+            rows = 8 + r % 5
+            h = np.linspace(0.0, hmax, rows)
+            spill = rng.uniform(0.5, 1.5)
+            a = area[r] * rng.uniform(2e-7, 6e-7)
+            q = a * np.maximum(h - spill, 0.0)[:, None] ** 1.5 * (1.0 + 0.3 * np.sin(2 * np.pi * day / 365.0))[None, :]
+            hq_h += list(h)
+            hq_q += list(q.ravel())
+        hq_ptr.append(len(hq_h))
+    A["reservoir_sh_ptr"] = np.array(sh_ptr, np.int64)
+    A["reservoir_sh_h"] = np.array(sh_h if sh_h else [0.0])
+    A["reservoir_sh_s"] = np.array(sh_s if sh_s else [0.0])
+    A["reservoir_hq_ptr"] = np.array(hq_ptr, np.int64)
+    A["reservoir_hq_h"] = np.array(hq_h if hq_h else [0.0])
+    A["reservoir_hq_q"] = np.array(hq_q if hq_q else [0.0] * 365)
 
 
 # ------------------------------------------------------------------------------------------
