@@ -73,6 +73,9 @@ typedef struct wfh_config {
     double subsurface_alpha_coefficient; /* subsurface_kinematic_wave__alpha_coefficient (:88), 1.0 */
     int32_t snow_gravitational_transport; /* snow_gravitational_transport__flag (:70): lateral snow
                                              transport downhill (surface_process.jl:9-19), needs snow */
+    int32_t floodplain_1d;               /* floodplain_1d__flag: a 1-D floodplain beside every river
+                                            cell (surface_kinwave.jl:387-434, 572-601); needs the
+                                            floodplain_* fields of wfh_static                   */
 } wfh_config;
 
 /* Static maps as the reference reads them (standard names in brackets), already in SI.
@@ -170,6 +173,14 @@ typedef struct wfh_static {
     const double* reservoir_hq_h;     /* [rows] water level H [m]                                */
     const double* reservoir_hq_q;     /* [rows][365] outflow Q [m3 s-1], one column per julian
                                          day (read_hq_csv's Q matrix, row-major)                */
+    /* 1-D floodplain (FloodPlainProfile / FloodPlainParameters, floodplain.jl:29-147, 374-402),
+     * read when wfh_config.floodplain_1d is set */
+    int32_t floodplain_ndepth;        /* number of flood depths (flood_depth, without 0 m)       */
+    const double* floodplain_depth;   /* [ndepth] flood depths [m], ascending                    */
+    const double* floodplain_volume;  /* [ndepth][nriv] cumulative floodplain volume at each depth
+                                         (floodplain_water__sum_of_volume_per_depth) [m3]       */
+    const double* floodplain_manning_n; /* [nriv] (floodplain_water_flow__manning_n_parameter)   */
+    const double* floodplain_slope;     /* [nriv] (floodplain__slope), clamped at 1e-5           */
 } wfh_static;
 
 /* Readable / writable variables (BMI get_value / set_value equivalents, bmi.jl:203-262). */
@@ -254,6 +265,14 @@ typedef enum wfh_var {
     /* lateral snow transport [n] (snow_in / snow_out of the snow model) [m s-1] */
     WFH_SNOW_IN,
     WFH_SNOW_OUT,
+    /* 1-D floodplain [nriv] (FloodPlainVariables, floodplain.jl:319-341); with a floodplain,
+     * WFH_RIVER_Q_AVERAGE and WFH_RIVER_QIN_AVERAGE are channel plus floodplain */
+    WFH_FLOODPLAIN_STORAGE,              /* [m3] (a state) */
+    WFH_FLOODPLAIN_H,                    /* flood depth [m] */
+    WFH_FLOODPLAIN_Q,                    /* [m3 s-1] of the last sub-step */
+    WFH_FLOODPLAIN_Q_AVERAGE,
+    WFH_FLOODPLAIN_QIN_AVERAGE,
+    WFH_RIVER_Q_CHANNEL_AVERAGE,         /* channel discharge alone */
     WFH_VAR_COUNT
 } wfh_var;
 

@@ -119,6 +119,9 @@ def lib():
         for f in ("wfo_reservoir_maximum_storage", "wfo_reservoir_initial_storage"):
             getattr(L, f).restype = _c
             getattr(L, f).argtypes = [ctypes.c_void_p]
+        L.wfo_floodplain_profile.argtypes = [ctypes.c_int64, _D, _D, ctypes.c_int64, _c, _c, _D]
+        L.wfo_flood_depth.restype = _c
+        L.wfo_flood_depth.argtypes = [_D, ctypes.c_int64, _D, _c, _c]
         L.wfo_set_julian_day.restype = ctypes.c_int32
         L.wfo_set_julian_day.argtypes = [ctypes.c_void_p, ctypes.c_int32]
         _lib = L
@@ -183,6 +186,30 @@ def interpolate_linear(x, xp, fp):
     xp = np.ascontiguousarray(xp, np.float64)
     fp = np.ascontiguousarray(fp, np.float64)
     return lib().wfo_interpolate_linear(float(x), xp.ctypes.data_as(_D), fp.ctypes.data_as(_D), xp.size, 1)
+
+
+def floodplain_profile(depth, volume, flow_width, flow_length):
+    """FloodPlainProfile (floodplain.jl:48-147) of one river cell: returns (depth with 0,
+    storage, width, flow area, wetted perimeter)."""
+    d = np.ascontiguousarray(depth, np.float64)
+    v = np.ascontiguousarray(volume, np.float64)
+    out = np.zeros(4 * (d.size + 1))
+    lib().wfo_floodplain_profile(d.size, d.ctypes.data_as(_D), v.ctypes.data_as(_D), 1, flow_width, flow_length,
+                                 out.ctypes.data_as(_D))
+    s, w, a, p = out.reshape(4, d.size + 1)
+    return np.concatenate([[0.0], d]), s, w, a, p
+
+
+def flood_depth(depth, volume, flow_width, flow_length, flood_storage):
+    """compute_flood_depth (floodplain.jl:316-327) on the profile of floodplain_profile."""
+    d = np.ascontiguousarray(depth, np.float64)
+    v = np.ascontiguousarray(volume, np.float64)
+    prof = np.zeros(4 * (d.size + 1))
+    lib().wfo_floodplain_profile(d.size, d.ctypes.data_as(_D), v.ctypes.data_as(_D), 1, flow_width, flow_length,
+                                 prof.ctypes.data_as(_D))
+    d0 = np.concatenate([[0.0], d])
+    return lib().wfo_flood_depth(prof.ctypes.data_as(_D), d.size + 1, d0.ctypes.data_as(_D), flood_storage,
+                                 flow_length)
 
 
 def reservoir_maximum_storage(res):

@@ -539,6 +539,59 @@ void wfo_compute_mass_balance_error(double tin, double tout, double rate, double
 /* ------------------------------------------------------------------------------------ */
 
 /* update_diagnostic_vars!  soil.jl:1400-1436 */
+/* ---- 1-D floodplain (Wflow/src/routing/surface/floodplain.jl) ---- */
+
+/* interpolation_indices  floodplain.jl:272-284: i1 = last index with v[i] <= x (first index when
+ * none), i2 = i1 + 1 except at the end; 0-based */
+static void fp_interpolation_indices(double x, const double* v, int64_t n, int64_t* i1, int64_t* i2) {
+    int64_t a = 0;
+    for (int64_t i = 0; i < n; ++i)
+        if (v[i] <= x) a = i;
+    *i1 = a;
+    *i2 = a == n - 1 ? a : a + 1;
+}
+
+void wfo_floodplain_profile(int64_t nd0, const double* depth, const double* vol, int64_t stride, double fw, double fl,
+                            double* out) {
+    const int64_t nd = nd0 + 1;
+    double *S = out, *W = S + nd, *A = W + nd, *P = A + nd;
+    double* seg = (double*)calloc((size_t)nd, sizeof(double));
+    for (int64_t j = 0; j < nd; ++j) S[j] = W[j] = A[j] = P[j] = 0.0;
+    W[0] = fw;
+    for (int64_t j = 0; j < nd - 1; ++j) {
+        const double h = depth[j] - (j ? depth[j - 1] : 0.0);
+        const double diff_storage = vol[j * stride] - (j ? vol[(j - 1) * stride] : 0.0);
+        W[j + 1] = diff_storage / (h * fl);
+        if (W[j + 1] < W[j]) W[j + 1] = W[j]; /* width constant or increasing (the reference warns) */
+        A[j + 1] = W[j + 1] * h;
+        P[j + 1] = (W[j + 1] - W[j]) + 2.0 * h;
+        seg[j + 1] = A[j + 1] * fl;
+        if (j == 0) P[0] = P[1] - 2.0 * h;
+    }
+    for (int64_t j = 2; j < nd; ++j) P[j] = P[j - 1] + P[j];   /* cumsum of rows 2:end */
+    for (int64_t j = 1; j < nd; ++j) A[j] = A[j - 1] + A[j];   /* cumsum */
+    S[0] = seg[0];
+    for (int64_t j = 1; j < nd; ++j) S[j] = S[j - 1] + seg[j]; /* cumsum of the segments */
+    free(seg);
+}
+
+double wfo_flood_depth(const double* prof, int64_t nd, const double* depth0, double flood_storage, double fl) {
+    const double *S = prof, *W = prof + nd;
+    int64_t i1, i2;
+    fp_interpolation_indices(flood_storage, S, nd, &i1, &i2);
+    const double delta_A = (flood_storage - S[i1]) / fl;
+    const double delta_h = delta_A / W[i2];
+    return depth0[i1] + delta_h;
+}
+
+/* compute_floodplain_flow_area  floodplain.jl:330-341 with compute_flood_flow_area :286-300 */
+static double fp_flow_area(const double* W, const double* A, const double* depth, double h, int64_t i1, int64_t i2) {
+    const double channel_area = W[0] * h;
+    const double delta_h = h - depth[i1];
+    const double flood_flow_area = A[i1] + (W[i2] * delta_h);
+    return wfh_max(flood_flow_area - channel_area, 0.0);
+}
+
 /* ---- reservoirs (Wflow/src/routing/surface/reservoir.jl), linear storage curve ---- */
 
 /* update_reservoir_simple  reservoir.jl:389-421; inflow, precipitation, evaporation in m3 s-1 */
@@ -1014,6 +1067,13 @@ struct wfo_model {
     uint8_t* land_cov;
     double *res_store_prev, *res_outflow_avg, *res_inflow_avg, *res_actevap_avg, *res_abs_avg, *bal_res;
     double* res_tab; /* copies of the S-H and H-Q tables the reservoirs point into */
+    /* 1-D floodplain (floodplain.jl); fp_prof: [nriv][4][fp_nd] storage, width, flow area,
+     * wetted perimeter of FloodPlainProfile, depth 0 included */
+    int fp;
+    int64_t fp_nd;
+    double *fp_depth, *fp_prof, *fp_n, *fp_slope, *r_bfs, *r_bfd;
+    double *fp_storage, *fp_h, *fp_q, *fp_qin, *fp_exch, *fp_cap, *fp_q_cum, *fp_qin_cum, *fp_q_avg, *fp_qin_avg;
+    double* r_qch_avg;
 };
 
 static double* dalloc(int64_t n) { return (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
@@ -1217,6 +1277,39 @@ wfo_model* wfo_create(const wfh_config* cfg, const wfh_static* st, char* err, in
             double at = wfh_pow(st->river_manning_n[r] / sqrt(s), 0.6);
             m->r_alpha[r] = at * wfh_pow(wp, apow);
             m->r_area[r] = m->area[m->river_land[r]];
+        }
+    }
+    /* ---- 1-D floodplain: FloodPlainProfile  floodplain.jl:48-147, FloodPlainModel :380-402,
+     * bankfull_storage  surface_kinwave.jl:83 ---- */
+    m->fp = m->cfg.floodplain_1d != 0;
+    if (m->fp) {
+        const int64_t nd = (int64_t)st->floodplain_ndepth + 1;
+        int ok = st->floodplain_ndepth >= 1 && st->floodplain_depth && st->floodplain_volume &&
+                 st->floodplain_manning_n && st->floodplain_slope && st->river_bankfull_depth;
+        for (int64_t j = 0; ok && j < nd - 1; ++j)
+            ok = isfinite(st->floodplain_depth[j]) && st->floodplain_depth[j] > (j ? st->floodplain_depth[j - 1] : 0.0);
+        for (int64_t j = 0; ok && j < (nd - 1) * nriv; ++j) ok = isfinite(st->floodplain_volume[j]);
+        if (!ok) {
+            snprintf(err, (size_t)errlen, "floodplain: flood depths (ascending, > 0), volumes, manning_n, slope "
+                     "and river_bankfull_depth needed");
+            goto fail;
+        }
+        m->fp_nd = nd;
+        m->fp_depth = dalloc(nd);
+        for (int64_t j = 1; j < nd; ++j) m->fp_depth[j] = st->floodplain_depth[j - 1];
+        m->fp_prof = dalloc(nriv * 4 * nd);
+        for (int64_t r = 0; r < nriv; ++r)
+            wfo_floodplain_profile(nd - 1, st->floodplain_depth, st->floodplain_volume + r, nriv, m->r_fw[r], m->r_fl[r],
+                                   m->fp_prof + r * 4 * nd);
+        m->fp_n = dalloc(nriv); m->fp_slope = dalloc(nriv); m->r_bfs = dalloc(nriv); m->r_bfd = dalloc(nriv);
+        m->fp_storage = dalloc(nriv); m->fp_h = dalloc(nriv); m->fp_q = dalloc(nriv); m->fp_qin = dalloc(nriv);
+        m->fp_exch = dalloc(nriv); m->fp_cap = dalloc(nriv); m->fp_q_cum = dalloc(nriv); m->fp_qin_cum = dalloc(nriv);
+        m->fp_q_avg = dalloc(nriv); m->fp_qin_avg = dalloc(nriv); m->r_qch_avg = dalloc(nriv);
+        for (int64_t r = 0; r < nriv; ++r) {
+            m->fp_n[r] = st->floodplain_manning_n[r];
+            m->fp_slope[r] = wfh_clamp(st->floodplain_slope[r], 0.00001, INFINITY);
+            m->r_bfd[r] = st->river_bankfull_depth[r];
+            m->r_bfs[r] = st->river_bankfull_depth[r] * m->r_fw[r] * m->r_fl[r];
         }
     }
     /* ---- reservoirs (network.jl:296-360, reservoir.jl:47-300) and the upstream lists without
@@ -1558,7 +1651,10 @@ void wfo_destroy(wfo_model* m) {
         (void**)&m->res_land, (void**)&m->res_river, (void**)&m->land_res, (void**)&m->river_res,
         (void**)&m->cov_ptr, (void**)&m->cov_idx, (void**)&m->land_cov, (void**)&m->res_store_prev,
         (void**)&m->res_outflow_avg, (void**)&m->res_inflow_avg, (void**)&m->res_actevap_avg, (void**)&m->res_abs_avg,
-        (void**)&m->bal_res, (void**)&m->res_tab, (void**)&m->snow_in, (void**)&m->snow_out, (void**)&m->snow_flux_w,
+        (void**)&m->bal_res, (void**)&m->res_tab, (void**)&m->fp_depth, (void**)&m->fp_prof, (void**)&m->fp_n,
+        (void**)&m->fp_slope, (void**)&m->r_bfs, (void**)&m->r_bfd, (void**)&m->fp_storage, (void**)&m->fp_h,
+        (void**)&m->fp_q, (void**)&m->fp_qin, (void**)&m->fp_exch, (void**)&m->fp_cap, (void**)&m->fp_q_cum,
+        (void**)&m->fp_qin_cum, (void**)&m->fp_q_avg, (void**)&m->fp_qin_avg, (void**)&m->r_qch_avg, (void**)&m->snow_in, (void**)&m->snow_out, (void**)&m->snow_flux_w,
         (void**)&m->snow_cap_s, (void**)&m->snow_cap_w};
     for (size_t k = 0; k < sizeof p / sizeof p[0]; ++k) free(*p[k]);
     free(m);
@@ -1662,6 +1758,79 @@ static double land_total_storage(const wfo_model* m, int64_t i) {
 
 /* update_model!  Wflow/src/sbm_model.jl:60-92, preceded by storage_prev! and followed by
  * compute_mass_balance! (Wflow.jl:315-317) when water_mass_balance is set. */
+/* river_channel_floodplain_exchange!  surface_kinwave.jl:572-601: the water above bankfull is
+ * shared instantaneously between channel and floodplain; the change of the channel's storage
+ * enters the river's kinematic wave as a lateral inflow */
+static void river_floodplain_exchange(wfo_model* m, double dt) {
+    const int64_t nd = m->fp_nd;
+    for (int64_t i = 0; i < m->nriv; ++i) {
+        const double storage_total = m->r_storage[i] + m->fp_storage[i];
+        double delta_river_storage;
+        if (storage_total > m->r_bfs[i]) {
+            const double flood_storage = storage_total - m->r_bfs[i];
+            const double h = wfo_flood_depth(m->fp_prof + i * 4 * nd, nd, m->fp_depth, flood_storage, m->r_fl[i]);
+            const double river_storage = (m->r_bfd[i] + h) * m->r_fw[i] * m->r_fl[i];
+            delta_river_storage = river_storage - m->r_storage[i];
+            m->fp_storage[i] = wfh_max(storage_total - river_storage, 0.0);
+            m->fp_h[i] = m->fp_storage[i] > 0.0 ? h : 0.0;
+        } else {
+            delta_river_storage = wfh_max(storage_total - m->r_storage[i], 0.0);
+            m->fp_h[i] = 0.0;
+            m->fp_storage[i] = 0.0;
+        }
+        m->fp_exch[i] = delta_river_storage / dt;
+    }
+}
+
+/* update_floodplain_model!  surface_kinwave.jl:387-430: Manning flow capacity from the flood
+ * depth (zero unless both this cell's and the downstream cell's floodplain flow area exceed
+ * 1e-5 m2), accucapacityflux! of the floodplain storage down the river graph in its
+ * topological order (routing/utils.jl:82-108, the storage is updated in place), flux_in!
+ * (:162-167) over the upstream lists without reservoir outlets */
+static void update_floodplain(wfo_model* m, double dt) {
+    const int64_t nd = m->fp_nd, nriv = m->nriv;
+    for (int64_t k = 0; k < nriv; ++k) {
+        const int64_t v = m->rorder[k];
+        const double h = m->fp_h[v];
+        double cap = 0.0;
+        if (h > 0.0) {
+            int64_t i1, i2;
+            fp_interpolation_indices(h, m->fp_depth, nd, &i1, &i2);
+            const double* Wv = m->fp_prof + (v * 4 + 1) * nd;
+            const double flow_area = fp_flow_area(Wv, Wv + nd, m->fp_depth, h, i1, i2);
+            const int64_t ds = m->rdown[v];
+            double flow_area_ds = flow_area;
+            if (ds >= 0) {
+                const double* Wd = m->fp_prof + (ds * 4 + 1) * nd;
+                flow_area_ds = fp_flow_area(Wd, Wd + nd, m->fp_depth, h, i1, i2);
+            }
+            if (flow_area > 1.0e-05 && flow_area_ds > 1.0e-05) {
+                const double* Pv = Wv + 2 * nd;
+                const double wetted_perimeter = Pv[i1] + 2.0 * (h - m->fp_depth[i1]);
+                const double hydraulic_radius = flow_area / wetted_perimeter;
+                /* manning_flow  surface_process.jl:166-169 */
+                cap = wfh_cbrt(hydraulic_radius * hydraulic_radius) * sqrt(m->fp_slope[v]) * flow_area / m->fp_n[v];
+            }
+        }
+        m->fp_cap[v] = cap;
+    }
+    for (int64_t k = 0; k < nriv; ++k) {
+        const int64_t v = m->rorder[k];
+        const double flux_val = wfh_min(m->fp_storage[v] / dt, m->fp_cap[v]);
+        const double material_update = flux_val * dt;
+        m->fp_storage[v] -= material_update;
+        m->fp_q[v] = flux_val;
+        if (m->rdown[v] >= 0) m->fp_storage[m->rdown[v]] += material_update;
+    }
+    for (int64_t r = 0; r < nriv; ++r) m->fp_q_cum[r] += m->fp_q[r] * dt;
+    for (int64_t v = 0; v < nriv; ++v) {
+        double s = 0.0;
+        for (int64_t e = m->rfup_ptr[v]; e < m->rfup_ptr[v + 1]; ++e) s = s + m->fp_q[m->rfup_idx[e]];
+        m->fp_qin[v] = s;
+    }
+    for (int64_t r = 0; r < nriv; ++r) m->fp_qin_cum[r] += m->fp_qin[r] * dt;
+}
+
 int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const double* temp) {
     const int64_t n = m->n, nriv = m->nriv;
     const double dt = m->cfg.dt;
@@ -1692,7 +1861,8 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
             m->land_store_prev[i] = land_total_storage(m, i);
             m->ol_store_prev[i] = m->ol_storage[i];
         }
-        for (int64_t r = 0; r < nriv; ++r) m->r_store_prev[r] = m->r_storage[r];
+        for (int64_t r = 0; r < nriv; ++r) /* get_storage  mass_balance.jl:132-140 */
+            m->r_store_prev[r] = m->fp ? m->r_storage[r] + m->fp_storage[r] : m->r_storage[r];
         for (int64_t k = 0; k < m->nres; ++k) m->res_store_prev[k] = m->res[k].storage;
     }
     /* ---- update_land_hydrology_model!  sbm.jl:82-132 ---- */
@@ -1878,6 +2048,11 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
         m->r_q_cum[r] = 0.0;
         m->r_qin_cum[r] = 0.0;
     }
+    if (m->fp)
+        for (int64_t r = 0; r < nriv; ++r) {
+            m->fp_q_cum[r] = 0.0;
+            m->fp_qin_cum[r] = 0.0;
+        }
     {
         double t = 0.0;
         while (t < dt) { /* update_river_flow_model!  :613-662 */
@@ -1886,6 +2061,7 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
                              : m->cfg.river_dt;
             dts = check_timestepsize(dts, t, dt);
             ++m->nsub[2];
+            if (m->fp) river_floodplain_exchange(m, dts);
             for (int64_t r = 0; r < nriv; ++r) m->r_qin[r] = 0.0;
             for (int64_t k = 0; k < nriv; ++k) { /* kinwave_river_update!  :492-566 */
                 int64_t v = m->rorder[k];
@@ -1894,6 +2070,7 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
                 m->r_qin[v] += s;
                 double inflow = 0.0 / m->r_fl[v];
                 inflow -= 0.0 / m->r_fl[v];
+                if (m->fp) inflow += m->fp_exch[v] / m->r_fl[v];
                 double o[2];
                 wfo_kinematic_wave_it(m->r_qin[v], m->r_q[v], m->r_qlat[v] + inflow, m->r_alpha[v], dts, m->r_fl[v],
                                       o, &m->kw_iters);
@@ -1905,12 +2082,21 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
                 m->r_q_cum[v] += m->r_q[v] * dts;
                 m->r_qin_cum[v] += m->r_qin[v] * dts;
             }
+            if (m->fp) update_floodplain(m, dts);
             t += dts;
         }
         for (int64_t r = 0; r < nriv; ++r) {
             m->r_q_avg[r] = m->r_q_cum[r] / dt;
             m->r_qin_avg[r] = m->r_qin_cum[r] / dt;
         }
+        if (m->fp) /* surface_kinwave.jl:652-659 */
+            for (int64_t r = 0; r < nriv; ++r) {
+                m->fp_q_avg[r] = m->fp_q_cum[r] / dt;
+                m->r_qch_avg[r] = m->r_q_avg[r];
+                m->r_q_avg[r] = m->r_qch_avg[r] + m->fp_q_avg[r];
+                m->fp_qin_avg[r] = m->fp_qin_cum[r] / dt;
+                m->r_qin_avg[r] = m->r_qin_avg[r] + m->fp_qin_avg[r];
+            }
         for (int64_t k = 0; k < m->nres; ++k) { /* average_reservoir_vars!  :244-258 */
             m->res_outflow_avg[k] = m->res[k].outflow_cumulative / dt;
             m->res_inflow_avg[k] = m->res[k].inflow_cumulative / dt;
@@ -1954,8 +2140,9 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
         }
         for (int64_t r = 0; r < nriv; ++r) {
             double o[2];
+            const double storage = m->fp ? m->r_storage[r] + m->fp_storage[r] : m->r_storage[r];
             wfo_compute_mass_balance_error(m->r_inwater[r] + m->r_qin_avg[r] + wfh_max(0.0, 0.0),
-                                           m->r_q_avg[r] + 0.0 + 0.0, (m->r_storage[r] - m->r_store_prev[r]) / dt, o);
+                                           m->r_q_avg[r] + 0.0 + 0.0, (storage - m->r_store_prev[r]) / dt, o);
             m->bal_r[r] = o[0];
         }
         for (int64_t k = 0; k < m->nres; ++k) { /* compute_flow_balance!(::ReservoirModel)  :261-283 */
@@ -2015,7 +2202,8 @@ int32_t wfo_get(const wfo_model* m, int32_t var, double* dst, int64_t count) {
     const int N = m->N;
     int64_t need = n;
     if (var == WFH_USTORELAYERDEPTH || var == WFH_VWC) need = (int64_t)N * n;
-    if ((var >= WFH_RIVER_Q && var <= WFH_RIVER_INWATER) || var == WFH_BALANCE_RIVER_ERROR || var == WFH_RIVER_ALPHA)
+    if ((var >= WFH_RIVER_Q && var <= WFH_RIVER_INWATER) || var == WFH_BALANCE_RIVER_ERROR || var == WFH_RIVER_ALPHA ||
+        (var >= WFH_FLOODPLAIN_STORAGE && var <= WFH_RIVER_Q_CHANNEL_AVERAGE))
         need = nr;
     if (var >= WFH_RESERVOIR_STORAGE && var <= WFH_BALANCE_RESERVOIR_ERROR) {
         if (count != m->nres) return WFH_ERR_ARG;
@@ -2107,6 +2295,12 @@ int32_t wfo_get(const wfo_model* m, int32_t var, double* dst, int64_t count) {
     case WFH_FLOW_FRACTION_TO_RIVER: ARR(m->fftr, n);
     case WFH_SNOW_IN: ARR(m->snow_in, n);
     case WFH_SNOW_OUT: ARR(m->snow_out, n);
+    case WFH_FLOODPLAIN_STORAGE: if (!m->fp) return WFH_ERR_ARG; ARR(m->fp_storage, nr);
+    case WFH_FLOODPLAIN_H: if (!m->fp) return WFH_ERR_ARG; ARR(m->fp_h, nr);
+    case WFH_FLOODPLAIN_Q: if (!m->fp) return WFH_ERR_ARG; ARR(m->fp_q, nr);
+    case WFH_FLOODPLAIN_Q_AVERAGE: if (!m->fp) return WFH_ERR_ARG; ARR(m->fp_q_avg, nr);
+    case WFH_FLOODPLAIN_QIN_AVERAGE: if (!m->fp) return WFH_ERR_ARG; ARR(m->fp_qin_avg, nr);
+    case WFH_RIVER_Q_CHANNEL_AVERAGE: if (!m->fp) return WFH_ERR_ARG; ARR(m->r_qch_avg, nr);
     default: return WFH_ERR_ARG;
     }
 }
@@ -2136,6 +2330,10 @@ int32_t wfo_set(wfo_model* m, int32_t var, const double* src, int64_t count) {
     case WFH_LAND_H: if (count != n) return WFH_ERR_ARG; memcpy(m->ol_h, src, 8 * (size_t)n); return 0;
     case WFH_RIVER_Q: if (count != nr) return WFH_ERR_ARG; memcpy(m->r_q, src, 8 * (size_t)nr); return 0;
     case WFH_RIVER_H: if (count != nr) return WFH_ERR_ARG; memcpy(m->r_h, src, 8 * (size_t)nr); return 0;
+    case WFH_FLOODPLAIN_STORAGE:
+        if (count != nr || !m->fp) return WFH_ERR_ARG;
+        memcpy(m->fp_storage, src, 8 * (size_t)nr);
+        return 0;
     case WFH_RESERVOIR_STORAGE:
     case WFH_RESERVOIR_WATERLEVEL:
         if (count != m->nres) return WFH_ERR_ARG;

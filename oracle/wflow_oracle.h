@@ -94,6 +94,13 @@ int32_t wfo_topology(const wfo_model* m, int64_t* land_order, int64_t* land_down
 int64_t wfo_kinwave_iterations(const wfo_model* m);
 /* subsurface, overland and river sub-step counts of the last wfo_step */
 void wfo_last_substeps(const wfo_model* m, int32_t* out3);
+/* FloodPlainProfile  floodplain.jl:48-147 for one river cell: nd0 flood depths (ascending, without
+ * 0), their cumulative volumes vol[j * stride]; out = [4][nd0 + 1] storage, width, flow area,
+ * wetted perimeter, depth 0 first */
+void wfo_floodplain_profile(int64_t nd0, const double* depth, const double* vol, int64_t stride, double fw, double fl,
+                            double* out);
+/* compute_flood_depth  floodplain.jl:316-327 on such a profile; depth0 = [0, depth...] */
+double wfo_flood_depth(const double* prof, int64_t nd, const double* depth0, double flood_storage, double fl);
 /* update_index_hq!  reservoir.jl:379-386: the rating-curve column (julian day 1..365); 0 or -1 */
 int32_t wfo_set_julian_day(wfo_model* m, int32_t julian_day);
 /* Statistics.quantile!(v, p) of k >= 2 values (sorts v); the kinematic-wave stable time step

@@ -131,6 +131,14 @@ struct Dev {
     int snow_lat;
     const int *sup_ptr, *sup_idx;  // [n+1], [E]: all upstream cells, in the reference's topological order
     double *snow_in, *snow_out;    // [n] (zeros without transport)
+    // ---- 1-D floodplain (floodplain_1d), indexed by river cell ----
+    int fp, fp_nd;
+    const double* fp_depth;        // [fp_nd] flood depths, 0 first
+    const double* fp_prof;         // [nriv][4][fp_nd] storage, width, flow area, wetted perimeter
+    const double *fp_n, *fp_slope, *r_bfs, *r_bfd;  // Manning n, slope, bankfull storage and depth
+    double *fp_storage, *fp_h, *fp_q, *fp_qin, *fp_exch, *fp_q_cum, *fp_qin_cum, *fp_q_avg, *fp_qin_avg;
+    double* r_qch_avg;             // channel discharge average
+    double *fp_rs, *fp_sprev;      // channel storage after the last sub-step; channel + floodplain at step start
 };
 // Dev is passed by value to every kernel: it must fit the kernel argument space
 static_assert(sizeof(Dev) + 512 <= 4096, "Dev too large for a kernel argument");
@@ -159,6 +167,8 @@ struct AdArgs {
     const int* rp_down;    // [nriv] river position downstream, or -1
     const double* rp_up_res;  // [Er] 1 when the edge's upstream river cell is a reservoir outlet
     const int* rp_node;    // [nriv] node of the river cell at each river position
+    const int *fp_sup_ptr, *fp_sup_idx;  // [nriv+1], [Er]: all upstream river positions, in the
+                                         // order of the reference's river topological sort
     int nchunks_r;
 };
 
@@ -1479,6 +1489,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_riv(Dev d, RouteArgs a, AdAr
         const double rqlat = r_inwater / rfl;
         double inflow = 0.0 / rfl;  // reservoir / lake / external inflow terms (none in this model)
         inflow -= 0.0 / rfl;
+        if (d.fp) inflow += d.fp_exch[r] / rfl;  // channel-floodplain exchange (surface_kinwave.jl:534-536)
         const double rqlat_all = rqlat + inflow;
         double r_storage = 0.0, rqin = 0.0;
         const KwPrep rpp = kw_prepare(rq, rqlat_all, ralpha, dts, rfl);
@@ -1502,6 +1513,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_riv(Dev d, RouteArgs a, AdAr
                                   (size_t)nr, rp, rdn >= 0 && !cons_in, cons_in ? (rdn & 63) : -1, lds_k, lds_q,
                                   riv_compute))
             return;
+        if (valid && d.fp) d.fp_rs[r] = r_storage;
         if (valid && !last) {
             d.r_q[r] = rq;
             d.r_h[r] = rh;
@@ -1511,6 +1523,164 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_riv(Dev d, RouteArgs a, AdAr
             riv_finish(d, r, rq, rh, r_storage, rqin, r_q_cum, r_qin_cum, r_inwater, d.r_storage[r]);
         }
     }
+}
+
+// ============================================================================================
+// 1-D floodplain of the kinematic-wave river (floodplain_1d); per river sub-step:
+// k_fp_exchange, the river sub-step (k_ad_riv), k_fp_accu, k_fp_qin; k_fp_finish after the last.
+// ============================================================================================
+// interpolation_indices  floodplain.jl:272-284, 0-based: the last index with v[i] <= x (the
+// first when none), and the next one except at the end
+__device__ __forceinline__ void fp_indices(const double x, const double* v, const int n, int& i1, int& i2) {
+    int a = 0;
+    for (int i = 0; i < n; ++i)
+        if (v[i] <= x) a = i;
+    i1 = a;
+    i2 = a == n - 1 ? a : a + 1;
+}
+
+// compute_floodplain_flow_area  floodplain.jl:330-341 (compute_flood_flow_area :286-300) on the
+// width (W) and flow area (A) rows of a cell's profile
+__device__ __forceinline__ double fp_flow_area(const double* W, const double* A, const double* depth, const double h,
+                                               const int i1, const int i2) {
+    const double channel_area = W[0] * h;
+    const double delta_h = h - depth[i1];
+    const double flood_flow_area = A[i1] + (W[i2] * delta_h);
+    return wfh_max(flood_flow_area - channel_area, 0.0);
+}
+
+// river_channel_floodplain_exchange!  surface_kinwave.jl:572-601, one thread per river cell; the
+// channel storage is the previous sub-step's (the previous step's at the first sub-step)
+__global__ void k_fp_exchange(Dev d, double dts, int first) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= d.nriv) return;
+    const int nd = d.fp_nd;
+    const double rs = first ? d.r_storage[r] : d.fp_rs[r];
+    double fs = d.fp_storage[r];
+    if (first) {
+        d.fp_q_cum[r] = 0.0;
+        d.fp_qin_cum[r] = 0.0;
+        if (d.balance) d.fp_sprev[r] = rs + fs;  // get_storage  mass_balance.jl:132-140
+    }
+    const double storage_total = rs + fs;
+    double delta_river_storage, fh;
+    if (storage_total > d.r_bfs[r]) {
+        const double flood_storage = storage_total - d.r_bfs[r];
+        const double* S = d.fp_prof + (size_t)r * 4 * nd;
+        const double* W = S + nd;
+        const double rfl = d.r_fl[r];
+        int i1, i2;
+        fp_indices(flood_storage, S, nd, i1, i2);  // compute_flood_depth  floodplain.jl:316-327
+        const double delta_A = (flood_storage - S[i1]) / rfl;
+        const double delta_h = delta_A / W[i2];
+        const double h = d.fp_depth[i1] + delta_h;
+        const double river_storage = (d.r_bfd[r] + h) * d.r_fw[r] * rfl;
+        delta_river_storage = river_storage - rs;
+        fs = wfh_max(storage_total - river_storage, 0.0);
+        fh = fs > 0.0 ? h : 0.0;
+    } else {
+        delta_river_storage = wfh_max(storage_total - rs, 0.0);
+        fh = 0.0;
+        fs = 0.0;
+    }
+    d.fp_storage[r] = fs;
+    d.fp_h[r] = fh;
+    d.fp_exch[r] = delta_river_storage / dts;
+}
+
+// update_floodplain_model!  surface_kinwave.jl:387-430: the Manning flow capacity of each cell
+// (zero unless this and the downstream cell's floodplain flow areas at this cell's depth exceed
+// 1e-5 m2), then accucapacityflux! of the floodplain storage down the unfiltered river graph
+// (routing/utils.jl:82-108) as a dataflow over river positions: a cell adds what its upstream
+// cells pass on, in the reference's topological order, to its storage, passes on
+// min(storage / dt, capacity) and keeps the rest.
+__global__ void __launch_bounds__(ROUTE_BLOCK) k_fp_accu(Dev d, RouteArgs a, AdArgs x, double dts) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    __shared__ int x_k[ROUTE_BLOCK];
+    __shared__ double x_q[ROUTE_BLOCK];
+    int* const lds_k = x_k + (threadIdx.x & ~(WAVE - 1));
+    double* const lds_q = x_q + (threadIdx.x & ~(WAVE - 1));
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[a.parity ^ 1] = 0ull;
+    const int nr = d.nriv, nd = d.fp_nd;
+    for (;;) {
+        AD_TICKET(x.nchunks_r)
+        const int rp = c * WAVE + lane;
+        const bool valid = rp < nr;
+        const int q_ = valid ? rp : 0;
+        const int r = x.rp_r[q_], rdn_ = x.rp_down[q_], e0 = x.fp_sup_ptr[q_], e1_ = x.fp_sup_ptr[q_ + 1];
+        const int rdn = valid ? rdn_ : -1;
+        const bool cons_in = rdn >= 0 && (rdn >> 6) == c;
+        const int e1 = valid ? e1_ : e0;
+        const double h = d.fp_h[r];
+        double cap = 0.0;
+        if (h > 0.0) {
+            int i1, i2;
+            fp_indices(h, d.fp_depth, nd, i1, i2);
+            const double* Wv = d.fp_prof + ((size_t)r * 4 + 1) * nd;
+            const double flow_area = fp_flow_area(Wv, Wv + nd, d.fp_depth, h, i1, i2);
+            double flow_area_ds = flow_area;
+            if (rdn_ >= 0) {
+                const double* Wd = d.fp_prof + ((size_t)x.rp_r[rdn_] * 4 + 1) * nd;
+                flow_area_ds = fp_flow_area(Wd, Wd + nd, d.fp_depth, h, i1, i2);
+            }
+            if (flow_area > 1.0e-05 && flow_area_ds > 1.0e-05) {
+                const double wetted_perimeter = Wv[2 * nd + i1] + 2.0 * (h - d.fp_depth[i1]);
+                const double hydraulic_radius = flow_area / wetted_perimeter;
+                // manning_flow  surface_process.jl:166-169
+                cap = wfh_cbrt(hydraulic_radius * hydraulic_radius) * sqrt(d.fp_slope[r]) * flow_area / d.fp_n[r];
+            }
+        }
+        const double material0 = d.fp_storage[r];
+        double s_new = material0, flux = 0.0;
+        auto pass = [&](int, double material, double) -> double {
+            const double flux_val = wfh_min(material / dts, cap);
+            const double update = flux_val * dts;
+            s_new = material - update;
+            flux = flux_val;
+            return update;
+        };
+        if (!run_family<FOLD_SNOW>(a, lane, c, valid, 1, x.fp_sup_idx, x.fp_sup_idx, nullptr, e0, e1, a.slot_r,
+                                   (size_t)nr, rp, rdn >= 0 && !cons_in, cons_in ? (rdn & 63) : -1, lds_k, lds_q, pass,
+                                   material0))
+            return;
+        if (valid) {
+            d.fp_storage[r] = s_new;
+            d.fp_q[r] = flux;
+            d.fp_q_cum[r] += flux * dts;
+        }
+    }
+}
+
+// flux_in!  routing/utils.jl:162-167: floodplain inflow = sum of the upstream cells' floodplain
+// flow over the upstream lists without reservoir outlets, ascending river index
+__global__ void k_fp_qin(Dev d, AdArgs x, double dts) {
+    const int rp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rp >= d.nriv) return;
+    double s = 0.0;
+    for (int e = x.rp_up_ptr[rp]; e < x.rp_up_ptr[rp + 1]; ++e)
+        if (x.rp_up_res[e] == 0.0) s = s + d.fp_q[x.rp_r[x.rp_up_idx[e]]];
+    const int r = x.rp_r[rp];
+    d.fp_qin[r] = s;
+    d.fp_qin_cum[r] += s * dts;
+}
+
+// After the last river sub-step (surface_kinwave.jl:652-659): the river averages become channel
+// plus floodplain, and the river balance counts the floodplain storage (mass_balance.jl:296-313)
+__global__ void k_fp_finish(Dev d) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= d.nriv) return;
+    const double dt = d.dt;
+    const double fq_avg = d.fp_q_cum[r] / dt, fqin_avg = d.fp_qin_cum[r] / dt;
+    const double qch = d.r_q_avg[r];
+    const double q_avg = qch + fq_avg, qin_avg = d.r_qin_avg[r] + fqin_avg;
+    d.fp_q_avg[r] = fq_avg;
+    d.fp_qin_avg[r] = fqin_avg;
+    d.r_qch_avg[r] = qch;
+    d.r_q_avg[r] = q_avg;
+    d.r_qin_avg[r] = qin_avg;
+    if (d.balance)
+        d.bal_r[r] = (d.r_inwater[r] + qin_avg + wfh_max(0.0, 0.0)) - (q_avg + 0.0 + 0.0) -
+                     ((d.r_storage[r] + d.fp_storage[r]) - d.fp_sprev[r]) / dt;
 }
 
 // update_total_water_storage! after the adaptive river sub-steps.
@@ -2054,6 +2224,7 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
     const int64_t n = st->n;
     const int N = cfg->n_config_layers + 1;
     const bool adaptive = cfg->kinematic_wave_adaptive != 0;
+    const bool fp = cfg->floodplain_1d != 0;
     if (n < 1 || n > (int64_t)INT32_MAX / 2 || N < 2 || N > 6 || !(cfg->dt > 0.0) ||
         (!adaptive && !(cfg->land_dt > 0.0 && cfg->river_dt > 0.0 && cfg->subsurface_dt > 0.0)) ||
         (adaptive && !(cfg->subsurface_alpha_coefficient > 0.0 && cfg->subsurface_alpha_coefficient < 1e300))) {
@@ -2197,6 +2368,19 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
             }
     }
     h->nres = nres;
+    if (fp) {  // FloodPlainProfile inputs (floodplain.jl:48-147, 380-402)
+        bool ok = st->floodplain_ndepth >= 1 && st->floodplain_depth && st->floodplain_volume &&
+                  st->floodplain_manning_n && st->floodplain_slope && st->river_bankfull_depth;
+        for (int64_t j = 0; ok && j < st->floodplain_ndepth; ++j)
+            ok = std::isfinite(st->floodplain_depth[j]) && st->floodplain_depth[j] > (j ? st->floodplain_depth[j - 1] : 0.0);
+        for (int64_t j = 0; ok && j < (int64_t)st->floodplain_ndepth * nriv; ++j) ok = std::isfinite(st->floodplain_volume[j]);
+        if (!ok) {
+            h->err = "floodplain: flood depths (ascending, > 0), finite volumes, manning_n, slope and "
+                     "river_bankfull_depth needed";
+            wfh_destroy_keep_error(h, out);
+            return WFH_ERR_ARG;
+        }
+    }
     // levels (longest upstream path) in DFS topological order, then the level-major node order
     std::vector<int32_t> level(n, 0);
     for (int64_t k = 0; k < n; ++k) {
@@ -2552,6 +2736,57 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
             if ((rc = put_state(h, {}, D.snow_in))) goto done;
             if ((rc = put_state(h, {}, D.snow_out))) goto done;
         }
+        // 1-D floodplain: the profile of every river cell (FloodPlainProfile  floodplain.jl:48-147),
+        // Manning n, slope clamped at 1e-5 (:395-396), bankfull storage (surface_kinwave.jl:83);
+        // the floodplain starts empty (FloodPlainVariables  floodplain.jl:319-341)
+        D.fp = fp ? 1 : 0;
+        if (fp) {
+            const int nd = st->floodplain_ndepth + 1;
+            D.fp_nd = nd;
+            std::vector<double> depth(nd, 0.0), prof((size_t)nriv * 4 * nd, 0.0), seg(nd);
+            for (int j = 1; j < nd; ++j) depth[j] = st->floodplain_depth[j - 1];
+            for (int64_t r = 0; r < nriv; ++r) {
+                double *S = &prof[(size_t)r * 4 * nd], *W = S + nd, *A = W + nd, *P = A + nd;
+                const double fl = st->river_length[r];
+                std::fill(seg.begin(), seg.end(), 0.0);
+                W[0] = st->river_width[r];
+                for (int j = 0; j < nd - 1; ++j) {
+                    const double hseg = depth[j + 1] - depth[j];
+                    const double v0 = j ? st->floodplain_volume[(size_t)(j - 1) * nriv + r] : 0.0;
+                    W[j + 1] = (st->floodplain_volume[(size_t)j * nriv + r] - v0) / (hseg * fl);
+                    if (W[j + 1] < W[j]) W[j + 1] = W[j];
+                    A[j + 1] = W[j + 1] * hseg;
+                    P[j + 1] = (W[j + 1] - W[j]) + 2.0 * hseg;
+                    seg[j + 1] = A[j + 1] * fl;
+                    if (j == 0) P[0] = P[1] - 2.0 * hseg;
+                }
+                for (int j = 2; j < nd; ++j) P[j] = P[j - 1] + P[j];
+                for (int j = 1; j < nd; ++j) A[j] = A[j - 1] + A[j];
+                S[0] = seg[0];
+                for (int j = 1; j < nd; ++j) S[j] = S[j - 1] + seg[j];
+            }
+            std::vector<double> fpn(nriv), fps(nriv), bfs(nriv), bfd(nriv);
+            for (int64_t r = 0; r < nriv; ++r) {
+                fpn[r] = st->floodplain_manning_n[r];
+                fps[r] = wfh_clamp(st->floodplain_slope[r], 0.00001, INFINITY);
+                bfd[r] = st->river_bankfull_depth[r];
+                bfs[r] = st->river_bankfull_depth[r] * st->river_width[r] * st->river_length[r];
+            }
+            if ((rc = put_raw<double>(h, depth, D.fp_depth))) goto done;
+            if ((rc = put_raw<double>(h, prof, D.fp_prof))) goto done;
+            if ((rc = put_raw<double>(h, fpn, D.fp_n))) goto done;
+            if ((rc = put_raw<double>(h, fps, D.fp_slope))) goto done;
+            if ((rc = put_raw<double>(h, bfs, D.r_bfs))) goto done;
+            if ((rc = put_raw<double>(h, bfd, D.r_bfd))) goto done;
+            const std::vector<double> zr((size_t)std::max<int64_t>(nriv, 1), 0.0);
+            double** fo[] = {&D.fp_storage, &D.fp_h, &D.fp_q, &D.fp_qin, &D.fp_exch, &D.fp_q_cum, &D.fp_qin_cum,
+                             &D.fp_q_avg, &D.fp_qin_avg, &D.r_qch_avg, &D.fp_rs, &D.fp_sprev};
+            for (double** o : fo) {
+                const double* tmp = nullptr;
+                if ((rc = put_raw<double>(h, zr, tmp))) goto done;
+                *o = const_cast<double*>(tmp);
+            }
+        }
         if ((rc = put_raw<int>(h, down_node, D.down))) goto done;
         if ((rc = put_raw<double>(h, r_alpha, D.r_alpha))) goto done;
         if ((rc = put_raw<double>(h, r_fw, D.r_fw))) goto done;
@@ -2778,7 +3013,7 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         if (cus < 1) cus = 1;
         const int want = (h->nchunks + (ROUTE_BLOCK / WAVE) - 1) / (ROUTE_BLOCK / WAVE);
         h->route_grid = std::max(1, std::min(want, per_cu * cus));
-        if (adaptive) {
+        if (adaptive || fp) {
             // river positions: river cells sorted by node, a topological order of the river graph
             // because every river edge is a land edge and node order is topological
             AdArgs& X = h->ad;
@@ -2814,6 +3049,20 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
             if ((rc = put_raw<int>(h, rp_down, X.rp_down))) goto done;
             if ((rc = put_raw<double>(h, rp_res, X.rp_up_res))) goto done;
             if ((rc = put_raw<int>(h, rp_node, X.rp_node))) goto done;
+            {  // floodplain accumulation: upstream positions ordered by the river topological order
+                std::vector<int64_t> opos(nriv);
+                for (int64_t k = 0; k < nriv; ++k) opos[h->river_order[k]] = k;
+                std::vector<int> sp(nriv + 1, 0), si;
+                for (int64_t k = 0; k < nriv && fp; ++k) {
+                    std::vector<int>& u = ups[rp_r[k]];
+                    std::vector<int> v(u.begin(), u.end());
+                    std::sort(v.begin(), v.end(), [&](int a_, int b_) { return opos[a_] < opos[b_]; });
+                    for (int w : v) si.push_back(rpos[w]);
+                    sp[k + 1] = (int)si.size();
+                }
+                if ((rc = put_raw<int>(h, sp, X.fp_sup_ptr))) goto done;
+                if ((rc = put_raw<int>(h, si, X.fp_sup_idx))) goto done;
+            }
             X.nchunks_r = (int)((nriv + WAVE - 1) / WAVE);
             hipError_t ea = hipSuccess, eb;
             X.szi = dalloc<double>(h, n, eb); if (eb != hipSuccess) ea = eb;
@@ -2974,7 +3223,10 @@ static int32_t launch_adaptive(wfh_handle* h) {
         int k = 0;
         while (t < dt) {
             double dts;
-            if (fam == 0)
+            rc = WFH_OK;
+            if (!h->cfg.kinematic_wave_adaptive)  // fixed sub-steps (a floodplain model)
+                dts = fam == 0 ? h->cfg.subsurface_dt : fam == 1 ? h->cfg.land_dt : h->cfg.river_dt;
+            else if (fam == 0)
                 rc = ssf_stable_timestep(h, dts);
             else if (fam == 1)
                 rc = kw_stable_timestep(h, h->d.ol_q, h->d.alpha, h->d.fl, n, 0.02, dts);
@@ -2993,17 +3245,28 @@ static int32_t launch_adaptive(wfh_handle* h) {
             else if (fam == 1)
                 hipLaunchKernelGGL(k_ad_ol, dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(), h->ad,
                                    dts, first, last);
-            else if (nriv > 0 && h->nres > 0)
-                hipLaunchKernelGGL(k_ad_riv<true>, dim3(grid_r), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(), h->ad,
-                                   dts, first, last);
-            else if (nriv > 0)
-                hipLaunchKernelGGL(k_ad_riv<false>, dim3(grid_r), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(), h->ad,
-                                   dts, first, last);
+            else if (nriv > 0) {
+                const unsigned gr1 = (unsigned)((nriv + 255) / 256);
+                if (h->d.fp) hipLaunchKernelGGL(k_fp_exchange, dim3(gr1), dim3(256), 0, h->stream, h->d, dts, first);
+                if (h->nres > 0)
+                    hipLaunchKernelGGL(k_ad_riv<true>, dim3(grid_r), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(),
+                                       h->ad, dts, first, last);
+                else
+                    hipLaunchKernelGGL(k_ad_riv<false>, dim3(grid_r), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(),
+                                       h->ad, dts, first, last);
+                if (h->d.fp) {
+                    hipLaunchKernelGGL(k_fp_accu, dim3(grid_r), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(), h->ad,
+                                       dts);
+                    hipLaunchKernelGGL(k_fp_qin, dim3(gr1), dim3(256), 0, h->stream, h->d, h->ad, dts);
+                }
+            }
             HIPCHK(h, hipGetLastError());
             t = tn;
         }
         h->nsub[fam] = k;
     }
+    if (h->d.fp && nriv > 0)
+        hipLaunchKernelGGL(k_fp_finish, dim3((unsigned)((nriv + 255) / 256)), dim3(256), 0, h->stream, h->d);
     hipLaunchKernelGGL(k_ad_total, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d);
     if (ev) hipEventRecord(ev[2], h->stream);
     HIPCHK(h, hipGetLastError());
@@ -3056,7 +3319,7 @@ extern "C" int32_t wfh_step(wfh_handle* h) {
     if (h->broken) return fail(h, WFH_ERR_STATE, "handle is in an error state (a previous step failed)");
     if (!h->forcing_set) return fail(h, WFH_ERR_STATE, "wfh_set_forcing must be called before wfh_step");
     HIPCHK(h, hipSetDevice(h->cfg.device));
-    if (h->cfg.kinematic_wave_adaptive) {
+    if (h->cfg.kinematic_wave_adaptive || h->d.fp) {  // one launch per sub-step
         switch (h->N) {
             case 2: return launch_adaptive<2>(h);
             case 3: return launch_adaptive<3>(h);
@@ -3186,6 +3449,12 @@ VarInfo var_info(wfh_handle* h, int32_t v) {
         case WFH_BALANCE_RESERVOIR_ERROR: return {D.bal_res, 1, false, false, true};
         case WFH_SNOW_IN: return {D.snow_in, 1, false, false};
         case WFH_SNOW_OUT: return {D.snow_out, 1, false, false};
+        case WFH_FLOODPLAIN_STORAGE: return {D.fp_storage, 1, true, true};
+        case WFH_FLOODPLAIN_H: return {D.fp_h, 1, true, false};
+        case WFH_FLOODPLAIN_Q: return {D.fp_q, 1, true, false};
+        case WFH_FLOODPLAIN_Q_AVERAGE: return {D.fp_q_avg, 1, true, false};
+        case WFH_FLOODPLAIN_QIN_AVERAGE: return {D.fp_qin_avg, 1, true, false};
+        case WFH_RIVER_Q_CHANNEL_AVERAGE: return {D.r_qch_avg, 1, true, false};
         default: return {nullptr, 0, false, false};
     }
 }
@@ -3369,7 +3638,7 @@ static std::vector<std::pair<double*, size_t>> state_blocks(wfh_handle* h) {
     return {{D.canopy, n},     {D.snow_s, n}, {D.snow_w, n}, {D.tsoil, n},          {D.swd, n},
             {D.ustore, n * h->N}, {D.tsws, n}, {D.ssf_q, n}, {D.ol_q, n},           {D.ol_h, n},
             {D.ol_storage, n}, {D.r_q, r},     {D.r_h, r},      {D.r_storage, r},
-            {D.res_storage, nr}, {D.res_wl, nr}, {D.res_outflow, nr}};
+            {D.res_storage, nr}, {D.res_wl, nr}, {D.res_outflow, nr}, {D.fp_storage, D.fp ? r : 0}};
 }
 
 extern "C" int64_t wfh_state_size(const wfh_handle* h) {

@@ -44,6 +44,7 @@ class WfhConfig(ctypes.Structure):
         ("kinematic_wave_adaptive", ctypes.c_int32),
         ("subsurface_alpha_coefficient", ctypes.c_double),
         ("snow_gravitational_transport", ctypes.c_int32),
+        ("floodplain_1d", ctypes.c_int32),
     ]
 
 
@@ -72,6 +73,8 @@ STATIC_FIELDS = [
     ("reservoir_storage_curve_type", _I32), ("reservoir_rating_curve_exponent", _D),
     ("reservoir_sh_ptr", _I64), ("reservoir_sh_h", _D), ("reservoir_sh_s", _D),
     ("reservoir_hq_ptr", _I64), ("reservoir_hq_h", _D), ("reservoir_hq_q", _D),
+    ("floodplain_ndepth", ctypes.c_int32), ("floodplain_depth", _D), ("floodplain_volume", _D),
+    ("floodplain_manning_n", _D), ("floodplain_slope", _D),
 ]
 
 
@@ -151,19 +154,29 @@ class Var(enum.IntEnum):
     BALANCE_RESERVOIR_ERROR = enum.auto()
     SNOW_IN = enum.auto()
     SNOW_OUT = enum.auto()
+    FLOODPLAIN_STORAGE = enum.auto()
+    FLOODPLAIN_H = enum.auto()
+    FLOODPLAIN_Q = enum.auto()
+    FLOODPLAIN_Q_AVERAGE = enum.auto()
+    FLOODPLAIN_QIN_AVERAGE = enum.auto()
+    RIVER_Q_CHANNEL_AVERAGE = enum.auto()
     VAR_COUNT = enum.auto()
 
 
 LAYERED_VARS = {Var.USTORELAYERDEPTH, Var.VWC}
 RIVER_VARS = {Var.RIVER_Q, Var.RIVER_QIN, Var.RIVER_H, Var.RIVER_STORAGE, Var.RIVER_Q_AVERAGE,
-              Var.RIVER_QIN_AVERAGE, Var.RIVER_INWATER, Var.BALANCE_RIVER_ERROR, Var.RIVER_ALPHA}
+              Var.RIVER_QIN_AVERAGE, Var.RIVER_INWATER, Var.BALANCE_RIVER_ERROR, Var.RIVER_ALPHA,
+              Var.FLOODPLAIN_STORAGE, Var.FLOODPLAIN_H, Var.FLOODPLAIN_Q, Var.FLOODPLAIN_Q_AVERAGE,
+              Var.FLOODPLAIN_QIN_AVERAGE, Var.RIVER_Q_CHANNEL_AVERAGE}
+FLOODPLAIN_VARS = [Var.FLOODPLAIN_STORAGE, Var.FLOODPLAIN_H, Var.FLOODPLAIN_Q, Var.FLOODPLAIN_Q_AVERAGE,
+                   Var.FLOODPLAIN_QIN_AVERAGE, Var.RIVER_Q_CHANNEL_AVERAGE]
 RESERVOIR_VARS = {Var.RESERVOIR_STORAGE, Var.RESERVOIR_WATERLEVEL, Var.RESERVOIR_OUTFLOW,
                   Var.RESERVOIR_OUTFLOW_AVERAGE, Var.RESERVOIR_INFLOW_AVERAGE, Var.RESERVOIR_ACTEVAP_AVERAGE,
                   Var.RESERVOIR_ABSTRACTION_AVERAGE, Var.RESERVOIR_PRECIPITATION, Var.RESERVOIR_EVAPORATION,
                   Var.BALANCE_RESERVOIR_ERROR}
 SETTABLE_VARS = {Var.CANOPY_STORAGE, Var.SNOW_STORAGE, Var.SNOW_WATER, Var.SOIL_TEMPERATURE, Var.SATWATERDEPTH,
                  Var.USTORELAYERDEPTH, Var.SSF_Q, Var.LAND_Q, Var.LAND_H, Var.RIVER_Q, Var.RIVER_H,
-                 Var.RESERVOIR_STORAGE, Var.RESERVOIR_WATERLEVEL}
+                 Var.RESERVOIR_STORAGE, Var.RESERVOIR_WATERLEVEL, Var.FLOODPLAIN_STORAGE}
 
 # entry points declared in include/wflow_hip.h (checked against the built library)
 EXPORTS = ["wfh_create", "wfh_set_forcing", "wfh_step", "wfh_get", "wfh_set", "wfh_get_topology",

@@ -103,7 +103,8 @@ class Catchment:
         return 0 if a is None else int(a.shape[0])
 
     def config(self, water_mass_balance=True, device=0, land_dt=3600.0, river_dt=900.0, subsurface_dt=86400.0,
-               adaptive=False, subsurface_alpha=1.0, snow_transport=False):
+               adaptive=False, subsurface_alpha=1.0, snow_transport=False, floodplain=None):
+        """floodplain=None: on when the catchment has floodplain profiles (add_floodplain)."""
         c = _abi.WfhConfig()
         c.abi_version = _abi.WFH_ABI_VERSION
         c.device = device
@@ -120,6 +121,7 @@ class Catchment:
         c.kinematic_wave_adaptive = int(adaptive)
         c.subsurface_alpha_coefficient = subsurface_alpha
         c.snow_gravitational_transport = int(snow_transport)
+        c.floodplain_1d = int("floodplain_volume" in self.arrays if floodplain is None else floodplain)
         return c
 
     def static(self):
@@ -127,8 +129,10 @@ class Catchment:
         s = _abi.WfhStatic()
         s.nx, s.ny, s.n, s.nriv = self.nx, self.ny, self.n, self.nriv
         s.nres = self.nres
+        fd = self.arrays.get("floodplain_depth")
+        s.floodplain_ndepth = 0 if fd is None else int(fd.size)
         for name, ctype in _abi.STATIC_FIELDS[4:]:
-            if name == "nres":
+            if name in ("nres", "floodplain_ndepth"):
                 continue
             a = self.arrays.get(name)
             if a is None:
@@ -401,6 +405,35 @@ def add_reservoirs(catchment, count, seed=0, radius=1, curves=False):
     A["reservoir_outflow_obs"] = obs
     if curves:
         _add_reservoir_curves(c, rng)
+    return c
+
+
+def add_floodplain(catchment, seed=0, depths=(0.5, 1.0, 2.0, 3.5, 5.0), bankfull_scale=0.02):
+    """Floodplain profiles for every river cell (in place): cumulative flood volumes at `depths`
+    of a floodplain widening with depth (5 to 60 times the river width), with a narrowing step
+    in every seventh cell (which the profile construction widens back); floodplain Manning n and
+    slope. The bankfull depths are scaled by `bankfull_scale` so that the synthetic rivers leave
+    their channels."""
+    c = catchment
+    A = c.arrays
+    rng = np.random.default_rng(seed + 777)
+    nriv = int(A["river_width"].size)
+    d = np.asarray(depths, np.float64)
+    h = np.diff(np.concatenate([[0.0], d]))
+    fw = A["river_width"]
+    fl = A["river_length"]
+    base = fw[None, :] * rng.uniform(5.0, 20.0, nriv)[None, :]
+    grow = np.cumprod(rng.uniform(1.0, 1.6, (d.size, nriv)), axis=0)
+    width = base * grow
+    width[2, ::7] *= 0.7  # a narrowing segment
+    vol = np.cumsum(width * h[:, None] * fl[None, :], axis=0)
+    A["floodplain_depth"] = d
+    A["floodplain_volume"] = np.ascontiguousarray(vol)
+    A["floodplain_manning_n"] = rng.uniform(0.05, 0.1, nriv)
+    slope = A["river_slope"].copy()
+    slope[::11] = 0.0  # clamped to 1e-5
+    A["floodplain_slope"] = slope
+    A["river_bankfull_depth"] = np.ascontiguousarray(A["river_bankfull_depth"] * bankfull_scale)
     return c
 
 
