@@ -43,6 +43,11 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reservoirs", type=int, default=0,
                     help="place this many reservoirs on the grid's river cells (simple / modified Puls)")
+    ap.add_argument("--reservoir-curves", action="store_true",
+                    help="with --reservoirs: cycle the outflow types 1-4 (rating curves, free weirs) and give every "
+                         "third reservoir an S-H storage table")
+    ap.add_argument("--floodplain", action="store_true",
+                    help="1-D floodplain beside every river cell (floodplain_1d); one launch per river sub-step")
     ap.add_argument("--snow-transport", action="store_true", help="lateral snow transport (gravitational)")
     ap.add_argument("--adaptive", action="store_true",
                     help="adaptive kinematic-wave sub-steps (kinematic_wave__adaptive_time_step_flag) instead of "
@@ -160,8 +165,12 @@ def main():
     if args.adaptive:
         workload = workload.replace("(land 1, river 4, subsurface 1 sub-steps)", "(adaptive sub-steps)")
     if args.reservoirs:
-        synth.add_reservoirs(c, args.reservoirs, seed=1, radius=2)
-        workload += f", {args.reservoirs} reservoirs"
+        synth.add_reservoirs(c, args.reservoirs, seed=1, radius=2, curves=args.reservoir_curves)
+        workload += f", {args.reservoirs} reservoirs" + (" (outflow types 1-4, S-H tables)" if args.reservoir_curves
+                                                          else "")
+    if args.floodplain:
+        synth.add_floodplain(c, seed=1)
+        workload += ", 1-D floodplain (5 flood depths, bankfull depth x 0.02)"
     if args.snow_transport:
         workload += ", lateral snow transport"
     m = Model(c, water_mass_balance=True, adaptive=args.adaptive, snow_transport=args.snow_transport)
@@ -187,7 +196,7 @@ def main():
     subs = []
     for k in range(args.warmup, args.warmup + args.steps):
         step(k)
-        if args.adaptive:
+        if args.adaptive or args.floodplain:
             subs.append(m.last_substeps())
     m.sync()
     torch.cuda.synchronize()
@@ -215,8 +224,8 @@ def main():
               "cell-timesteps/s FP64 SBM+kinwave (batched GPU step)")
     if args.adaptive:
         metric = metric.replace(" step)", " step, adaptive sub-steps)")
-    if args.reservoirs or args.snow_transport:
-        metric = metric.replace(" step", " step with reservoirs / snow transport", 1)
+    if args.reservoirs or args.snow_transport or args.floodplain:
+        metric = metric.replace(" step", " step with reservoirs / floodplain / snow transport", 1)
     result = {
         "metric": metric,
         "value": c.n * args.steps / elapsed,
@@ -239,7 +248,7 @@ def main():
     }
     # FP64 VALU rate: FLOPs per launch from the counter run of the same workload over this run's
     # live kernel times
-    if os.path.exists(args.fp64_file) and not args.adaptive:
+    if os.path.exists(args.fp64_file) and not args.adaptive and not args.floodplain:
         try:
             ff = json.load(open(args.fp64_file))
             if ff.get("n") == c.n:
@@ -249,7 +258,7 @@ def main():
                     for k, ms in (("k_route", k2_ms), ("k_vertical", k1_ms))}
         except Exception:
             pass
-    if args.adaptive:
+    if args.adaptive or args.floodplain:
         # the routing time covers all sub-step launches and the host read-backs between them
         result["roofline"] = None
         result["config"]["kernel_ms"] = {"k_vertical": k1_ms, "routing": k2_ms}
@@ -265,7 +274,7 @@ def main():
         torch.cuda.synchronize()
         print(f"# calibration kernel (neg): {nb} bytes read, {nb} bytes written", file=sys.stderr, flush=True)
         del src, dst
-    if not args.no_cpu_baseline and not args.reservoirs:  # basin samples carry no reservoirs
+    if not args.no_cpu_baseline and not args.reservoirs and not args.floodplain:  # basin samples carry neither
         frames_host = []
         inv = np.empty_like(node)
         inv[node] = np.arange(node.size)
