@@ -98,7 +98,10 @@ def cpu_baseline(catchment, frames_host, args):
         sizes[g] += counts[k]
     groups = [g for g in groups if g]
     masks = [np.isin(lab, np.array(g)) for g in groups]
-    models = [O.OracleModel(catchment.subset(mk), adaptive=args.adaptive) for mk in masks]
+    kw = dict(adaptive=args.adaptive, snow_transport=args.snow_transport)
+    subs = [catchment.subset(mk) for mk in masks]
+    mdl_nres = [sc.nres for sc in subs]
+    models = [O.OracleModel(sc, **kw) for sc in subs]
     frames = [[tuple(f[mk] for f in frames_host[k % len(frames_host)]) for k in range(args.cpu_sample_steps)]
               for mk in masks]
 
@@ -111,7 +114,7 @@ def cpu_baseline(catchment, frames_host, args):
     run(0)
     t_single = time.perf_counter() - t0
     models[0].close()
-    models[0] = O.OracleModel(catchment.subset(masks[0]), adaptive=args.adaptive)
+    models[0] = O.OracleModel(catchment.subset(masks[0]), **kw)
     ths = [threading.Thread(target=run, args=(j,)) for j in range(len(models))]
     t0 = time.perf_counter()
     for t in ths:
@@ -128,8 +131,9 @@ def cpu_baseline(catchment, frames_host, args):
         "cores": len(models),
         "kind": "port",
         "single_thread_value": int(masks[0].sum()) * args.cpu_sample_steps / t_single,
-        "sample": f"{len(take)} complete basins ({cells} cells) of the same workload in {len(models)} groups, one "
-                  f"thread each, {args.cpu_sample_steps} steps, C oracle (-O2), update_model! only"
+        "sample": f"{len(take)} complete basins ({cells} cells, {sum(mdl_nres)} reservoirs) of the same workload in "
+                  f"{len(models)} groups, one thread each, {args.cpu_sample_steps} steps, C oracle (-O2), "
+                  f"update_model! only"
                   + (" (adaptive sub-steps of each group itself)" if args.adaptive else ""),
     }
 
@@ -274,7 +278,7 @@ def main():
         torch.cuda.synchronize()
         print(f"# calibration kernel (neg): {nb} bytes read, {nb} bytes written", file=sys.stderr, flush=True)
         del src, dst
-    if not args.no_cpu_baseline and not args.reservoirs and not args.floodplain:  # basin samples carry neither
+    if not args.no_cpu_baseline:
         frames_host = []
         inv = np.empty_like(node)
         inv[node] = np.arange(node.size)

@@ -282,3 +282,28 @@ def test_bad_tables_rejected(what, c):
     h = ctypes.c_void_p()
     assert lib.wfh_create(ctypes.byref(cfg), ctypes.byref(st), ctypes.byref(h)) == _abi.WFH_ERR_ARG, what
     assert "reservoir" in lib.wfh_last_error(None).decode()
+
+
+def test_subset_carries_reservoirs_and_floodplain():
+    """A basin subset (the CPU baseline's sample) keeps the reservoirs whose outlets it holds,
+    with their tables, and the floodplain profiles of its river cells: basins exchange no water,
+    so the subset model reproduces the full model on those basins exactly."""
+    c = synth.make_catchment(96, cell=250.0, dt=3600.0, river_threshold=60, seed=51)
+    synth.add_reservoirs(c, 8, seed=5, curves=True)
+    synth.add_floodplain(c, seed=1)
+    lab = synth.basin_ids(c.arrays["ldd"], c.nx, c.ny, c.arrays["cell_index"])
+    ids, cnt = np.unique(lab, return_counts=True)
+    mk = np.isin(lab, ids[np.argsort(-cnt)[:3]])
+    sub = c.subset(mk)
+    assert sub.nres == 8
+    full, part = O.OracleModel(c), O.OracleModel(sub)
+    riv = c.arrays["river_location"].astype(bool)
+    for k in range(3):
+        P, E, T = synth.forcing_numpy(c, k)
+        full.step(P, E, T)
+        part.step(P[mk], E[mk], T[mk])
+        np.testing.assert_array_equal(full.get(Var.RIVER_Q)[mk[riv]], part.get(Var.RIVER_Q))
+        np.testing.assert_array_equal(full.get(Var.FLOODPLAIN_STORAGE)[mk[riv]], part.get(Var.FLOODPLAIN_STORAGE))
+        np.testing.assert_array_equal(full.get(Var.RESERVOIR_STORAGE), part.get(Var.RESERVOIR_STORAGE))
+    full.close()
+    part.close()

@@ -144,14 +144,27 @@ class Catchment:
 
     def subset(self, mask):
         """The same maps restricted to the cells where mask (land order) is true; used to run
-        complete basins on their own (basins exchange no water)."""
+        complete basins on their own (basins exchange no water). Reservoirs whose outlet is kept
+        come along (their area restricted to the kept cells), and so do the floodplain profiles
+        of the kept river cells."""
         mask = np.asarray(mask, bool)
         out = Catchment(self.nx, self.ny, self.cell, self.dt, {}, self.nlayers_cfg, self.layers_mm, self.snow,
                         self.infiltration_reduction)
         riv = self.arrays["river_location"].astype(bool)
         rmask = mask[riv]
         n = self.n
+        A = self.arrays
+        if "floodplain_volume" in A:
+            nd = A["floodplain_depth"].size
+            out.arrays["floodplain_depth"] = A["floodplain_depth"].copy()
+            out.arrays["floodplain_volume"] = np.ascontiguousarray(A["floodplain_volume"].reshape(nd, -1)[:, rmask])
+            for k in ("floodplain_manning_n", "floodplain_slope"):
+                out.arrays[k] = np.ascontiguousarray(A[k][rmask])
+        if "reservoir_outlet" in A:
+            self._subset_reservoirs(out, mask)
         for k, a in self.arrays.items():
+            if k.startswith("reservoir_") or k.startswith("floodplain_"):
+                continue
             if k.startswith("river_") and k not in ("river_location", "river_width_land", "river_length_land"):
                 out.arrays[k] = np.ascontiguousarray(a[rmask])
             elif a.ndim == 1 and a.shape[0] == n:
@@ -161,6 +174,38 @@ class Catchment:
             else:
                 raise ValueError(k)
         return out
+
+    def _subset_reservoirs(self, out, mask):
+        A, B = self.arrays, out.arrays
+        outlet = A["reservoir_outlet"]
+        nres = outlet.size
+        keep = np.nonzero(mask[outlet])[0]
+        new_index = np.cumsum(mask) - 1
+        B["reservoir_outlet"] = np.ascontiguousarray(new_index[outlet[keep]].astype(np.int64))
+        remap = np.zeros(nres + 1, np.int32)
+        remap[keep + 1] = np.arange(1, keep.size + 1, dtype=np.int32)
+        B["reservoir_coverage"] = np.ascontiguousarray(remap[A["reservoir_coverage"][mask]])
+        for k, a in A.items():
+            if k.startswith("reservoir_") and k not in ("reservoir_outlet", "reservoir_coverage") and \
+                    not k.startswith(("reservoir_sh_", "reservoir_hq_")):
+                assert a.shape[0] == nres, k
+                B[k] = np.ascontiguousarray(a[keep])
+        for tab, cols in (("sh", ("h", "s")), ("hq", ("h", "q"))):
+            ptr = A.get(f"reservoir_{tab}_ptr")
+            if ptr is None:
+                continue
+            width = {"h": 1, "s": 1, "q": 365}
+            new_ptr, parts = [0], {c: [] for c in cols}
+            for r in keep:
+                a, b = int(ptr[r]), int(ptr[r + 1])
+                for c in cols:
+                    w = width[c]
+                    parts[c].append(A[f"reservoir_{tab}_{c}"][a * w:b * w])
+                new_ptr.append(new_ptr[-1] + b - a)
+            B[f"reservoir_{tab}_ptr"] = np.array(new_ptr, np.int64)
+            for c in cols:
+                v = np.concatenate(parts[c]) if parts[c] else np.zeros(0)
+                B[f"reservoir_{tab}_{c}"] = np.ascontiguousarray(v if v.size else np.zeros(width[c]))
 
 
 def make_catchment(nx, ny=None, cell=1000.0, dt=86400.0, river_threshold=100, seed=0, snow=True,
