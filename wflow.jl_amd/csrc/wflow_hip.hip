@@ -1456,9 +1456,95 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_ol(Dev d, RouteArgs a, AdArg
     }
 }
 
+// ============================================================================================
+// 1-D floodplain of the kinematic-wave river (floodplain_1d). One launch per river sub-step does
+// the channel-floodplain exchange of each cell, the river sub-step and the floodplain
+// accumulation (k_ad_riv<RES, true>); k_fp_qin follows, k_fp_finish after the last sub-step.
+// ============================================================================================
+// interpolation_indices  floodplain.jl:272-284, 0-based: the last index with v[i] <= x (the
+// first when none), and the next one except at the end
+__device__ __forceinline__ void fp_indices(const double x, const double* v, const int n, int& i1, int& i2) {
+    int a = 0;
+    for (int i = 0; i < n; ++i)
+        if (v[i] <= x) a = i;
+    i1 = a;
+    i2 = a == n - 1 ? a : a + 1;
+}
+
+// compute_floodplain_flow_area  floodplain.jl:330-341 (compute_flood_flow_area :286-300) on the
+// width (W) and flow area (A) rows of a cell's profile
+__device__ __forceinline__ double fp_flow_area(const double* W, const double* A, const double* depth, const double h,
+                                               const int i1, const int i2) {
+    const double channel_area = W[0] * h;
+    const double delta_h = h - depth[i1];
+    const double flood_flow_area = A[i1] + (W[i2] * delta_h);
+    return wfh_max(flood_flow_area - channel_area, 0.0);
+}
+
+// river_channel_floodplain_exchange!  surface_kinwave.jl:572-601 for river cell r: the channel
+// storage is the previous sub-step's (the previous step's at the first sub-step). Returns the
+// floodplain storage and depth after the exchange and the exchange flux (the river's extra
+// lateral inflow times the flow length).
+__device__ __forceinline__ void fp_exchange(const Dev& d, const int r, const bool first, const double dts, double& fs,
+                                            double& fh, double& exch) {
+    const int nd = d.fp_nd;
+    const double rs = first ? d.r_storage[r] : d.fp_rs[r];
+    fs = d.fp_storage[r];
+    const double storage_total = rs + fs;
+    double delta_river_storage;
+    if (storage_total > d.r_bfs[r]) {
+        const double flood_storage = storage_total - d.r_bfs[r];
+        const double* S = d.fp_prof + (size_t)r * 4 * nd;
+        const double* W = S + nd;
+        const double rfl = d.r_fl[r];
+        int i1, i2;
+        fp_indices(flood_storage, S, nd, i1, i2);  // compute_flood_depth  floodplain.jl:316-327
+        const double delta_A = (flood_storage - S[i1]) / rfl;
+        const double delta_h = delta_A / W[i2];
+        const double h = d.fp_depth[i1] + delta_h;
+        const double river_storage = (d.r_bfd[r] + h) * d.r_fw[r] * rfl;
+        delta_river_storage = river_storage - rs;
+        fs = wfh_max(storage_total - river_storage, 0.0);
+        fh = fs > 0.0 ? h : 0.0;
+    } else {
+        delta_river_storage = wfh_max(storage_total - rs, 0.0);
+        fh = 0.0;
+        fs = 0.0;
+    }
+    exch = delta_river_storage / dts;
+}
+
+// The Manning flow capacity of update_floodplain_model! (surface_kinwave.jl:395-425) for river
+// cell r at flood depth h: zero unless this and the downstream cell's floodplain flow areas at
+// this depth exceed 1e-5 m2; rd = the downstream river cell or -1.
+__device__ __forceinline__ double fp_capacity(const Dev& d, const int r, const int rd, const double h) {
+    if (!(h > 0.0)) return 0.0;
+    const int nd = d.fp_nd;
+    int i1, i2;
+    fp_indices(h, d.fp_depth, nd, i1, i2);
+    const double* Wv = d.fp_prof + ((size_t)r * 4 + 1) * nd;
+    const double flow_area = fp_flow_area(Wv, Wv + nd, d.fp_depth, h, i1, i2);
+    double flow_area_ds = flow_area;
+    if (rd >= 0) {
+        const double* Wd = d.fp_prof + ((size_t)rd * 4 + 1) * nd;
+        flow_area_ds = fp_flow_area(Wd, Wd + nd, d.fp_depth, h, i1, i2);
+    }
+    if (!(flow_area > 1.0e-05 && flow_area_ds > 1.0e-05)) return 0.0;
+    const double wetted_perimeter = Wv[2 * nd + i1] + 2.0 * (h - d.fp_depth[i1]);
+    const double hydraulic_radius = flow_area / wetted_perimeter;
+    // manning_flow  surface_process.jl:166-169
+    return wfh_cbrt(hydraulic_radius * hydraulic_radius) * sqrt(d.fp_slope[r]) * flow_area / d.fp_n[r];
+}
+
 // One river sub-step of length dts, over the river cells in river-position order (river cells
 // sorted by node, a topological order of the river graph).
-template <bool RES>
+// FP (a floodplain model): each lane first does its cell's channel-floodplain exchange, which
+// enters the river sub-step as lateral inflow; after the river family the wave runs the
+// floodplain accumulation (accucapacityflux! of update_floodplain_model!, routing/utils.jl:82-108)
+// down the unfiltered river graph as a second family: a cell adds what its upstream cells pass
+// on, in the reference's river topological order, to its floodplain storage, passes on
+// min(storage / dt, capacity) and keeps the rest.
+template <bool RES, bool FP>
 __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_riv(Dev d, RouteArgs a, AdArgs x, double dts, int first,
                                                         int last) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -1487,9 +1573,11 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_riv(Dev d, RouteArgs a, AdAr
             r_qin_cum = x.rcum[(size_t)nr + q_];
         }
         const double rqlat = r_inwater / rfl;
+        double fs = 0.0, fh = 0.0, fexch = 0.0;
+        if (FP) fp_exchange(d, r, first != 0, dts, fs, fh, fexch);
         double inflow = 0.0 / rfl;  // reservoir / lake / external inflow terms (none in this model)
         inflow -= 0.0 / rfl;
-        if (d.fp) inflow += d.fp_exch[r] / rfl;  // channel-floodplain exchange (surface_kinwave.jl:534-536)
+        if (FP) inflow += fexch / rfl;  // channel-floodplain exchange (surface_kinwave.jl:534-536)
         const double rqlat_all = rqlat + inflow;
         double r_storage = 0.0, rqin = 0.0;
         const KwPrep rpp = kw_prepare(rq, rqlat_all, ralpha, dts, rfl);
@@ -1513,7 +1601,34 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_riv(Dev d, RouteArgs a, AdAr
                                   (size_t)nr, rp, rdn >= 0 && !cons_in, cons_in ? (rdn & 63) : -1, lds_k, lds_q,
                                   riv_compute))
             return;
-        if (valid && d.fp) d.fp_rs[r] = r_storage;
+        if (FP) {
+            const double cap = fp_capacity(d, r, rdn_ >= 0 ? x.rp_r[rdn_] : -1, fh);
+            double s_new = fs, flux = 0.0;
+            auto pass = [&](int, double material, double) -> double {
+                const double flux_val = wfh_min(material / dts, cap);
+                const double update = flux_val * dts;
+                s_new = material - update;
+                flux = flux_val;
+                return update;
+            };
+            if (!run_family<FOLD_SNOW>(a, lane, c, valid, 1, x.fp_sup_idx, x.fp_sup_idx, nullptr, x.fp_sup_ptr[q_],
+                                       valid ? x.fp_sup_ptr[q_ + 1] : x.fp_sup_ptr[q_], a.slot_ssf, (size_t)nr, rp,
+                                       rdn >= 0 && !cons_in, cons_in ? (rdn & 63) : -1, lds_k, lds_q, pass, fs))
+                return;
+            if (valid) {
+                if (first) {
+                    d.fp_q_cum[r] = 0.0;
+                    d.fp_qin_cum[r] = 0.0;
+                    if (d.balance) d.fp_sprev[r] = d.r_storage[r] + d.fp_storage[r];  // get_storage  mass_balance.jl:132-140
+                }
+                d.fp_rs[r] = r_storage;
+                d.fp_h[r] = fh;
+                d.fp_exch[r] = fexch;
+                d.fp_storage[r] = s_new;
+                d.fp_q[r] = flux;
+                d.fp_q_cum[r] = (first ? 0.0 : d.fp_q_cum[r]) + flux * dts;
+            }
+        }
         if (valid && !last) {
             d.r_q[r] = rq;
             d.r_h[r] = rh;
@@ -1521,132 +1636,6 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_riv(Dev d, RouteArgs a, AdAr
             x.rcum[(size_t)nr + rp] = r_qin_cum;
         } else if (valid) {
             riv_finish(d, r, rq, rh, r_storage, rqin, r_q_cum, r_qin_cum, r_inwater, d.r_storage[r]);
-        }
-    }
-}
-
-// ============================================================================================
-// 1-D floodplain of the kinematic-wave river (floodplain_1d); per river sub-step:
-// k_fp_exchange, the river sub-step (k_ad_riv), k_fp_accu, k_fp_qin; k_fp_finish after the last.
-// ============================================================================================
-// interpolation_indices  floodplain.jl:272-284, 0-based: the last index with v[i] <= x (the
-// first when none), and the next one except at the end
-__device__ __forceinline__ void fp_indices(const double x, const double* v, const int n, int& i1, int& i2) {
-    int a = 0;
-    for (int i = 0; i < n; ++i)
-        if (v[i] <= x) a = i;
-    i1 = a;
-    i2 = a == n - 1 ? a : a + 1;
-}
-
-// compute_floodplain_flow_area  floodplain.jl:330-341 (compute_flood_flow_area :286-300) on the
-// width (W) and flow area (A) rows of a cell's profile
-__device__ __forceinline__ double fp_flow_area(const double* W, const double* A, const double* depth, const double h,
-                                               const int i1, const int i2) {
-    const double channel_area = W[0] * h;
-    const double delta_h = h - depth[i1];
-    const double flood_flow_area = A[i1] + (W[i2] * delta_h);
-    return wfh_max(flood_flow_area - channel_area, 0.0);
-}
-
-// river_channel_floodplain_exchange!  surface_kinwave.jl:572-601, one thread per river cell; the
-// channel storage is the previous sub-step's (the previous step's at the first sub-step)
-__global__ void k_fp_exchange(Dev d, double dts, int first) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= d.nriv) return;
-    const int nd = d.fp_nd;
-    const double rs = first ? d.r_storage[r] : d.fp_rs[r];
-    double fs = d.fp_storage[r];
-    if (first) {
-        d.fp_q_cum[r] = 0.0;
-        d.fp_qin_cum[r] = 0.0;
-        if (d.balance) d.fp_sprev[r] = rs + fs;  // get_storage  mass_balance.jl:132-140
-    }
-    const double storage_total = rs + fs;
-    double delta_river_storage, fh;
-    if (storage_total > d.r_bfs[r]) {
-        const double flood_storage = storage_total - d.r_bfs[r];
-        const double* S = d.fp_prof + (size_t)r * 4 * nd;
-        const double* W = S + nd;
-        const double rfl = d.r_fl[r];
-        int i1, i2;
-        fp_indices(flood_storage, S, nd, i1, i2);  // compute_flood_depth  floodplain.jl:316-327
-        const double delta_A = (flood_storage - S[i1]) / rfl;
-        const double delta_h = delta_A / W[i2];
-        const double h = d.fp_depth[i1] + delta_h;
-        const double river_storage = (d.r_bfd[r] + h) * d.r_fw[r] * rfl;
-        delta_river_storage = river_storage - rs;
-        fs = wfh_max(storage_total - river_storage, 0.0);
-        fh = fs > 0.0 ? h : 0.0;
-    } else {
-        delta_river_storage = wfh_max(storage_total - rs, 0.0);
-        fh = 0.0;
-        fs = 0.0;
-    }
-    d.fp_storage[r] = fs;
-    d.fp_h[r] = fh;
-    d.fp_exch[r] = delta_river_storage / dts;
-}
-
-// update_floodplain_model!  surface_kinwave.jl:387-430: the Manning flow capacity of each cell
-// (zero unless this and the downstream cell's floodplain flow areas at this cell's depth exceed
-// 1e-5 m2), then accucapacityflux! of the floodplain storage down the unfiltered river graph
-// (routing/utils.jl:82-108) as a dataflow over river positions: a cell adds what its upstream
-// cells pass on, in the reference's topological order, to its storage, passes on
-// min(storage / dt, capacity) and keeps the rest.
-__global__ void __launch_bounds__(ROUTE_BLOCK) k_fp_accu(Dev d, RouteArgs a, AdArgs x, double dts) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    __shared__ int x_k[ROUTE_BLOCK];
-    __shared__ double x_q[ROUTE_BLOCK];
-    int* const lds_k = x_k + (threadIdx.x & ~(WAVE - 1));
-    double* const lds_q = x_q + (threadIdx.x & ~(WAVE - 1));
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[a.parity ^ 1] = 0ull;
-    const int nr = d.nriv, nd = d.fp_nd;
-    for (;;) {
-        AD_TICKET(x.nchunks_r)
-        const int rp = c * WAVE + lane;
-        const bool valid = rp < nr;
-        const int q_ = valid ? rp : 0;
-        const int r = x.rp_r[q_], rdn_ = x.rp_down[q_], e0 = x.fp_sup_ptr[q_], e1_ = x.fp_sup_ptr[q_ + 1];
-        const int rdn = valid ? rdn_ : -1;
-        const bool cons_in = rdn >= 0 && (rdn >> 6) == c;
-        const int e1 = valid ? e1_ : e0;
-        const double h = d.fp_h[r];
-        double cap = 0.0;
-        if (h > 0.0) {
-            int i1, i2;
-            fp_indices(h, d.fp_depth, nd, i1, i2);
-            const double* Wv = d.fp_prof + ((size_t)r * 4 + 1) * nd;
-            const double flow_area = fp_flow_area(Wv, Wv + nd, d.fp_depth, h, i1, i2);
-            double flow_area_ds = flow_area;
-            if (rdn_ >= 0) {
-                const double* Wd = d.fp_prof + ((size_t)x.rp_r[rdn_] * 4 + 1) * nd;
-                flow_area_ds = fp_flow_area(Wd, Wd + nd, d.fp_depth, h, i1, i2);
-            }
-            if (flow_area > 1.0e-05 && flow_area_ds > 1.0e-05) {
-                const double wetted_perimeter = Wv[2 * nd + i1] + 2.0 * (h - d.fp_depth[i1]);
-                const double hydraulic_radius = flow_area / wetted_perimeter;
-                // manning_flow  surface_process.jl:166-169
-                cap = wfh_cbrt(hydraulic_radius * hydraulic_radius) * sqrt(d.fp_slope[r]) * flow_area / d.fp_n[r];
-            }
-        }
-        const double material0 = d.fp_storage[r];
-        double s_new = material0, flux = 0.0;
-        auto pass = [&](int, double material, double) -> double {
-            const double flux_val = wfh_min(material / dts, cap);
-            const double update = flux_val * dts;
-            s_new = material - update;
-            flux = flux_val;
-            return update;
-        };
-        if (!run_family<FOLD_SNOW>(a, lane, c, valid, 1, x.fp_sup_idx, x.fp_sup_idx, nullptr, e0, e1, a.slot_r,
-                                   (size_t)nr, rp, rdn >= 0 && !cons_in, cons_in ? (rdn & 63) : -1, lds_k, lds_q, pass,
-                                   material0))
-            return;
-        if (valid) {
-            d.fp_storage[r] = s_new;
-            d.fp_q[r] = flux;
-            d.fp_q_cum[r] += flux * dts;
         }
     }
 }
@@ -3246,19 +3235,14 @@ static int32_t launch_adaptive(wfh_handle* h) {
                 hipLaunchKernelGGL(k_ad_ol, dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(), h->ad,
                                    dts, first, last);
             else if (nriv > 0) {
-                const unsigned gr1 = (unsigned)((nriv + 255) / 256);
-                if (h->d.fp) hipLaunchKernelGGL(k_fp_exchange, dim3(gr1), dim3(256), 0, h->stream, h->d, dts, first);
-                if (h->nres > 0)
-                    hipLaunchKernelGGL(k_ad_riv<true>, dim3(grid_r), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(),
-                                       h->ad, dts, first, last);
-                else
-                    hipLaunchKernelGGL(k_ad_riv<false>, dim3(grid_r), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(),
-                                       h->ad, dts, first, last);
-                if (h->d.fp) {
-                    hipLaunchKernelGGL(k_fp_accu, dim3(grid_r), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(), h->ad,
-                                       dts);
-                    hipLaunchKernelGGL(k_fp_qin, dim3(gr1), dim3(256), 0, h->stream, h->d, h->ad, dts);
-                }
+                const bool res = h->nres > 0, fp = h->d.fp != 0;
+                auto kr = res ? (fp ? k_ad_riv<true, true> : k_ad_riv<true, false>)
+                              : (fp ? k_ad_riv<false, true> : k_ad_riv<false, false>);
+                hipLaunchKernelGGL(kr, dim3(grid_r), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(), h->ad, dts, first,
+                                   last);
+                if (fp)  // flux_in! of the floodplain flow this sub-step produced
+                    hipLaunchKernelGGL(k_fp_qin, dim3((unsigned)((nriv + 255) / 256)), dim3(256), 0, h->stream, h->d,
+                                       h->ad, dts);
             }
             HIPCHK(h, hipGetLastError());
             t = tn;
