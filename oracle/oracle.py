@@ -122,6 +122,9 @@ def lib():
         L.wfo_floodplain_profile.argtypes = [ctypes.c_int64, _D, _D, ctypes.c_int64, _c, _c, _D]
         L.wfo_flood_depth.restype = _c
         L.wfo_flood_depth.argtypes = [_D, ctypes.c_int64, _D, _c, _c]
+        L.wfo_floodplain_exchange.argtypes = [_D, ctypes.c_int64, _D] + [_c] * 6 + [_D]
+        L.wfo_floodplain_capacity.restype = _c
+        L.wfo_floodplain_capacity.argtypes = [_D, _D, ctypes.c_int64, _D, _c, _c, _c]
         L.wfo_set_julian_day.restype = ctypes.c_int32
         L.wfo_set_julian_day.argtypes = [ctypes.c_void_p, ctypes.c_int32]
         _lib = L
@@ -210,6 +213,36 @@ def flood_depth(depth, volume, flow_width, flow_length, flood_storage):
     d0 = np.concatenate([[0.0], d])
     return lib().wfo_flood_depth(prof.ctypes.data_as(_D), d.size + 1, d0.ctypes.data_as(_D), flood_storage,
                                  flow_length)
+
+
+def _profile_buf(depth, volume, flow_width, flow_length):
+    d = np.ascontiguousarray(depth, np.float64)
+    v = np.ascontiguousarray(volume, np.float64)
+    prof = np.zeros(4 * (d.size + 1))
+    lib().wfo_floodplain_profile(d.size, d.ctypes.data_as(_D), v.ctypes.data_as(_D), 1, flow_width, flow_length,
+                                 prof.ctypes.data_as(_D))
+    return prof, np.concatenate([[0.0], d])
+
+
+def floodplain_exchange(depth, volume, flow_width, flow_length, bankfull_depth, river_storage, fp_storage, dt):
+    """river_channel_floodplain_exchange! (surface_kinwave.jl:572-601) for one river cell:
+    returns (floodplain storage, flood depth, exchange flux)."""
+    prof, d0 = _profile_buf(depth, volume, flow_width, flow_length)
+    out = np.zeros(3)
+    lib().wfo_floodplain_exchange(prof.ctypes.data_as(_D), d0.size, d0.ctypes.data_as(_D), river_storage, fp_storage,
+                                  bankfull_depth, flow_width, flow_length, dt, out.ctypes.data_as(_D))
+    return tuple(out)
+
+
+def floodplain_capacity(depth, volume, flow_width, flow_length, h, mannings_n, slope, downstream=None):
+    """Manning capacity of update_floodplain_model! (surface_kinwave.jl:395-425); downstream =
+    (volume, flow_width, flow_length) of the downstream cell, None at a pit."""
+    prof, d0 = _profile_buf(depth, volume, flow_width, flow_length)
+    pd = None
+    if downstream is not None:
+        pd, _ = _profile_buf(depth, *downstream)
+    return lib().wfo_floodplain_capacity(prof.ctypes.data_as(_D), None if pd is None else pd.ctypes.data_as(_D),
+                                         d0.size, d0.ctypes.data_as(_D), h, mannings_n, slope)
 
 
 def reservoir_maximum_storage(res):

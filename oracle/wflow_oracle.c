@@ -1761,25 +1761,59 @@ static double land_total_storage(const wfo_model* m, int64_t i) {
 /* river_channel_floodplain_exchange!  surface_kinwave.jl:572-601: the water above bankfull is
  * shared instantaneously between channel and floodplain; the change of the channel's storage
  * enters the river's kinematic wave as a lateral inflow */
+void wfo_floodplain_exchange(const double* prof, int64_t nd, const double* depth0, double river_storage0,
+                             double fp_storage0, double bankfull_depth, double flow_width, double flow_length, double dt,
+                             double* out3) {
+    const double bankfull_storage = bankfull_depth * flow_width * flow_length;
+    const double storage_total = river_storage0 + fp_storage0;
+    double delta_river_storage, fs, fh;
+    if (storage_total > bankfull_storage) {
+        const double flood_storage = storage_total - bankfull_storage;
+        const double h = wfo_flood_depth(prof, nd, depth0, flood_storage, flow_length);
+        const double river_storage = (bankfull_depth + h) * flow_width * flow_length;
+        delta_river_storage = river_storage - river_storage0;
+        fs = wfh_max(storage_total - river_storage, 0.0);
+        fh = fs > 0.0 ? h : 0.0;
+    } else {
+        delta_river_storage = wfh_max(storage_total - river_storage0, 0.0);
+        fh = 0.0;
+        fs = 0.0;
+    }
+    out3[0] = fs;
+    out3[1] = fh;
+    out3[2] = delta_river_storage / dt;
+}
+
 static void river_floodplain_exchange(wfo_model* m, double dt) {
     const int64_t nd = m->fp_nd;
     for (int64_t i = 0; i < m->nriv; ++i) {
-        const double storage_total = m->r_storage[i] + m->fp_storage[i];
-        double delta_river_storage;
-        if (storage_total > m->r_bfs[i]) {
-            const double flood_storage = storage_total - m->r_bfs[i];
-            const double h = wfo_flood_depth(m->fp_prof + i * 4 * nd, nd, m->fp_depth, flood_storage, m->r_fl[i]);
-            const double river_storage = (m->r_bfd[i] + h) * m->r_fw[i] * m->r_fl[i];
-            delta_river_storage = river_storage - m->r_storage[i];
-            m->fp_storage[i] = wfh_max(storage_total - river_storage, 0.0);
-            m->fp_h[i] = m->fp_storage[i] > 0.0 ? h : 0.0;
-        } else {
-            delta_river_storage = wfh_max(storage_total - m->r_storage[i], 0.0);
-            m->fp_h[i] = 0.0;
-            m->fp_storage[i] = 0.0;
-        }
-        m->fp_exch[i] = delta_river_storage / dt;
+        double o[3];
+        wfo_floodplain_exchange(m->fp_prof + i * 4 * nd, nd, m->fp_depth, m->r_storage[i], m->fp_storage[i], m->r_bfd[i],
+                                m->r_fw[i], m->r_fl[i], dt, o);
+        m->fp_storage[i] = o[0];
+        m->fp_h[i] = o[1];
+        m->fp_exch[i] = o[2];
     }
+}
+
+double wfo_floodplain_capacity(const double* prof, const double* prof_ds, int64_t nd, const double* depth0, double h,
+                               double mannings_n, double slope) {
+    double cap = 0.0;
+    if (h > 0.0) {
+        int64_t i1, i2;
+        fp_interpolation_indices(h, depth0, nd, &i1, &i2);
+        const double* Wv = prof + nd;
+        const double flow_area = fp_flow_area(Wv, Wv + nd, depth0, h, i1, i2);
+        double flow_area_ds = flow_area;
+        if (prof_ds) flow_area_ds = fp_flow_area(prof_ds + nd, prof_ds + 2 * nd, depth0, h, i1, i2);
+        if (flow_area > 1.0e-05 && flow_area_ds > 1.0e-05) {
+            const double wetted_perimeter = Wv[2 * nd + i1] + 2.0 * (h - depth0[i1]);
+            const double hydraulic_radius = flow_area / wetted_perimeter;
+            /* manning_flow  surface_process.jl:166-169 */
+            cap = wfh_cbrt(hydraulic_radius * hydraulic_radius) * sqrt(slope) * flow_area / mannings_n;
+        }
+    }
+    return cap;
 }
 
 /* update_floodplain_model!  surface_kinwave.jl:387-430: Manning flow capacity from the flood
@@ -1790,38 +1824,12 @@ static void river_floodplain_exchange(wfo_model* m, double dt) {
 static void update_floodplain(wfo_model* m, double dt) {
     const int64_t nd = m->fp_nd, nriv = m->nriv;
     for (int64_t k = 0; k < nriv; ++k) {
-        const int64_t v = m->rorder[k];
-        const double h = m->fp_h[v];
-        double cap = 0.0;
-        if (h > 0.0) {
-            int64_t i1, i2;
-            fp_interpolation_indices(h, m->fp_depth, nd, &i1, &i2);
-            const double* Wv = m->fp_prof + (v * 4 + 1) * nd;
-            const double flow_area = fp_flow_area(Wv, Wv + nd, m->fp_depth, h, i1, i2);
-            const int64_t ds = m->rdown[v];
-            double flow_area_ds = flow_area;
-            if (ds >= 0) {
-                const double* Wd = m->fp_prof + (ds * 4 + 1) * nd;
-                flow_area_ds = fp_flow_area(Wd, Wd + nd, m->fp_depth, h, i1, i2);
-            }
-            if (flow_area > 1.0e-05 && flow_area_ds > 1.0e-05) {
-                const double* Pv = Wv + 2 * nd;
-                const double wetted_perimeter = Pv[i1] + 2.0 * (h - m->fp_depth[i1]);
-                const double hydraulic_radius = flow_area / wetted_perimeter;
-                /* manning_flow  surface_process.jl:166-169 */
-                cap = wfh_cbrt(hydraulic_radius * hydraulic_radius) * sqrt(m->fp_slope[v]) * flow_area / m->fp_n[v];
-            }
-        }
-        m->fp_cap[v] = cap;
+        const int64_t v = m->rorder[k], ds = m->rdown[v];
+        m->fp_cap[v] = wfo_floodplain_capacity(m->fp_prof + v * 4 * nd, ds >= 0 ? m->fp_prof + ds * 4 * nd : NULL, nd,
+                                               m->fp_depth, m->fp_h[v], m->fp_n[v], m->fp_slope[v]);
     }
-    for (int64_t k = 0; k < nriv; ++k) {
-        const int64_t v = m->rorder[k];
-        const double flux_val = wfh_min(m->fp_storage[v] / dt, m->fp_cap[v]);
-        const double material_update = flux_val * dt;
-        m->fp_storage[v] -= material_update;
-        m->fp_q[v] = flux_val;
-        if (m->rdown[v] >= 0) m->fp_storage[m->rdown[v]] += material_update;
-    }
+    /* the same accucapacityflux! as the snow transport's, over the river graph and its order */
+    wfo_accucapacityflux(nriv, m->rorder, m->rdown, m->fp_storage, m->fp_cap, m->fp_q, dt);
     for (int64_t r = 0; r < nriv; ++r) m->fp_q_cum[r] += m->fp_q[r] * dt;
     for (int64_t v = 0; v < nriv; ++v) {
         double s = 0.0;

@@ -101,6 +101,15 @@ void wfo_floodplain_profile(int64_t nd0, const double* depth, const double* vol,
                             double* out);
 /* compute_flood_depth  floodplain.jl:316-327 on such a profile; depth0 = [0, depth...] */
 double wfo_flood_depth(const double* prof, int64_t nd, const double* depth0, double flood_storage, double fl);
+/* river_channel_floodplain_exchange!  surface_kinwave.jl:572-601 for one river cell with profile
+ * prof ([4][nd], depth0 = [0, depths...]): out3 = floodplain storage, flood depth, exchange flux */
+void wfo_floodplain_exchange(const double* prof, int64_t nd, const double* depth0, double river_storage0,
+                             double fp_storage0, double bankfull_depth, double flow_width, double flow_length, double dt,
+                             double* out3);
+/* Manning flow capacity of update_floodplain_model!  surface_kinwave.jl:395-425 at flood depth h;
+ * prof_ds = the downstream cell's profile, NULL at a pit */
+double wfo_floodplain_capacity(const double* prof, const double* prof_ds, int64_t nd, const double* depth0, double h,
+                               double mannings_n, double slope);
 /* update_index_hq!  reservoir.jl:379-386: the rating-curve column (julian day 1..365); 0 or -1 */
 int32_t wfo_set_julian_day(wfo_model* m, int32_t julian_day);
 /* Statistics.quantile!(v, p) of k >= 2 values (sorts v); the kinematic-wave stable time step

@@ -11,6 +11,7 @@ Reference: Wflow/src/routing/surface/floodplain.jl:29-147, 264-341, 380-402 and
 surface_kinwave.jl:387-434, 505-537, 572-662.
 """
 import ctypes
+import math
 
 import numpy as np
 import pytest
@@ -60,6 +61,54 @@ def test_flood_depth_inverts_the_storage():
     # above the table: extrapolated with the last segment's width
     top = float(vol[-1])
     assert O.flood_depth(d, vol, 20.0, fl, top + 1000.0 * 2.0 * fl) == pytest.approx(6.0, rel=1e-13)
+
+
+def test_exchange_of_a_rectangle():
+    """Above bankfull the excess storage F fills a rectangle of width W: flood depth F / (W L),
+    the channel holds (bankfull depth + h) w L and the floodplain (W - w) h L; below bankfull
+    the floodplain returns all its water to the channel (surface_kinwave.jl:585-599)."""
+    d = np.array([0.5, 1.0, 2.0, 4.0])
+    W, w, L, bfd, dt = 400.0, 40.0, 1000.0, 2.0, 900.0
+    vol = W * d * L
+    fs, fh, ex = O.floodplain_exchange(d, vol, w, L, bfd, 100000.0, 30000.0, dt)
+    h = 50000.0 / (W * L)
+    assert fh == pytest.approx(h, rel=1e-15)
+    assert fs == pytest.approx((W - w) * h * L, rel=1e-14)
+    assert ex == pytest.approx(((bfd + h) * w * L - 100000.0) / dt, rel=1e-14)
+    assert fs + (bfd + h) * w * L == pytest.approx(130000.0, rel=1e-15)  # water is conserved
+    # deeper than the first segment: h = 1.5 m
+    F = W * 1.5 * L
+    fs, fh, ex = O.floodplain_exchange(d, vol, w, L, bfd, bfd * w * L + F, 0.0, dt)
+    assert fh == pytest.approx(1.5, rel=1e-14)
+    assert fs == pytest.approx((W - w) * 1.5 * L, rel=1e-14)
+    # below bankfull
+    fs, fh, ex = O.floodplain_exchange(d, vol, w, L, bfd, 50000.0, 10000.0, dt)
+    assert (fs, fh) == (0.0, 0.0)
+    assert ex == 10000.0 / dt
+
+
+@pytest.mark.parametrize("h", [0.125, 0.5, 1.5, 3.0])
+def test_capacity_of_a_rectangle(h):
+    """Manning flow of the floodplain part of a rectangle: area (W - w) h, wetted perimeter
+    (W - w) + 2 h, Q = R^(2/3) S^(1/2) A / n (surface_process.jl:166-169)."""
+    d = np.array([0.5, 1.0, 2.0, 4.0])
+    W, w, L, n, S = 400.0, 40.0, 1000.0, 0.07, 2e-4
+    vol = W * d * L
+    A = (W - w) * h
+    P = (W - w) + 2.0 * h
+    want = (A / P) ** (2.0 / 3.0) * math.sqrt(S) * A / n
+    got = O.floodplain_capacity(d, vol, w, L, h, n, S, downstream=(vol, w, L))
+    assert got == pytest.approx(want, rel=1e-14)
+    assert O.floodplain_capacity(d, vol, w, L, h, n, S) == got  # a pit uses its own area
+
+
+def test_capacity_zero_cases():
+    d = np.array([0.5, 1.0, 2.0, 4.0])
+    W, w, L = 400.0, 40.0, 1000.0
+    vol = W * d * L
+    assert O.floodplain_capacity(d, vol, w, L, 0.0, 0.07, 1e-3) == 0.0
+    # a downstream cell without floodplain beyond its channel (width w everywhere): its area is 0
+    assert O.floodplain_capacity(d, vol, w, L, 0.8, 0.07, 1e-3, downstream=(w * d * L, w, L)) == 0.0
 
 
 def _balance(m):
