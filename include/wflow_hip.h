@@ -140,7 +140,7 @@ typedef struct wfh_static {
     const double* river_width;
     /* reservoirs (reservoir__flag; Wflow/src/routing/surface/reservoir.jl, network.jl:296-360),
      * nres = 0 for none. Outflow types 1 (rating curve, H-Q table per julian day), 2 (free weir,
-     * without a linked lower reservoir), 3 (modified Puls) and 4 (simple), or the observed outflow
+     * optionally linked to a lower reservoir), 3 (modified Puls) and 4 (simple), or the observed outflow
      * where given; linear or interpolated (S-H table) storage curve (fields at the end). */
     int64_t nres;
     const int64_t* reservoir_outlet;       /* [nres] land index of each outlet, ascending; a river
@@ -181,6 +181,10 @@ typedef struct wfh_static {
                                          (floodplain_water__sum_of_volume_per_depth) [m3]       */
     const double* floodplain_manning_n; /* [nriv] (floodplain_water_flow__manning_n_parameter)   */
     const double* floodplain_slope;     /* [nriv] (floodplain__slope), clamped at 1e-5           */
+    /* [nres] or NULL: 1-based lower reservoir linked to a free weir (lower_reservoir_ind,
+     * reservoir.jl:152-159), 0 = none. The lower reservoir's outlet must lie downstream of the
+     * free weir's on the river, and at most one free weir may link to a reservoir. */
+    const int32_t* reservoir_lower;
 } wfh_static;
 
 /* Readable / writable variables (BMI get_value / set_value equivalents, bmi.jl:203-262). */

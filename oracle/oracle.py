@@ -153,7 +153,8 @@ class Reservoir(ctypes.Structure):
         "outflow_cumulative", "actevap_cumulative", "abstraction_cumulative")] + [
         ("storage_curve_type", ctypes.c_int32), ("rating_curve_exponent", ctypes.c_double),
         ("sh_n", ctypes.c_int64), ("sh_h", _D), ("sh_s", _D),
-        ("hq_n", ctypes.c_int64), ("hq_h", _D), ("hq_q", _D), ("hq_col", ctypes.c_int32)]
+        ("hq_n", ctypes.c_int64), ("hq_h", _D), ("hq_q", _D), ("hq_col", ctypes.c_int32),
+        ("lower", ctypes.c_void_p)]
 
     @classmethod
     def make(cls, sh=None, hq=None, julian_day=1, **kw):

@@ -678,22 +678,48 @@ static void reservoir_hq(const wfo_reservoir* r, double inflow, double precipita
     *storage_out = storage;
 }
 
-/* update_reservoir_free_weir  reservoir.jl:485-553 for a reservoir without a linked lower
- * reservoir (lower_reservoir_ind = 0, so diff_wl = 0) */
+/* update_reservoir_free_weir  reservoir.jl:485-553. With a linked lower reservoir whose water
+ * level is above this one's, the flow reverses (negative outflow, from the lower reservoir) and
+ * the lower reservoir's storage, water level, outflow and cumulative outflow are updated in
+ * place. */
 static void reservoir_free_weir(const wfo_reservoir* r, double inflow, double precipitation, double evaporation,
                                 double dt, double* outflow_out, double* storage_out) {
+    wfo_reservoir* lo = r->lower;
+    const double diff_wl = lo ? r->waterlevel - lo->waterlevel : 0.0;
     const double storage_input = wfh_max(r->storage / dt + precipitation - evaporation + inflow, 0.0);
     double outflow;
-    if (r->waterlevel > r->threshold) {
-        const double dh = r->waterlevel - r->threshold;
-        outflow = r->rating_curve_coefficient * wfh_pow(dh, r->rating_curve_exponent);
-        const double maxflow = dh * r->area / dt;
-        outflow = wfh_min(outflow, maxflow);
+    if (diff_wl >= 0.0) {
+        if (r->waterlevel > r->threshold) {
+            const double dh = r->waterlevel - r->threshold;
+            outflow = r->rating_curve_coefficient * wfh_pow(dh, r->rating_curve_exponent);
+            const double maxflow = dh * r->area / dt;
+            outflow = wfh_min(outflow, maxflow);
+        } else {
+            outflow = 0.0;
+        }
     } else {
-        outflow = 0.0;
+        if (lo->waterlevel > r->threshold) {
+            const double dh = lo->waterlevel - r->threshold;
+            outflow = -r->rating_curve_coefficient * wfh_pow(dh, r->rating_curve_exponent);
+            const double maxflow = dh * lo->area / dt;
+            outflow = wfh_max(outflow, -maxflow);
+        } else {
+            outflow = 0.0;
+        }
+    }
+    const double storage = (storage_input - outflow) * dt;
+    if (diff_wl < 0.0) {
+        const double lower_res_storage = lo->storage + outflow * dt;
+        const double lower_res_waterlevel =
+            lo->storage_curve_type == 2 ? wfo_interpolate_linear(lower_res_storage, lo->sh_s, lo->sh_h, lo->sh_n, 1)
+                                        : lo->waterlevel + (lower_res_storage - lo->storage) / lo->area;
+        lo->outflow = -outflow;
+        lo->outflow_cumulative -= outflow * dt;
+        lo->storage = lower_res_storage;
+        lo->waterlevel = lower_res_waterlevel;
     }
     *outflow_out = outflow;
-    *storage_out = (storage_input - outflow) * dt;
+    *storage_out = storage;
 }
 
 /* update_reservoir_outflow_obs  reservoir.jl:555-575 */
@@ -1413,8 +1439,38 @@ wfo_model* wfo_create(const wfh_config* cfg, const wfh_static* st, char* err, in
             R->sh_h = R->sh_s = R->hq_h = R->hq_q = NULL; /* re-pointed into res_tab below */
             R->hq_col = 0;
             R->outflow = NaN;
+            R->lower = NULL;
         }
         free(taken);
+        /* linked free weirs (lower_reservoir_ind, reservoir.jl:152-159): the lower reservoir's
+         * outlet must lie downstream of the upper one's on the river, and at most one upper
+         * reservoir may link to a lower one (the GPU hands the transfer down the river) */
+        if (st->reservoir_lower) {
+            int64_t* has_upper = ialloc(nr);
+            for (int64_t k = 0; k < nr; ++k) {
+                const int32_t lw = st->reservoir_lower[k];
+                if (lw == 0) continue;
+                int ok = lw >= 1 && lw <= nr && lw - 1 != k && m->res[k].outflow_type == 2 && !has_upper[lw - 1];
+                if (ok) { /* walk down the river from the upper outlet */
+                    ok = 0;
+                    for (int64_t rr = m->rdown[m->res_river[k]]; rr >= 0; rr = m->rdown[rr])
+                        if (rr == m->res_river[lw - 1]) {
+                            ok = 1;
+                            break;
+                        }
+                }
+                if (!ok) {
+                    snprintf(err, (size_t)errlen, "reservoir %lld: linked lower reservoir %d must be another reservoir "
+                             "whose outlet lies downstream on the river, linked from one free weir only",
+                             (long long)k, lw);
+                    free(has_upper);
+                    goto fail;
+                }
+                has_upper[lw - 1] = 1;
+                m->res[k].lower = &m->res[lw - 1];
+            }
+            free(has_upper);
+        }
         {
             int64_t tot = 0;
             for (int64_t k = 0; k < nr; ++k) tot += 2 * m->res[k].sh_n + 366 * m->res[k].hq_n;

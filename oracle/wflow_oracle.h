@@ -60,6 +60,7 @@ typedef struct wfo_reservoir {
     int64_t hq_n;                   /* H-Q table rows (outflow type 1) */
     const double *hq_h, *hq_q;      /* hq_q: [hq_n][365], one column per julian day */
     int32_t hq_col;                 /* col_index_hq - 1 */
+    struct wfo_reservoir* lower;    /* linked lower reservoir of a free weir (lower_reservoir_ind), or NULL */
 } wfo_reservoir;
 
 /* interpolate_linear(x, xp, fp)  reservoir.jl:364-377; fp read with stride fs */

@@ -75,3 +75,28 @@ def test_julian_day_argument_checked():
         with pytest.raises(RuntimeError, match="julian day"):
             gpu.set_julian_day(bad)
     gpu.close()
+
+
+def test_linked_free_weirs_hourly():
+    """Free weirs linked to a lower reservoir downstream, with reverse flow: the weir hands its
+    reverse flow to the lower reservoir's lane within each river sub-step launch."""
+    _say("linked free weirs hourly 96x96")
+    c = synth.make_catchment(96, cell=250.0, dt=3600.0, river_threshold=60, seed=53)
+    synth.add_reservoirs(c, 12, seed=2, curves=True, linked=True)
+    assert int((c.arrays["reservoir_lower"] > 0).sum()) == 2
+    out = run_curves(c, 6, start_day=100)
+    assert np.all(out[c.arrays["reservoir_lower"] > 0] < 0.0)
+    # an observed outflow replaces a linked weir's update: its lower reservoir gets no transfer
+    up = int(np.nonzero(c.arrays["reservoir_lower"])[0][0])
+    c.arrays["reservoir_outflow_obs"][up] = 0.5
+    run_curves(c, 2, start_day=100)
+
+
+def test_linked_free_weirs_daily_and_adaptive():
+    _say("linked free weirs daily and adaptive 96x96")
+    c = synth.make_catchment(96, cell=1000.0, dt=86400.0, river_threshold=60, seed=53)
+    synth.add_reservoirs(c, 12, seed=2, curves=True, linked=True)
+    run_curves(c, 3, start_day=300)
+    c = synth.make_catchment(96, cell=250.0, dt=3600.0, river_threshold=60, seed=53)
+    synth.add_reservoirs(c, 12, seed=2, curves=True, linked=True)
+    run_curves(c, 3, start_day=1, adaptive=True)

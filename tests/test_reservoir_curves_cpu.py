@@ -307,3 +307,75 @@ def test_subset_carries_reservoirs_and_floodplain():
         np.testing.assert_array_equal(full.get(Var.RESERVOIR_STORAGE), part.get(Var.RESERVOIR_STORAGE))
     full.close()
     part.close()
+
+
+def _pair(wl_up, wl_lo, area_lo=8e6, lower_sh=None):
+    up = O.Reservoir.make(outflow_type=2, area=5e7, threshold=2.0, rating_curve_coefficient=140.0,
+                          rating_curve_exponent=1.5, precipitation=0.0, evaporation=0.0, waterlevel=wl_up,
+                          storage=wl_up * 5e7)
+    lo = O.Reservoir.make(outflow_type=3, area=area_lo, threshold=1.0, rating_curve_coefficient=0.3,
+                          precipitation=0.0, evaporation=0.0, waterlevel=wl_lo, storage=wl_lo * area_lo, sh=lower_sh)
+    up.lower = ctypes.addressof(lo)
+    return up, lo
+
+
+def test_linked_free_weir_reverse_flow():
+    """The lower reservoir above the upper one: the weir flow reverses, -c (H_lo - thr)^e capped
+    at (H_lo - thr) A_lo / dt, and the water leaves the lower reservoir (reservoir.jl:506-551)."""
+    for area_lo in (8e6, 1e4):  # uncapped, then capped by the lower reservoir's volume above thr
+        up, lo = _pair(3.0, 4.0, area_lo)
+        s_lo0, cout0 = lo.storage, lo.outflow_cumulative
+        O.update_reservoir(up, 10.0, DT)
+        dh = 4.0 - 2.0
+        want = max(-140.0 * dh ** 1.5, -dh * area_lo / DT)
+        assert up.outflow == pytest.approx(want, rel=1e-14)
+        assert up.storage == pytest.approx((3.0 * 5e7 / DT + 10.0 - want) * DT, rel=1e-14)
+        assert lo.storage == pytest.approx(s_lo0 + want * DT, rel=1e-14)
+        assert lo.waterlevel == pytest.approx(4.0 + want * DT / area_lo, rel=1e-13)
+        assert lo.outflow == -up.outflow
+        assert lo.outflow_cumulative == cout0 - want * DT
+        # the two storages together changed by the inflow alone
+        assert up.storage + lo.storage == pytest.approx(3.0 * 5e7 + s_lo0 + 10.0 * DT, rel=1e-14)
+
+
+def test_linked_free_weir_forward_and_interpolated_lower():
+    up, lo = _pair(4.0, 3.0)  # upper above: ordinary free weir, the lower reservoir untouched
+    s_lo0 = lo.storage
+    O.update_reservoir(up, 0.0, DT)
+    assert up.outflow == pytest.approx(min(140.0 * 2.0 ** 1.5, 2.0 * 5e7 / DT), rel=1e-14)
+    assert lo.storage == s_lo0
+    h = np.linspace(0.0, 10.0, 11)
+    up, lo = _pair(3.0, 4.0, 8e6, lower_sh=(h, 8e6 * h * (1.0 + 0.1 * h)))
+    lo.storage = O.reservoir_initial_storage(lo)
+    s0 = lo.storage
+    O.update_reservoir(up, 0.0, DT)
+    assert lo.storage == pytest.approx(s0 + up.outflow * DT, rel=1e-14)
+    assert lo.waterlevel == pytest.approx(float(np.interp(lo.storage, 8e6 * h * (1.0 + 0.1 * h), h)), rel=1e-12)
+
+
+def test_linked_reservoirs_in_model():
+    """A model with linked free weirs (synthetic: each free weir linked to the next reservoir
+    downstream, which starts higher): the flow reverses and every reservoir's balance closes."""
+    c = synth.make_catchment(96, cell=250.0, dt=3600.0, river_threshold=60, seed=53)
+    synth.add_reservoirs(c, 12, seed=2, curves=True, linked=True)
+    lw = c.arrays["reservoir_lower"]
+    assert int((lw > 0).sum()) == 2
+    m = O.OracleModel(c)
+    for k in range(4):
+        m.step(*synth.forcing_numpy(c, k))
+        st = m.get(Var.RESERVOIR_STORAGE)
+        assert np.all(np.abs(m.get(Var.BALANCE_RESERVOIR_ERROR)) <= 1e-12 * np.maximum(1.0, st / 3600.0))
+    assert np.all(m.get(Var.RESERVOIR_OUTFLOW)[lw > 0] < 0.0)
+    m.close()
+    # links must point downstream, from free weirs, one per lower reservoir
+    bad = synth.make_catchment(96, cell=250.0, dt=3600.0, river_threshold=60, seed=53)
+    synth.add_reservoirs(bad, 12, seed=2, curves=True, linked=True)
+    up = int(np.nonzero(lw)[0][0])
+    bad.arrays["reservoir_lower"][:] = 0
+    bad.arrays["reservoir_lower"][lw[up] - 1] = up + 1  # the lower one linked back upstream
+    with pytest.raises(RuntimeError):
+        O.OracleModel(bad)
+    lib = load_library()
+    cfg, st = bad.config(), bad.static()
+    h = ctypes.c_void_p()
+    assert lib.wfh_create(ctypes.byref(cfg), ctypes.byref(st), ctypes.byref(h)) == _abi.WFH_ERR_ARG

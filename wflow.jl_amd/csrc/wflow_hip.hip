@@ -123,6 +123,11 @@ struct Dev {
     int res_hq_col;            // julian day - 1
     const double *res_area, *res_demand, *res_maxrel, *res_maxsto, *res_tff, *res_tmf, *res_thr, *res_rcc;
     const double* res_rce;     // [nres] free weir rating curve exponent
+    int* abort_flag;           // the routing kernels' abort flag (RouteArgs::abort_flag)
+    int res_linked;            // some free weir is linked to a lower reservoir
+    const int *res_lower, *res_upper;  // [nres] linked lower / upper reservoir, or -1
+    uint64_t* res_xfer;        // [nres] single-consumer slot: the linked free weir's reverse flow of
+                               // the sub-step into this reservoir (+inf: none)
     const double *res_ext, *res_obs;
     double *res_storage, *res_wl, *res_outflow, *res_P, *res_E;
     double* res_cum;           // [4][nres] sums of the step: inflow, outflow, actual evaporation, abstraction
@@ -763,14 +768,44 @@ __device__ __forceinline__ double interp_table(const double x, const double* xp,
 // lives in memory and only this lane touches it. first / last: the step's first / last river
 // sub-step (set_reservoir_vars!, average_reservoir_vars!, the reservoir balance). Returns the
 // outflow, which becomes the downstream river cell's inflow.
+// LINK (linked free weirs, one launch per sub-step): a reservoir with a linked free weir upstream
+// first takes that weir's reverse flow of the sub-step from its slot and applies it to its
+// storage, water level and cumulative outflow, as the weir's update does in the reference
+// (reservoir.jl:539-550); the weir reads the lower reservoir's level of the previous sub-step.
+template <bool LINK = false>
 __device__ __forceinline__ double reservoir_step(const Dev& d, const int ri, const int p, const double q_v,
                                               const double dts, const bool first, const bool last) {
     const int nr = d.nres;
     const double area = d.res_area[ri];
-    double storage0 = d.res_storage[ri];
+    double storage0 = d.res_storage[ri], wl0 = d.res_wl[ri];
     double cin = first ? 0.0 : d.res_cum[ri], cout = first ? 0.0 : d.res_cum[nr + ri];
     double cev = first ? 0.0 : d.res_cum[2 * nr + ri], cab = first ? 0.0 : d.res_cum[3 * nr + ri];
     if (first && d.balance) d.res_store_prev[ri] = storage0;
+    if (LINK && d.res_upper[ri] >= 0) {
+        // the weir is upstream, so it has computed; its slot store becomes visible shortly
+        double xfer;
+        long spin = 0;
+        while (!slot_take(d.res_xfer + ri, xfer)) {
+            if (++spin > SPIN_LIMIT) {
+                __hip_atomic_store(d.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return 0.0 / 0.0;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (xfer != INFINITY) {
+            const double lower_res_storage = storage0 + xfer * dts;
+            if (d.res_sct[ri] == 2) {
+                const int r0 = d.res_sh_ptr[ri];
+                const double* ext = d.res_shx + (size_t)ri * 4;
+                wl0 = interp_table(lower_res_storage, d.res_sh_s + r0, d.res_sh_h + r0, 1, d.res_sh_ptr[ri + 1] - r0,
+                                   ext[0], ext[1], ext[2], ext[3]);
+            } else {
+                wl0 = wl0 + (lower_res_storage - storage0) / area;
+            }
+            cout -= xfer * dts;
+            storage0 = lower_res_storage;
+        }
+    }
     // update_inflow!  surface_kinwave.jl:772-799: the outlet cell's overland and subsurface flow
     const double io = d.ol_q_avg[p], is = d.ssf_q_avg[p];
     const double inflow_ext = d.res_ext[ri];
@@ -789,6 +824,7 @@ __device__ __forceinline__ double reservoir_step(const Dev& d, const int ri, con
     const double evaporation = wfh_min(available_storage / dts, potential_evaporation);
     const double obs = d.res_obs[ri], maxsto = d.res_maxsto[ri];
     double outflow, storage;
+    double xfer_out = INFINITY;  // reverse flow into a linked lower reservoir (+inf: none)
     if (!wfh_isnan(obs)) {  // update_reservoir_outflow_obs  reservoir.jl:555-575
         const double storage_input = wfh_max(storage0 / dts + precipitation - evaporation + net_inflow, 0.0);
         outflow = wfh_min(obs, storage_input);
@@ -802,24 +838,35 @@ __device__ __forceinline__ double reservoir_step(const Dev& d, const int ri, con
         const double storage_input = storage0 / dts + precipitation - evaporation + net_inflow;
         const int r0 = d.res_hq_ptr[ri], col = d.res_hq_col;
         const double* ext = d.res_hq_ext + (size_t)ri * 732;
-        outflow = interp_table(d.res_wl[ri], d.res_hq_h + r0, d.res_hq_q + (size_t)r0 * 365 + col, 365,
+        outflow = interp_table(wl0, d.res_hq_h + r0, d.res_hq_q + (size_t)r0 * 365 + col, 365,
                                d.res_hq_ptr[ri + 1] - r0, ext[0], ext[1], ext[2 + 2 * col], ext[3 + 2 * col]);
         outflow = wfh_min(outflow, storage_input);
         storage = (storage_input - outflow) * dts;
         const double overflow = wfh_max(0.0, (storage - maxsto) / dts);
         storage = wfh_min(storage, maxsto);
         outflow += overflow;
-    } else if (d.res_type[ri] == 2) {  // update_reservoir_free_weir  reservoir.jl:485-553, diff_wl = 0
+    } else if (d.res_type[ri] == 2) {  // update_reservoir_free_weir  reservoir.jl:485-553
         const double storage_input = wfh_max(storage0 / dts + precipitation - evaporation + net_inflow, 0.0);
-        const double wl0 = d.res_wl[ri], thr = d.res_thr[ri];
+        const double thr = d.res_thr[ri];
+        const int lo = LINK ? d.res_lower[ri] : -1;  // (ri's own lower reservoir)
+        const double wl_lo = lo >= 0 ? d.res_wl[lo] : 0.0;
+        const double diff_wl = lo >= 0 ? wl0 - wl_lo : 0.0;
         outflow = 0.0;
-        if (wl0 > thr) {
-            const double dh = wl0 - thr;
-            outflow = d.res_rcc[ri] * wfh_pow(dh, d.res_rce[ri]);
-            const double maxflow = dh * area / dts;
-            outflow = wfh_min(outflow, maxflow);
+        if (diff_wl >= 0.0) {
+            if (wl0 > thr) {
+                const double dh = wl0 - thr;
+                outflow = d.res_rcc[ri] * wfh_pow(dh, d.res_rce[ri]);
+                const double maxflow = dh * area / dts;
+                outflow = wfh_min(outflow, maxflow);
+            }
+        } else if (wl_lo > thr) {  // reverse flow from the linked lower reservoir
+            const double dh = wl_lo - thr;
+            outflow = -d.res_rcc[ri] * wfh_pow(dh, d.res_rce[ri]);
+            const double maxflow = dh * d.res_area[lo] / dts;
+            outflow = wfh_max(outflow, -maxflow);
         }
         storage = (storage_input - outflow) * dts;
+        if (diff_wl < 0.0) xfer_out = outflow;
     } else if (d.res_type[ri] == 3) {  // update_reservoir_modified_puls  reservoir.jl:427-456
         const double res_factor = area / (dts * sqrt(d.res_rcc[ri]));
         const double si_factor = storage0 / dts + precipitation - evaporation + net_inflow;
@@ -844,13 +891,16 @@ __device__ __forceinline__ double reservoir_step(const Dev& d, const int ri, con
         storage -= release_realized * dts;
         outflow = release_realized + demand_release;
     }
+    // every sub-step a linked weir hands its lower reservoir a value, whichever update ran (an
+    // observed outflow takes the place of the weir's)
+    if (LINK && d.res_lower[ri] >= 0) slot_put(d.res_xfer + d.res_lower[ri], xfer_out);
     if (d.res_sct[ri] == 2) {  // interpolate_linear(storage, S, H)  reservoir.jl:613-621
         const int r0 = d.res_sh_ptr[ri];
         const double* ext = d.res_shx + (size_t)ri * 4;
         d.res_wl[ri] = interp_table(storage, d.res_sh_s + r0, d.res_sh_h + r0, 1, d.res_sh_ptr[ri + 1] - r0, ext[0],
                                     ext[1], ext[2], ext[3]);
     } else {
-        d.res_wl[ri] = d.res_wl[ri] + (storage - storage0) / area;
+        d.res_wl[ri] = wl0 + (storage - storage0) / area;
     }
     d.res_storage[ri] = storage;
     d.res_outflow[ri] = outflow;
@@ -1544,7 +1594,7 @@ __device__ __forceinline__ double fp_capacity(const Dev& d, const int r, const i
 // down the unfiltered river graph as a second family: a cell adds what its upstream cells pass
 // on, in the reference's river topological order, to its floodplain storage, passes on
 // min(storage / dt, capacity) and keeps the rest.
-template <bool RES, bool FP>
+template <bool RES, bool FP, bool LINK>
 __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_riv(Dev d, RouteArgs a, AdArgs x, double dts, int first,
                                                         int last) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -1593,7 +1643,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_riv(Dev d, RouteArgs a, AdAr
             r_q_cum += qn * dts;
             r_qin_cum += qin * dts;
             rqin = qin;
-            if (RES && rres >= 0) return reservoir_step(d, rres, pnode, qn, dts, first != 0, last != 0);
+            if (RES && rres >= 0) return reservoir_step<LINK>(d, rres, pnode, qn, dts, first != 0, last != 0);
             return qn;
         };
         if (!run_family<RES ? FOLD_RIV_RES : FOLD_RIV>(a, lane, c, valid, 1, x.rp_up_idx, x.rp_up_idx, x.rp_up_res, e0,
@@ -2356,6 +2406,33 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
                 return WFH_ERR_ARG;
             }
     }
+    // linked free weirs (lower_reservoir_ind, reservoir.jl:152-159): the lower reservoir's outlet
+    // lies downstream of the weir's on the river, and one weir links to a reservoir at most
+    std::vector<int> res_lower(std::max<int64_t>(nres, 1), -1), res_upper(std::max<int64_t>(nres, 1), -1);
+    bool linked = false;
+    for (int64_t k = 0; k < nres && st->reservoir_lower; ++k) {
+        const int32_t lw = st->reservoir_lower[k];
+        if (lw == 0) continue;
+        bool ok = lw >= 1 && lw <= nres && lw - 1 != k && st->reservoir_outflow_type[k] == 2 && res_upper[lw - 1] < 0;
+        if (ok) {
+            ok = false;
+            const int64_t target = land_river[st->reservoir_outlet[lw - 1]];
+            for (int64_t rr = h->river_down[land_river[st->reservoir_outlet[k]]]; rr >= 0; rr = h->river_down[rr])
+                if (rr == target) {
+                    ok = true;
+                    break;
+                }
+        }
+        if (!ok) {
+            h->err = "reservoir " + std::to_string(k) + ": linked lower reservoir must be another reservoir whose "
+                     "outlet lies downstream on the river, linked from one free weir only";
+            wfh_destroy_keep_error(h, out);
+            return WFH_ERR_ARG;
+        }
+        res_lower[k] = lw - 1;
+        res_upper[lw - 1] = (int)k;
+        linked = true;
+    }
     h->nres = nres;
     if (fp) {  // FloodPlainProfile inputs (floodplain.jl:48-147, 380-402)
         bool ok = st->floodplain_ndepth >= 1 && st->floodplain_depth && st->floodplain_volume &&
@@ -2910,6 +2987,14 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
             if ((rc = put_raw<double>(h, shx, D.res_shx))) goto done;
             if ((rc = put_raw<double>(h, hq_ext, D.res_hq_ext))) goto done;
             if ((rc = put_raw<double>(h, res_vec(st->reservoir_rating_curve_exponent, nan), D.res_rce))) goto done;
+            D.res_linked = linked ? 1 : 0;
+            if ((rc = put_raw<int>(h, res_lower, D.res_lower))) goto done;
+            if ((rc = put_raw<int>(h, res_upper, D.res_upper))) goto done;
+            {
+                hipError_t ex;
+                D.res_xfer = dalloc<uint64_t>(h, nr, ex);  // emptied once the stream exists
+                if (ex != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipMalloc reservoir slots"); goto done; }
+            }
             std::vector<double> zeros(nr, 0.0), zeros4(4 * nr, 0.0), nans(nr, nan);
             if ((rc = put_raw<int>(h, rtype, D.res_type))) goto done;
             if ((rc = put_raw<int>(h, cov_ptr, D.res_cov_ptr))) goto done;
@@ -2967,6 +3052,7 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         if (e != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipMalloc slots"); goto done; }
         A.counters = dalloc<unsigned long long>(h, 2, e);
         A.abort_flag = dalloc<int>(h, 1, e);
+        h->d.abort_flag = A.abort_flag;
         if (e != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipMalloc counters"); goto done; }
         A.S_s = h->S_s;
         A.S_l = h->S_l;
@@ -2980,6 +3066,7 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         A.nchunks = h->nchunks;
         if (hipStreamCreate(&h->stream) != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipStreamCreate"); goto done; }
         if ((rc = launch_fill(h, A.slot_ssf, SLOT_EMPTY, (size_t)h->S_s * n))) goto done;
+        if ((rc = launch_fill(h, h->d.res_xfer, SLOT_EMPTY, (size_t)std::max<int64_t>(nres, 1)))) goto done;
         if ((rc = launch_fill(h, A.slot_ol, SLOT_EMPTY, (size_t)h->S_l * n))) goto done;
         if ((rc = launch_fill(h, A.slot_r, SLOT_EMPTY, (size_t)h->S_r * std::max<int64_t>(nriv, 1)))) goto done;
         if ((rc = launch_fill(h, (uint64_t*)A.counters, 0, 2))) goto done;
@@ -3002,7 +3089,7 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         if (cus < 1) cus = 1;
         const int want = (h->nchunks + (ROUTE_BLOCK / WAVE) - 1) / (ROUTE_BLOCK / WAVE);
         h->route_grid = std::max(1, std::min(want, per_cu * cus));
-        if (adaptive || fp) {
+        if (adaptive || fp || linked) {
             // river positions: river cells sorted by node, a topological order of the river graph
             // because every river edge is a land edge and node order is topological
             AdArgs& X = h->ad;
@@ -3235,9 +3322,10 @@ static int32_t launch_adaptive(wfh_handle* h) {
                 hipLaunchKernelGGL(k_ad_ol, dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(), h->ad,
                                    dts, first, last);
             else if (nriv > 0) {
-                const bool res = h->nres > 0, fp = h->d.fp != 0;
-                auto kr = res ? (fp ? k_ad_riv<true, true> : k_ad_riv<true, false>)
-                              : (fp ? k_ad_riv<false, true> : k_ad_riv<false, false>);
+                const bool res = h->nres > 0, fp = h->d.fp != 0, link = h->d.res_linked != 0;
+                auto kr = res ? (link ? (fp ? k_ad_riv<true, true, true> : k_ad_riv<true, false, true>)
+                                      : (fp ? k_ad_riv<true, true, false> : k_ad_riv<true, false, false>))
+                              : (fp ? k_ad_riv<false, true, false> : k_ad_riv<false, false, false>);
                 hipLaunchKernelGGL(kr, dim3(grid_r), dim3(ROUTE_BLOCK), 0, h->stream, h->d, args(), h->ad, dts, first,
                                    last);
                 if (fp)  // flux_in! of the floodplain flow this sub-step produced
@@ -3303,7 +3391,7 @@ extern "C" int32_t wfh_step(wfh_handle* h) {
     if (h->broken) return fail(h, WFH_ERR_STATE, "handle is in an error state (a previous step failed)");
     if (!h->forcing_set) return fail(h, WFH_ERR_STATE, "wfh_set_forcing must be called before wfh_step");
     HIPCHK(h, hipSetDevice(h->cfg.device));
-    if (h->cfg.kinematic_wave_adaptive || h->d.fp) {  // one launch per sub-step
+    if (h->cfg.kinematic_wave_adaptive || h->d.fp || h->d.res_linked) {  // one launch per sub-step
         switch (h->N) {
             case 2: return launch_adaptive<2>(h);
             case 3: return launch_adaptive<3>(h);

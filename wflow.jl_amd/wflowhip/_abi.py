@@ -75,6 +75,7 @@ STATIC_FIELDS = [
     ("reservoir_hq_ptr", _I64), ("reservoir_hq_h", _D), ("reservoir_hq_q", _D),
     ("floodplain_ndepth", ctypes.c_int32), ("floodplain_depth", _D), ("floodplain_volume", _D),
     ("floodplain_manning_n", _D), ("floodplain_slope", _D),
+    ("reservoir_lower", _I32),
 ]
 
 

@@ -185,8 +185,10 @@ class Catchment:
         remap = np.zeros(nres + 1, np.int32)
         remap[keep + 1] = np.arange(1, keep.size + 1, dtype=np.int32)
         B["reservoir_coverage"] = np.ascontiguousarray(remap[A["reservoir_coverage"][mask]])
+        if "reservoir_lower" in A:  # 1-based links; a link to a dropped reservoir goes
+            B["reservoir_lower"] = np.ascontiguousarray(remap[A["reservoir_lower"][keep]])
         for k, a in A.items():
-            if k.startswith("reservoir_") and k not in ("reservoir_outlet", "reservoir_coverage") and \
+            if k.startswith("reservoir_") and k not in ("reservoir_outlet", "reservoir_coverage", "reservoir_lower") and \
                     not k.startswith(("reservoir_sh_", "reservoir_hq_")):
                 assert a.shape[0] == nres, k
                 B[k] = np.ascontiguousarray(a[keep])
@@ -387,7 +389,7 @@ def land_downstream(catchment):
     return np.where(ok, k, -1)
 
 
-def add_reservoirs(catchment, count, seed=0, radius=1, curves=False):
+def add_reservoirs(catchment, count, seed=0, radius=1, curves=False, linked=False):
     """Place `count` reservoirs on river cells (in place): outlets on river cells whose river
     continues downstream, at most one reservoir feeding any river cell, each with the cells
     within `radius` raster steps of its outlet as its area (cells of an earlier reservoir
@@ -397,7 +399,9 @@ def add_reservoirs(catchment, count, seed=0, radius=1, curves=False):
     curves=True cycles the outflow types 1 (rating curve), 2 (free weir), 3 and 4, gives every
     third reservoir (k % 3 == 1) an interpolated storage curve (S-H table), and gives the
     rating-curve reservoirs H-Q tables with a seasonal cycle over the 365 day columns; reservoir 4
-    starts above the top of its H-Q table, so it overflows."""
+    starts above the top of its H-Q table, so it overflows. linked=True (with curves) links every
+    free weir to the first unlinked reservoir downstream of it on the river, if any, and starts
+    that lower reservoir above the free weir's level, so the flow reverses."""
     c = catchment
     rng = np.random.default_rng(seed + 4242)
     down = land_downstream(c)
@@ -450,7 +454,30 @@ def add_reservoirs(catchment, count, seed=0, radius=1, curves=False):
     A["reservoir_outflow_obs"] = obs
     if curves:
         _add_reservoir_curves(c, rng)
+        if linked:
+            _link_free_weirs(c)
     return c
+
+
+def _link_free_weirs(c):
+    A = c.arrays
+    down = land_downstream(c)
+    outlets = A["reservoir_outlet"]
+    at = {int(o): k for k, o in enumerate(outlets)}
+    lower = np.zeros(outlets.size, np.int32)
+    taken = set()
+    wl, thr = A["reservoir_waterlevel"], A["reservoir_threshold"]
+    for k in np.nonzero(A["reservoir_outflow_type"] == 2)[0]:
+        i = int(down[outlets[k]])
+        while i >= 0:
+            j = at.get(i)
+            if j is not None and j not in taken and j != k:
+                lower[k] = j + 1
+                taken.add(j)
+                wl[j] = max(wl[j], wl[k] + 0.5, thr[k] + 1.0)
+                break
+            i = int(down[i])
+    A["reservoir_lower"] = lower
 
 
 def add_floodplain(catchment, seed=0, depths=(0.5, 1.0, 2.0, 3.5, 5.0), bankfull_scale=0.02):
