@@ -28,10 +28,16 @@ def test_defaults_and_profiles():
     assert info["exponential_constant"] and not info["reservoirs"]
 
 
+def test_floodplain_flag():
+    c, info = config.load(text='[time]\ntimestepsecs = 86400\n[model]\ntype = "sbm"\nfloodplain_1d__flag = true\n')
+    assert c.floodplain_1d == 1 and info["floodplain"]
+    c, info = config.load(text='[time]\ntimestepsecs = 86400\n[model]\ntype = "sbm"\n')
+    assert c.floodplain_1d == 0 and not info["floodplain"]
+
+
 @pytest.mark.parametrize("model", ['type = "sediment"', 'type = "sbm"\nglacier__flag = true',
                                    'type = "sbm"\nriver_routing = "local_inertial"',
                                    'type = "sbm"\nsaturated_hydraulic_conductivity_profile = "layered"',
-                                   'type = "sbm"\nfloodplain_1d__flag = true',
                                    'type = "sbm"\nsome_unknown__flag = true'])
 def test_unsupported_settings_are_rejected(model):
     with pytest.raises(ValueError):

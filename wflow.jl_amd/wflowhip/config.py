@@ -53,14 +53,15 @@ _REQUIRE = {  # key: the only value the GPU step implements
     "land_routing": "kinematic_wave",
     "river_routing": "kinematic_wave",
     "pit__flag": False,
-    "floodplain_1d__flag": False,
 }
 _PROFILES = ("exponential", "exponential_constant")
 
 
 def load(path=None, text=None):
     """Parse a Wflow TOML file (or its text) into (wfh_config, info). info holds the settings the
-    caller has to act on: 'reservoirs' (fill wfh_static's reservoir arrays), 'exponential_constant'
+    caller has to act on: 'reservoirs' (fill wfh_static's reservoir arrays and tables, and call
+    wfh_set_julian_day for rating curves), 'floodplain' (fill wfh_static's floodplain_* fields;
+    wfh_config.floodplain_1d is set), 'exponential_constant'
     (fill wfh_static.z_exp), 'cold_start', 'cell_length_in_meter' and 'schedule_only' (keys that
     do not change results)."""
     if text is None:
@@ -108,8 +109,10 @@ def load(path=None, text=None):
     c.kinematic_wave_adaptive = int(bool(model["kinematic_wave__adaptive_time_step_flag"]))
     c.subsurface_alpha_coefficient = float(model["subsurface_kinematic_wave__alpha_coefficient"])
     c.snow_gravitational_transport = int(bool(model["snow_gravitational_transport__flag"]))
+    c.floodplain_1d = int(bool(model["floodplain_1d__flag"]))
     info = {
         "reservoirs": bool(model["reservoir__flag"]),
+        "floodplain": bool(model["floodplain_1d__flag"]),
         "exponential_constant": model["saturated_hydraulic_conductivity_profile"] == "exponential_constant",
         "cold_start": bool(model["cold_start__flag"]),
         "cell_length_in_meter": bool(model["cell_length_in_meter__flag"]),
