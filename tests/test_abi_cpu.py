@@ -76,3 +76,31 @@ def test_create_detects_cycle():
     ldd[0], ldd[1] = 6, 4  # cells (0,0) -> (1,0) -> (0,0)
     rc, msg, _ = _create(c)
     assert rc == _abi.WFH_ERR_CYCLE, msg
+
+
+def test_struct_layout_matches_header(tmp_path):
+    """Offsets and sizes of wfh_config and wfh_static as a C compiler lays them out, against the
+    ctypes mirror (a field of the wrong width would shift every field after it)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = (("wfh_config", _abi.WfhConfig), ("wfh_static", _abi.WfhStatic))
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "wflow_hip.h"', "int main(void) {"]
+    for name, py in structs:
+        lines.append(f'printf("%zu\\n", sizeof({name}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({name}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    include = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run([cc, "-I", include, str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    want = []
+    for _, py in structs:
+        want.append(ctypes.sizeof(py))
+        want += [getattr(py, f).offset for f, _ in py._fields_]
+    assert got == want
