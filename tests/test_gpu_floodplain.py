@@ -88,3 +88,12 @@ def test_floodplain_state_roundtrip():
         assert np.array_equal(a.get_value(v), b.get_value(v), equal_nan=True), v
     a.close()
     b.close()
+
+
+def test_floodplain_with_linked_free_weirs_and_snow_transport():
+    _say("floodplain + linked free weirs + snow transport hourly 96x96")
+    c = synth.make_catchment(96, cell=250.0, dt=3600.0, river_threshold=60, seed=53)
+    synth.add_reservoirs(c, 12, seed=2, curves=True, linked=True)
+    synth.add_floodplain(c, seed=6)
+    assert int((c.arrays["reservoir_lower"] > 0).sum()) == 2
+    run_fp(c, 4, res=True, snow_transport=True)
