@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--floodplain", action="store_true",
                     help="1-D floodplain beside every river cell (floodplain_1d); one launch per river sub-step")
     ap.add_argument("--snow-transport", action="store_true", help="lateral snow transport (gravitational)")
+    ap.add_argument("--host-forcing", action="store_true",
+                    help="hand the forcing over from host memory every step (WFH_MEM_HOST: permutation into the "
+                         "node order and a PCIe upload inside the timed region); not the headline line")
     ap.add_argument("--adaptive", action="store_true",
                     help="adaptive kinematic-wave sub-steps (kinematic_wave__adaptive_time_step_flag) instead of "
                          "the fixed ones; not the headline line: no roofline, every sub-step waits for the last")
@@ -187,8 +190,17 @@ def main():
     torch.cuda.synchronize()
     print(f"# {nframes} forcing frames on the GPU ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
 
+    if args.host_forcing:  # the same frames in host memory, in the reference's cell order
+        inv_h = np.empty_like(node)
+        inv_h[node] = np.arange(node.size)
+        frames_h = [tuple(np.ascontiguousarray(f.cpu().numpy()[inv_h]) for f in fr) for fr in frames]
+        workload += ", forcing handed over from host memory every step (PCIe-inclusive)"
+
     def step(k):
-        m.set_forcing(*frames[k % nframes], memkind=_abi.WFH_MEM_DEVICE_NODE_ORDER)
+        if args.host_forcing:
+            m.set_forcing(*frames_h[k % nframes])
+        else:
+            m.set_forcing(*frames[k % nframes], memkind=_abi.WFH_MEM_DEVICE_NODE_ORDER)
         m.update_model()
 
     for k in range(args.warmup):
@@ -230,6 +242,8 @@ def main():
         metric = metric.replace(" step)", " step, adaptive sub-steps)")
     if args.reservoirs or args.snow_transport or args.floodplain:
         metric = metric.replace(" step", " step with reservoirs / floodplain / snow transport", 1)
+    if args.host_forcing:
+        metric = metric.replace(" step", " step, host forcing", 1)
     result = {
         "metric": metric,
         "value": c.n * args.steps / elapsed,

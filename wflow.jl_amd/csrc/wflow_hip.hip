@@ -1979,7 +1979,11 @@ struct wfh_handle {
     RouteArgs ra{};
     double *fP = nullptr, *fE = nullptr, *fT = nullptr;  // node-order forcing buffers
     int* d_land_of = nullptr;
-    std::vector<double> staging;
+    // host forcing: uploaded in the caller's cell order on a copy stream, then gathered into the
+    // node order on the model's stream (allocated at the first host forcing)
+    double* hf_tmp = nullptr;              // [3][n] device
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t hf_copied = nullptr, hf_gathered = nullptr;
     // adaptive sub-steps
     AdArgs ad{};
     SelCtl* sel = nullptr;       // device
@@ -3351,18 +3355,36 @@ extern "C" int32_t wfh_set_forcing(wfh_handle* h, const double* P, const double*
     if (!P || !E || !T) return fail(h, WFH_ERR_ARG, "null forcing pointer");
     const int n = (int)h->n;
     if (memkind == WFH_MEM_HOST) {
-        h->staging.resize((size_t)3 * n);
-        for (int q = 0; q < n; ++q) {
-            const int i = h->land_of[q];
-            h->staging[q] = P[i];
-            h->staging[n + q] = E[i];
-            h->staging[2 * n + q] = T[i];
+        // The upload overlaps the previous step's kernels: it goes to a buffer of its own on the
+        // copy stream, which first waits until the previous step has gathered from that buffer.
+        // The call returns once the copies are done, so the caller may reuse its arrays.
+        if (!h->hf_tmp) {
+            hipError_t e;
+            h->hf_tmp = dalloc<double>(h, (size_t)3 * n, e);
+            if (e != hipSuccess) {
+                h->hf_tmp = nullptr;
+                return fail(h, WFH_ERR_HIP, "hipMalloc host-forcing buffer");
+            }
+            HIPCHK(h, hipStreamCreate(&h->copy_stream));
+            HIPCHK(h, hipEventCreateWithFlags(&h->hf_copied, hipEventDisableTiming));
+            HIPCHK(h, hipEventCreateWithFlags(&h->hf_gathered, hipEventDisableTiming));
+            HIPCHK(h, hipEventRecord(h->hf_gathered, h->stream));
         }
-        HIPCHK(h, hipMemcpyAsync(h->fP, h->staging.data(), n * sizeof(double), hipMemcpyHostToDevice, h->stream));
-        HIPCHK(h, hipMemcpyAsync(h->fE, h->staging.data() + n, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
-        HIPCHK(h, hipMemcpyAsync(h->fT, h->staging.data() + 2 * n, n * sizeof(double), hipMemcpyHostToDevice,
-                                 h->stream));
-        HIPCHK(h, hipStreamSynchronize(h->stream));  // staging buffer reuse
+        const double* src[3] = {P, E, T};
+        HIPCHK(h, hipStreamWaitEvent(h->copy_stream, h->hf_gathered, 0));
+        for (int k = 0; k < 3; ++k)
+            HIPCHK(h, hipMemcpyAsync(h->hf_tmp + (size_t)k * n, src[k], n * sizeof(double), hipMemcpyHostToDevice,
+                                     h->copy_stream));
+        HIPCHK(h, hipEventRecord(h->hf_copied, h->copy_stream));
+        HIPCHK(h, hipStreamSynchronize(h->copy_stream));  // the caller's arrays are free again
+        HIPCHK(h, hipStreamWaitEvent(h->stream, h->hf_copied, 0));
+        const int g = (n + 255) / 256;
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(256), 0, h->stream, h->fP, h->hf_tmp, h->d_land_of, n);
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(256), 0, h->stream, h->fE, h->hf_tmp + n, h->d_land_of, n);
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(256), 0, h->stream, h->fT, h->hf_tmp + 2 * (size_t)n, h->d_land_of,
+                           n);
+        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipEventRecord(h->hf_gathered, h->stream));
         h->d.P = h->fP;
         h->d.E = h->fE;
         h->d.T = h->fT;
@@ -3788,6 +3810,10 @@ extern "C" void wfh_destroy(wfh_handle* h) {
     if (h->sel_host) hipHostFree(h->sel_host);
     for (hipEvent_t ev : h->ev)
         if (ev) hipEventDestroy(ev);
+    if (h->copy_stream) hipStreamSynchronize(h->copy_stream);
+    if (h->hf_copied) hipEventDestroy(h->hf_copied);
+    if (h->hf_gathered) hipEventDestroy(h->hf_gathered);
+    if (h->copy_stream) hipStreamDestroy(h->copy_stream);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
 }
