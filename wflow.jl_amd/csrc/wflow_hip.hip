@@ -1984,6 +1984,7 @@ struct wfh_handle {
     double* hf_tmp = nullptr;              // [3][n] device
     hipStream_t copy_stream = nullptr;
     hipEvent_t hf_copied = nullptr, hf_gathered = nullptr;
+    double* get_tmp = nullptr;             // [N][n] device: host wfh_get scatters into cell order here
     // adaptive sub-steps
     AdArgs ad{};
     SelCtl* sel = nullptr;       // device
@@ -3606,10 +3607,21 @@ extern "C" int32_t wfh_get(wfh_handle* h, int32_t var, double* dst, int64_t coun
         return WFH_OK;
     }
     if (count != n * vi.layers) return fail(h, WFH_ERR_ARG, "count mismatch");
-    std::vector<double> tmp((size_t)n * vi.layers);
-    HIPCHK(h, hipMemcpy(tmp.data(), vi.p, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    // into the reference's cell order on the device, then one copy
+    if (!h->get_tmp) {
+        hipError_t e;
+        h->get_tmp = dalloc<double>(h, (size_t)n * h->N, e);
+        if (e != hipSuccess) {
+            h->get_tmp = nullptr;
+            return fail(h, WFH_ERR_HIP, "hipMalloc read-back buffer");
+        }
+    }
     for (int k = 0; k < vi.layers; ++k)
-        for (int64_t q = 0; q < n; ++q) dst[k * n + h->land_of[q]] = tmp[k * n + q];
+        hipLaunchKernelGGL(k_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->get_tmp + k * n,
+                           vi.p + k * n, h->d_land_of, (int)n);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(dst, h->get_tmp, (size_t)count * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return WFH_OK;
 }
 
