@@ -91,3 +91,43 @@ def test_adaptive_then_state_roundtrip():
         assert np.array_equal(a.get_value(v), b.get_value(v), equal_nan=True), v
     a.close()
     b.close()
+
+
+def _chain_chunks(gpu):
+    """Chunks of river positions that the river launch runs as chains: river cells sorted by the
+    library's node order; full chunks of 64 with at most 4 upstream cells per lane, where lane
+    l's only upstream cell inside the chunk is lane l - 1."""
+    lo, ld, ro, rd, rl, _ = gpu.topology()
+    node = gpu.node_order()
+    pos_of_land = np.empty_like(node)
+    pos_of_land[node] = np.arange(node.size)
+    rp_r = np.argsort(pos_of_land[rl], kind="stable")
+    rpos = np.empty_like(rp_r)
+    rpos[rp_r] = np.arange(rp_r.size)
+    ups = [[] for _ in range(rp_r.size)]
+    for r in range(rd.size):
+        if rd[r] >= 0:
+            ups[rd[r]].append(r)
+    count = 0
+    for c in range(rp_r.size // 64):
+        ok = True
+        for l in range(64):
+            p = c * 64 + l
+            up = [int(rpos[u]) for u in ups[rp_r[p]]]
+            inside = [u for u in up if u // 64 == c]
+            ok = ok and len(up) <= 4 and inside == ([p - 1] if l > 0 else [])
+        count += ok
+    return count
+
+
+def test_adaptive_river_chain_chunks():
+    """A catchment whose main stem gives chain chunks (run as a lane-by-lane chain): the same
+    results as the oracle."""
+    _say("adaptive with chain chunks 512x512")
+    c = synth.make_catchment(512, cell=250.0, dt=3600.0, river_threshold=500, seed=36)
+    probe = Model(c, adaptive=True)
+    n_chain = _chain_chunks(probe)
+    probe.close()
+    _say(f"  {n_chain} chain chunks")
+    assert n_chain > 0
+    run_adaptive(c, 3)

@@ -174,6 +174,7 @@ struct AdArgs {
     const int* rp_node;    // [nriv] node of the river cell at each river position
     const int *fp_sup_ptr, *fp_sup_idx;  // [nriv+1], [Er]: all upstream river positions, in the
                                          // order of the reference's river topological sort
+    const int* chain_r;    // [nchunks_r] 1: a chain chunk (see run_chain)
     int nchunks_r;
 };
 
@@ -736,6 +737,73 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
         }
     }
     __builtin_amdgcn_wave_barrier();
+    return true;
+}
+
+__device__ __forceinline__ double readlane_f64(const double v, const int l) {
+    const uint64_t b = wfh_bits(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return wfh_from_bits(((uint64_t)hi << 32) | lo);
+}
+
+// A family of one sub-step over a chain chunk: all 64 lanes valid, at most 4 upstream cells
+// each, and lane l's only upstream cell inside the chunk is lane l-1 (checked on the host). The
+// lanes first take their inflow from other chunks; then the lanes fold and solve one after the
+// other, each handing its outflow to the next through a register broadcast. The folds, in edge
+// order, and the solves are those of run_family, so the results are the same; what goes is the
+// per-hop exchange through LDS and the wave votes of run_family's in-round passes.
+template <int FOLD, class Compute>
+__device__ __forceinline__ bool run_chain(const RouteArgs& a, const int lane, const int c,
+                                          const int* __restrict__ up_node, const int* __restrict__ up_slot,
+                                          const double* __restrict__ up_f, const int e0, const int e1,
+                                          uint64_t* slots, const int my_slot, const bool pub_cross, Compute& compute,
+                                          const double acc0 = 0.0) {
+    constexpr int PF = 4;
+    const int ne = e1 - e0;
+    int en[PF], es[PF];
+    double ef[PF], buf[PF];
+    bool have[PF];
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+        const bool use = j < ne;
+        const int ej = use ? e0 + j : (ne > 0 ? e0 : 0);  // (edge arrays hold at least one entry)
+        en[j] = use ? up_node[ej] : c * WAVE;
+        es[j] = use ? up_slot[ej] : 0;
+        ef[j] = (use && FOLD != FOLD_RIV && FOLD != FOLD_SNOW) ? up_f[ej] : 0.0;
+        buf[j] = 0.0;
+        have[j] = !use || (en[j] >> 6) == c;
+    }
+    long spin = 0;
+    for (;;) {
+        bool missing = false;
+#pragma unroll
+        for (int j = 0; j < PF; ++j) {
+            if (!have[j]) have[j] = slot_take(slots + es[j], buf[j]);
+            missing = missing || !have[j];
+        }
+        if (!__any(missing)) break;
+        __builtin_amdgcn_s_sleep(1);
+        ++spin;
+        if ((spin & 255) == 0 && __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return false;
+        if (spin > SPIN_LIMIT) {
+            __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+    }
+    double prev = 0.0, qv = 0.0;
+    for (int l = 0; l < WAVE; ++l) {
+        if (lane == l) {
+            double acc1 = acc0, acc2 = 0.0;
+#pragma unroll
+            for (int j = 0; j < PF; ++j)
+                if (j < ne) fold_in<FOLD>(acc1, acc2, (en[j] >> 6) == c ? prev : buf[j], ef[j]);
+            qv = compute(0, acc1, acc2);
+            if (pub_cross) slot_put(slots + my_slot, qv);
+        }
+        prev = readlane_f64(qv, l);
+    }
     return true;
 }
 
@@ -1646,10 +1714,13 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_riv(Dev d, RouteArgs a, AdAr
             if (RES && rres >= 0) return reservoir_step<LINK>(d, rres, pnode, qn, dts, first != 0, last != 0);
             return qn;
         };
-        if (!run_family<RES ? FOLD_RIV_RES : FOLD_RIV>(a, lane, c, valid, 1, x.rp_up_idx, x.rp_up_idx, x.rp_up_res, e0,
-                                                      e1, a.slot_r,
-                                  (size_t)nr, rp, rdn >= 0 && !cons_in, cons_in ? (rdn & 63) : -1, lds_k, lds_q,
-                                  riv_compute))
+        const bool chain = x.chain_r[c] != 0;  // wave-uniform
+        if (chain ? !run_chain<RES ? FOLD_RIV_RES : FOLD_RIV>(a, lane, c, x.rp_up_idx, x.rp_up_idx, x.rp_up_res, e0, e1,
+                                                                a.slot_r, rp, rdn >= 0 && !cons_in, riv_compute)
+                  : !run_family<RES ? FOLD_RIV_RES : FOLD_RIV>(a, lane, c, valid, 1, x.rp_up_idx, x.rp_up_idx,
+                                                                 x.rp_up_res, e0, e1, a.slot_r, (size_t)nr, rp,
+                                                                 rdn >= 0 && !cons_in, cons_in ? (rdn & 63) : -1,
+                                                                 lds_k, lds_q, riv_compute))
             return;
         if (FP) {
             const double cap = fp_capacity(d, r, rdn_ >= 0 ? x.rp_r[rdn_] : -1, fh);
@@ -1913,24 +1984,28 @@ __global__ void __launch_bounds__(256) k_sel_next(const uint64_t* __restrict__ k
 // Subsurface Courant time steps: minimum of flow_length / ssf_celerity over the cells with a
 // positive water table depth (reduced per wave, then one atomic per wave).
 // lateral_subsurface_flow.jl:314-344
+// Grid-stride over the cells with one minimum and one count per lane, reduced per wave: the
+// minimum and the count do not depend on the order, and a fixed grid keeps the atomics on the
+// two shared words to one pair per wave.
 __global__ void __launch_bounds__(256) k_ssf_min(Dev d, AdArgs x, SelCtl* s) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
     unsigned long long key = ~0ull;
-    bool elig = false;
-    if (i < d.n) {
+    unsigned cnt = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < d.n; i += gridDim.x * 256) {
         const double z = x.szi[i];
         if (z > 0.0) {
             const double ze = ld(d.zexp, i);
             const double cel = (ld(d.kh0, i) * wfh_exp(-ld(d.f, i) * (z < ze ? z : ze)) * ld(d.slope, i)) / ld(d.sy, i);
-            key = wfh_bits(ld(d.fl, i) / cel);
-            elig = true;
+            const unsigned long long k = wfh_bits(ld(d.fl, i) / cel);
+            key = k < key ? k : key;
+            ++cnt;
         }
     }
-    const unsigned long long b = __ballot(elig);
     key = wave_min_u64(key);
-    if ((threadIdx.x & (WAVE - 1)) == 0 && b) {
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && cnt) {
         atomicMin(&s->minkey, key);
-        atomicAdd(&s->count, (unsigned long long)__popcll(b));
+        atomicAdd(&s->count, (unsigned long long)cnt);
     }
 }
 
@@ -3145,6 +3220,26 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
                 if ((rc = put_raw<int>(h, si, X.fp_sup_idx))) goto done;
             }
             X.nchunks_r = (int)((nriv + WAVE - 1) / WAVE);
+            {  // chain chunks (run_chain): full, at most 4 upstream cells per lane, and lane l's only
+               // upstream cell inside the chunk is lane l-1
+                std::vector<int> chain((size_t)std::max(X.nchunks_r, 1), 0);
+                for (int c = 0; c < X.nchunks_r; ++c) {
+                    bool ok = (int64_t)(c + 1) * WAVE <= nriv;
+                    for (int l = 0; ok && l < WAVE; ++l) {
+                        const int p = c * WAVE + l;
+                        int inside = 0;
+                        ok = rp_ptr[p + 1] - rp_ptr[p] <= 4;
+                        for (int e = rp_ptr[p]; ok && e < rp_ptr[p + 1]; ++e)
+                            if (rp_idx[e] / WAVE == c) {
+                                ++inside;
+                                ok = rp_idx[e] == p - 1;
+                            }
+                        ok = ok && inside == (l > 0 ? 1 : 0);
+                    }
+                    chain[c] = ok ? 1 : 0;
+                }
+                if ((rc = put_raw<int>(h, chain, X.chain_r))) goto done;
+            }
             hipError_t ea = hipSuccess, eb;
             X.szi = dalloc<double>(h, n, eb); if (eb != hipSuccess) ea = eb;
             X.scum = dalloc<double>(h, 6 * (size_t)n, eb); if (eb != hipSuccess) ea = eb;
@@ -3230,7 +3325,8 @@ static int32_t read_sel(wfh_handle* h) {
 // stable_timestep of the subsurface model  lateral_subsurface_flow.jl:314-344
 static int32_t ssf_stable_timestep(wfh_handle* h, double& dts) {
     hipLaunchKernelGGL(k_sel_reset, dim3(1), dim3(256), 0, h->stream, h->sel);
-    hipLaunchKernelGGL(k_ssf_min, dim3((unsigned)((h->n + 255) / 256)), dim3(256), 0, h->stream, h->d, h->ad, h->sel);
+    hipLaunchKernelGGL(k_ssf_min, dim3((unsigned)std::min<int64_t>((h->n + 255) / 256, 2048)), dim3(256), 0, h->stream,
+                       h->d, h->ad, h->sel);
     HIPCHK(h, hipGetLastError());
     int32_t rc = read_sel(h);
     if (rc) return rc;
