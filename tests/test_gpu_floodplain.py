@@ -97,3 +97,13 @@ def test_floodplain_with_linked_free_weirs_and_snow_transport():
     synth.add_floodplain(c, seed=6)
     assert int((c.arrays["reservoir_lower"] > 0).sum()) == 2
     run_fp(c, 4, res=True, snow_transport=True)
+
+
+def test_floodplain_chain_chunks():
+    """A catchment whose main stem gives chain chunks: river and floodplain families run as
+    lane-by-lane chains there."""
+    _say("floodplain with chain chunks 512x512")
+    c = synth.make_catchment(512, cell=250.0, dt=3600.0, river_threshold=500, seed=36)
+    synth.add_floodplain(c, seed=7)
+    fs = run_fp(c, 3)
+    assert int((fs > 0).sum()) > 0

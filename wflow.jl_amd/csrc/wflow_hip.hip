@@ -1732,9 +1732,14 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_riv(Dev d, RouteArgs a, AdAr
                 flux = flux_val;
                 return update;
             };
-            if (!run_family<FOLD_SNOW>(a, lane, c, valid, 1, x.fp_sup_idx, x.fp_sup_idx, nullptr, x.fp_sup_ptr[q_],
-                                       valid ? x.fp_sup_ptr[q_ + 1] : x.fp_sup_ptr[q_], a.slot_ssf, (size_t)nr, rp,
-                                       rdn >= 0 && !cons_in, cons_in ? (rdn & 63) : -1, lds_k, lds_q, pass, fs))
+            // the floodplain lists hold the same upstream cells as the river's (in another order),
+            // so a chain chunk of the river is one of the floodplain too
+            const int f0 = x.fp_sup_ptr[q_], f1 = valid ? x.fp_sup_ptr[q_ + 1] : f0;
+            if (chain ? !run_chain<FOLD_SNOW>(a, lane, c, x.fp_sup_idx, x.fp_sup_idx, nullptr, f0, f1, a.slot_ssf, rp,
+                                              rdn >= 0 && !cons_in, pass, fs)
+                      : !run_family<FOLD_SNOW>(a, lane, c, valid, 1, x.fp_sup_idx, x.fp_sup_idx, nullptr, f0, f1,
+                                               a.slot_ssf, (size_t)nr, rp, rdn >= 0 && !cons_in,
+                                               cons_in ? (rdn & 63) : -1, lds_k, lds_q, pass, fs))
                 return;
             if (valid) {
                 if (first) {
