@@ -78,3 +78,45 @@ def test_tiny_catchments():
     gpu, ref = run_pair(one, 4)
     gpu.close()
     ref.close()
+
+
+def _land_chain_chunks(gpu):
+    """Chunks of the node order that the routing kernels run as chains: full chunks of 64 with
+    at most 4 upstream cells per lane, where lane l's only upstream cell inside the chunk is
+    lane l - 1 (no reservoirs here, so the upstream lists are the flow graph's)."""
+    lo, ld, _, _, _, _ = gpu.topology()
+    node = gpu.node_order()
+    pos = np.empty_like(node)
+    pos[node] = np.arange(node.size)
+    ups = [[] for _ in range(node.size)]
+    for i in range(ld.size):
+        if ld[i] >= 0:
+            ups[pos[ld[i]]].append(int(pos[i]))
+    count = 0
+    for c in range(node.size // 64):
+        ok = True
+        for l in range(64):
+            q = c * 64 + l
+            inside = [u for u in ups[q] if u // 64 == c]
+            ok = ok and len(ups[q]) <= 4 and inside == ([q - 1] if l > 0 else [])
+        count += ok
+    return count
+
+
+def test_fixed_substeps_with_chain_chunks():
+    """Hourly fixed sub-steps (1 subsurface, 1 overland): the main stem's chunks run their
+    subsurface and overland families as lane-by-lane chains in the persistent kernel."""
+    _say("fixed sub-steps with chain chunks 512x512")
+    c = synth.make_catchment(512, cell=250.0, dt=3600.0, river_threshold=500, seed=36)
+    gpu = Model(c)
+    n_chain = _land_chain_chunks(gpu)
+    _say(f"  {n_chain} land chain chunks")
+    assert n_chain > 0
+    ref = O.OracleModel(c)
+    for k in range(3):
+        P, E, T = synth.forcing_numpy(c, k)
+        gpu.run_timestep(P, E, T)
+        ref.step(P, E, T)
+        compare(gpu, ref, STATE_VARS + FLUX_VARS, f"step {k}")
+    gpu.close()
+    ref.close()

@@ -124,6 +124,7 @@ struct Dev {
     const double *res_area, *res_demand, *res_maxrel, *res_maxsto, *res_tff, *res_tmf, *res_thr, *res_rcc;
     const double* res_rce;     // [nres] free weir rating curve exponent
     int* abort_flag;           // the routing kernels' abort flag (RouteArgs::abort_flag)
+    const int* chain_land;     // [nchunks] 1: a chain chunk of the land lists (see run_chain)
     int res_linked;            // some free weir is linked to a lower reservoir
     const int *res_lower, *res_upper;  // [nres] linked lower / upper reservoir, or -1
     uint64_t* res_xfer;        // [nres] single-consumer slot: the linked free weir's reverse flow of
@@ -1315,8 +1316,13 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             cum.qnet += net * area * dts;
             return qn;
         };
-        if (!run_family<FOLD_SSF>(a, lane, c, valid, a.S_s, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ssf,
-                                  (size_t)n, p, (fl & 1) && !lcons_in, lcons_in ? (dn & 63) : -1, lds_k, lds_q, ssf_compute))
+        const bool lchain = d.chain_land[c] != 0;  // wave-uniform
+        if (lchain && a.S_s == 1
+                ? !run_chain<FOLD_SSF>(a, lane, c, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ssf, p,
+                                       (fl & 1) && !lcons_in, ssf_compute)
+                : !run_family<FOLD_SSF>(a, lane, c, valid, a.S_s, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ssf,
+                                        (size_t)n, p, (fl & 1) && !lcons_in, lcons_in ? (dn & 63) : -1, lds_k, lds_q,
+                                        ssf_compute))
             return;
         if (col_upd) update_ustorelayerdepth<N>(col, col_z0, col_z1);
 
@@ -1350,8 +1356,12 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             oqin = qin;
             return qn;
         };
-        if (!run_family<FOLD_OL>(a, lane, c, valid, a.S_l, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ol, (size_t)n,
-                                 p, (fl & 1) && !lcons_in, lcons_in ? (dn & 63) : -1, lds_k, lds_q, ol_compute))
+        if (lchain && a.S_l == 1
+                ? !run_chain<FOLD_OL>(a, lane, c, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ol, p,
+                                      (fl & 1) && !lcons_in, ol_compute)
+                : !run_family<FOLD_OL>(a, lane, c, valid, a.S_l, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ol,
+                                       (size_t)n, p, (fl & 1) && !lcons_in, lcons_in ? (dn & 63) : -1, lds_k, lds_q,
+                                       ol_compute))
             return;
         double r_inwater = 0.0;
         if (valid)
@@ -1480,8 +1490,12 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_ssf(Dev d, RouteArgs a, AdAr
             cum.qnet += net * area * dts;
             return qn;
         };
-        if (!run_family<FOLD_SSF>(a, lane, c, valid, 1, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ssf, (size_t)n, p,
-                                  (fl & 1) && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q, ssf_compute))
+        if (d.chain_land[c] != 0
+                ? !run_chain<FOLD_SSF>(a, lane, c, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ssf, p,
+                                       (fl & 1) && !cons_in, ssf_compute)
+                : !run_family<FOLD_SSF>(a, lane, c, valid, 1, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ssf,
+                                        (size_t)n, p, (fl & 1) && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q,
+                                        ssf_compute))
             return;
         if (col_upd) update_ustorelayerdepth<N>(col, col_z0, col_z1);
         if (valid && !last) {
@@ -1556,8 +1570,12 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_ad_ol(Dev d, RouteArgs a, AdArg
             oqin = qin;
             return qn;
         };
-        if (!run_family<FOLD_OL>(a, lane, c, valid, 1, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ol, (size_t)n, p,
-                                 (fl & 1) && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q, ol_compute))
+        if (d.chain_land[c] != 0
+                ? !run_chain<FOLD_OL>(a, lane, c, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ol, p,
+                                      (fl & 1) && !cons_in, ol_compute)
+                : !run_family<FOLD_OL>(a, lane, c, valid, 1, d.up_idx, d.up_idx, d.up_f, e0, e1, a.slot_ol,
+                                       (size_t)n, p, (fl & 1) && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q,
+                                       ol_compute))
             return;
         if (valid && !last) {
             d.ol_q[p] = oq;
@@ -2853,6 +2871,27 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         for (size_t e = 0; e < up_idx.size(); ++e) up_f[e] = fftr[h->land_of[up_idx[e]]];
         if ((rc = put_raw<int>(h, up_ptr, D.up_ptr))) goto done;
         if ((rc = put_raw<int>(h, up_idx, D.up_idx))) goto done;
+        {  // chain chunks of the land lists (run_chain): full, at most 4 upstream cells per lane,
+           // and lane l's only upstream cell inside the chunk is lane l-1
+            const int64_t nch = (n + WAVE - 1) / WAVE;
+            std::vector<int> chain((size_t)nch, 0);
+            for (int64_t c = 0; c < nch; ++c) {
+                bool ok = (c + 1) * WAVE <= n;
+                for (int l = 0; ok && l < WAVE; ++l) {
+                    const int64_t q = c * WAVE + l;
+                    int inside = 0;
+                    ok = up_ptr[q + 1] - up_ptr[q] <= 4;
+                    for (int e = up_ptr[q]; ok && e < up_ptr[q + 1]; ++e)
+                        if (up_idx[e] / WAVE == c) {
+                            ++inside;
+                            ok = up_idx[e] == q - 1;
+                        }
+                    ok = ok && inside == (l > 0 ? 1 : 0);
+                }
+                chain[c] = ok ? 1 : 0;
+            }
+            if ((rc = put_raw<int>(h, chain, D.chain_land))) goto done;
+        }
         if ((rc = put_raw<double>(h, up_f, D.up_f))) goto done;
         if ((rc = put_raw<int>(h, river_of, D.river_of))) goto done;
         if ((rc = put_raw<uint8_t>(h, flags, D.flags))) goto done;
