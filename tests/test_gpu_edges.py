@@ -103,15 +103,44 @@ def _land_chain_chunks(gpu):
     return count
 
 
+def _river_chain_chunks(gpu):
+    """Chunks of the node order whose 64 cells are all river cells forming a chain of the river
+    lists (the persistent kernel runs their river family as a diagonal over a chain)."""
+    lo, ld, ro, rd, rl, _ = gpu.topology()
+    node = gpu.node_order()
+    pos = np.empty_like(node)
+    pos[node] = np.arange(node.size)
+    river_at = np.full(node.size, -1)
+    river_at[pos[rl]] = np.arange(rl.size)
+    ups = [[] for _ in range(rl.size)]
+    for r in range(rd.size):
+        if rd[r] >= 0:
+            ups[rd[r]].append(int(pos[rl[r]]))
+    count = 0
+    for c in range(node.size // 64):
+        ok = True
+        for l in range(64):
+            q = c * 64 + l
+            r = river_at[q]
+            if r < 0:
+                ok = False
+                break
+            inside = [u for u in ups[r] if u // 64 == c]
+            ok = ok and len(ups[r]) <= 4 and inside == ([q - 1] if l > 0 else [])
+        count += ok
+    return count
+
+
 def test_fixed_substeps_with_chain_chunks():
-    """Hourly fixed sub-steps (1 subsurface, 1 overland): the main stem's chunks run their
-    subsurface and overland families as lane-by-lane chains in the persistent kernel."""
+    """Hourly fixed sub-steps (1 subsurface, 1 overland, 4 river): the main stem's chunks run
+    their subsurface and overland families as lane-by-lane chains and their river family as a
+    diagonal over the chain in the persistent kernel."""
     _say("fixed sub-steps with chain chunks 512x512")
     c = synth.make_catchment(512, cell=250.0, dt=3600.0, river_threshold=500, seed=36)
     gpu = Model(c)
-    n_chain = _land_chain_chunks(gpu)
-    _say(f"  {n_chain} land chain chunks")
-    assert n_chain > 0
+    n_chain, n_rchain = _land_chain_chunks(gpu), _river_chain_chunks(gpu)
+    _say(f"  {n_chain} land chain chunks, {n_rchain} river chain chunks")
+    assert n_chain > 0 and n_rchain > 0
     ref = O.OracleModel(c)
     for k in range(3):
         P, E, T = synth.forcing_numpy(c, k)

@@ -125,6 +125,7 @@ struct Dev {
     const double* res_rce;     // [nres] free weir rating curve exponent
     int* abort_flag;           // the routing kernels' abort flag (RouteArgs::abort_flag)
     const int* chain_land;     // [nchunks] 1: a chain chunk of the land lists (see run_chain)
+    const int* chain_rl;       // [nchunks] 1: all 64 cells are river cells and a chain of the river lists
     int res_linked;            // some free weir is linked to a lower reservoir
     const int *res_lower, *res_upper;  // [nres] linked lower / upper reservoir, or -1
     uint64_t* res_xfer;        // [nres] single-consumer slot: the linked free weir's reverse flow of
@@ -808,6 +809,81 @@ __device__ __forceinline__ bool run_chain(const RouteArgs& a, const int lane, co
     return true;
 }
 
+__device__ __forceinline__ double shfl_up1_f64(const double v) {
+    const uint64_t b = wfh_bits(v);
+    const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)b, 1u), hi = (uint32_t)__shfl_up((int)(uint32_t)(b >> 32), 1u);
+    return wfh_from_bits(((uint64_t)hi << 32) | lo);
+}
+
+// A family of S <= 4 sub-steps over a chain chunk (as run_chain) as a diagonal wavefront: in
+// round r lane l computes sub-step r - l, with lane l-1's outflow of the same sub-step (computed
+// in round r-1) shifted over from the neighbouring lane, and the inflow from other chunks of all
+// S sub-steps taken beforehand. Each lane computes its sub-steps in order, and its neighbour has
+// read an outflow before it is replaced, so the results are run_family's.
+template <int FOLD, class Compute>
+__device__ __forceinline__ bool run_chain_diag(const RouteArgs& a, const int lane, const int c, const int S,
+                                               const int* __restrict__ up_node, const int* __restrict__ up_slot,
+                                               const double* __restrict__ up_f, const int e0, const int e1,
+                                               uint64_t* slots, const size_t stride, const int my_slot,
+                                               const bool pub_cross, Compute& compute) {
+    constexpr int PF = 4, SM = 4;
+    const int ne = e1 - e0;
+    int en[PF], es[PF];
+    double ef[PF], buf[PF][SM];
+    bool have[PF][SM];
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+        const bool use = j < ne;
+        const int ej = use ? e0 + j : (ne > 0 ? e0 : 0);
+        en[j] = use ? up_node[ej] : c * WAVE;
+        es[j] = use ? up_slot[ej] : 0;
+        ef[j] = (use && FOLD != FOLD_RIV && FOLD != FOLD_SNOW) ? up_f[ej] : 0.0;
+#pragma unroll
+        for (int k = 0; k < SM; ++k) {
+            buf[j][k] = 0.0;
+            have[j][k] = !use || (en[j] >> 6) == c || k >= S;
+        }
+    }
+    long spin = 0;
+    for (;;) {
+        bool missing = false;
+#pragma unroll
+        for (int j = 0; j < PF; ++j)
+#pragma unroll
+            for (int k = 0; k < SM; ++k) {
+                if (!have[j][k]) have[j][k] = slot_take(slots + (size_t)k * stride + es[j], buf[j][k]);
+                missing = missing || !have[j][k];
+            }
+        if (!__any(missing)) break;
+        __builtin_amdgcn_s_sleep(1);
+        ++spin;
+        if ((spin & 255) == 0 && __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return false;
+        if (spin > SPIN_LIMIT) {
+            __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+    }
+    double qv = 0.0;
+    for (int r = 0; r < WAVE + S - 1; ++r) {
+        const double from_prev = shfl_up1_f64(qv);
+        const int k = r - lane;
+        if (k >= 0 && k < S) {
+            double acc1 = 0.0, acc2 = 0.0;
+#pragma unroll
+            for (int j = 0; j < PF; ++j) {
+                double bk = buf[j][0];
+#pragma unroll
+                for (int kk = 1; kk < SM; ++kk) bk = k == kk ? buf[j][kk] : bk;
+                if (j < ne) fold_in<FOLD>(acc1, acc2, (en[j] >> 6) == c ? from_prev : bk, ef[j]);
+            }
+            qv = compute(k, acc1, acc2);
+            if (pub_cross) slot_put(slots + (size_t)k * stride + my_slot, qv);
+        }
+    }
+    return true;
+}
+
 // interpolate_linear(x, xp, fp)  reservoir.jl:364-377 for a table of n rows whose xp is
 // non-decreasing (checked at create). The minimum and maximum of xp and fp come precomputed; the
 // last row with xp <= x is found by bisection (xp[lo] <= x < xp[hi] throughout), which for a
@@ -1392,10 +1468,14 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             if (EXT && rres >= 0) return reservoir_step(d, rres, p, qn, dts, k == 0, k == a.S_r - 1);
             return qn;
         };
-        if (!run_family<EXT ? FOLD_RIV_RES : FOLD_RIV>(a, lane, c, riv, a.S_r, d.rup_node, d.rup_idx, d.rup_res, re0,
-                                                      re1, a.slot_r,
-                                  (size_t)d.nriv, rr, riv && (fl & 4) && !rcons_in, rcons_in ? (dn & 63) : -1,
-                                  lds_k, lds_q, riv_compute))
+        if (d.chain_rl[c] != 0 && a.S_r <= 4
+                ? !run_chain_diag<EXT ? FOLD_RIV_RES : FOLD_RIV>(a, lane, c, a.S_r, d.rup_node, d.rup_idx, d.rup_res,
+                                                                 re0, re1, a.slot_r, (size_t)d.nriv, rr,
+                                                                 riv && (fl & 4) && !rcons_in, riv_compute)
+                : !run_family<EXT ? FOLD_RIV_RES : FOLD_RIV>(a, lane, c, riv, a.S_r, d.rup_node, d.rup_idx,
+                                                             d.rup_res, re0, re1, a.slot_r, (size_t)d.nriv, rr,
+                                                             riv && (fl & 4) && !rcons_in,
+                                                             rcons_in ? (dn & 63) : -1, lds_k, lds_q, riv_compute))
             return;
         if (riv) riv_finish(d, r, rq, rh, r_storage, rqin, r_q_cum, r_qin_cum, r_inwater, r_store_prev);
 
@@ -2891,6 +2971,27 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
                 chain[c] = ok ? 1 : 0;
             }
             if ((rc = put_raw<int>(h, chain, D.chain_land))) goto done;
+            // the same for the river lists over the node order: every cell a river cell
+            std::vector<int> chain_r((size_t)nch, 0);
+            for (int64_t c = 0; c < nch; ++c) {
+                bool ok = (c + 1) * WAVE <= n;
+                for (int l = 0; ok && l < WAVE; ++l) {
+                    const int64_t q = c * WAVE + l;
+                    const int r = river_of[q];
+                    ok = r >= 0;
+                    if (!ok) break;
+                    int inside = 0;
+                    ok = rup_ptr[r + 1] - rup_ptr[r] <= 4;
+                    for (int e = rup_ptr[r]; ok && e < rup_ptr[r + 1]; ++e)
+                        if (rup_node[e] / WAVE == c) {
+                            ++inside;
+                            ok = rup_node[e] == q - 1;
+                        }
+                    ok = ok && inside == (l > 0 ? 1 : 0);
+                }
+                chain_r[c] = ok ? 1 : 0;
+            }
+            if ((rc = put_raw<int>(h, chain_r, D.chain_rl))) goto done;
         }
         if ((rc = put_raw<double>(h, up_f, D.up_f))) goto done;
         if ((rc = put_raw<int>(h, river_of, D.river_of))) goto done;
