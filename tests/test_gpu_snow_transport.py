@@ -60,3 +60,17 @@ def test_snow_transport_adaptive():
     _say("snow transport adaptive 64x64")
     c = synth.make_catchment(64, cell=250.0, dt=3600.0, river_threshold=60, seed=73)
     run_snow(c, 6, adaptive=True)
+
+
+def test_snow_transport_chain_chunks():
+    """A catchment whose main stem gives chain chunks of the transport's lists: both snow
+    families run there as lane-by-lane chains."""
+    from test_gpu_edges import _land_chain_chunks
+    _say("snow transport with chain chunks 512x512")
+    c = synth.make_catchment(512, cell=250.0, dt=3600.0, river_threshold=500, seed=36)
+    probe = Model(c, snow_transport=True)
+    n_chain = _land_chain_chunks(probe)  # no reservoirs: the transport's lists are the same cells
+    probe.close()
+    _say(f"  {n_chain} chain chunks")
+    assert n_chain > 0
+    run_snow(c, 4)

@@ -126,6 +126,7 @@ struct Dev {
     int* abort_flag;           // the routing kernels' abort flag (RouteArgs::abort_flag)
     const int* chain_land;     // [nchunks] 1: a chain chunk of the land lists (see run_chain)
     const int* chain_rl;       // [nchunks] 1: all 64 cells are river cells and a chain of the river lists
+    const int* chain_sup;      // [nchunks] 1: a chain chunk of the snow transport's lists (sup_ptr)
     int res_linked;            // some free weir is linked to a lower reservoir
     const int *res_lower, *res_upper;  // [nres] linked lower / upper reservoir, or -1
     uint64_t* res_xfer;        // [nres] single-consumer slot: the linked free weir's reverse flow of
@@ -1955,8 +1956,12 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_snow_transport(Dev d, RouteArgs
             flux_s = flux_val;
             return update;
         };
-        if (!run_family<FOLD_SNOW>(a, lane, c, valid, 1, d.sup_idx, d.sup_idx, nullptr, e0, e1, a.slot_ssf, (size_t)n,
-                                   p, dn >= 0 && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q, pass_s, ss))
+        const bool chain = d.chain_sup[c] != 0;  // wave-uniform
+        if (chain ? !run_chain<FOLD_SNOW>(a, lane, c, d.sup_idx, d.sup_idx, nullptr, e0, e1, a.slot_ssf, p,
+                                          dn >= 0 && !cons_in, pass_s, ss)
+                  : !run_family<FOLD_SNOW>(a, lane, c, valid, 1, d.sup_idx, d.sup_idx, nullptr, e0, e1, a.slot_ssf,
+                                           (size_t)n, p, dn >= 0 && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q,
+                                           pass_s, ss))
             return;
         auto pass_w = [&](int, double material, double) -> double {
             const double flux_val = wfh_min(material / dt, cap_w);
@@ -1965,8 +1970,11 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_snow_transport(Dev d, RouteArgs
             flux_w = flux_val;
             return update;
         };
-        if (!run_family<FOLD_SNOW>(a, lane, c, valid, 1, d.sup_idx, d.sup_idx, nullptr, e0, e1, a.slot_ol, (size_t)n,
-                                   p, dn >= 0 && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q, pass_w, sw))
+        if (chain ? !run_chain<FOLD_SNOW>(a, lane, c, d.sup_idx, d.sup_idx, nullptr, e0, e1, a.slot_ol, p,
+                                          dn >= 0 && !cons_in, pass_w, sw)
+                  : !run_family<FOLD_SNOW>(a, lane, c, valid, 1, d.sup_idx, d.sup_idx, nullptr, e0, e1, a.slot_ol,
+                                           (size_t)n, p, dn >= 0 && !cons_in, cons_in ? (dn & 63) : -1, lds_k, lds_q,
+                                           pass_w, sw))
             return;
         if (valid) {
             d.snow_s[p] = ss_new;
@@ -3024,6 +3032,26 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
             }
             if ((rc = put_raw<int>(h, sup_ptr, D.sup_ptr))) goto done;
             if ((rc = put_raw<int>(h, sup_idx, D.sup_idx))) goto done;
+            {  // chain chunks of these lists (run_chain), as for the land lists
+                const int64_t nch = (n + WAVE - 1) / WAVE;
+                std::vector<int> chain((size_t)nch, 0);
+                for (int64_t c = 0; c < nch && D.snow_lat; ++c) {
+                    bool ok = (c + 1) * WAVE <= n;
+                    for (int l = 0; ok && l < WAVE; ++l) {
+                        const int64_t q = c * WAVE + l;
+                        int inside = 0;
+                        ok = sup_ptr[q + 1] - sup_ptr[q] <= 4;
+                        for (int e = sup_ptr[q]; ok && e < sup_ptr[q + 1]; ++e)
+                            if (sup_idx[e] / WAVE == c) {
+                                ++inside;
+                                ok = sup_idx[e] == q - 1;
+                            }
+                        ok = ok && inside == (l > 0 ? 1 : 0);
+                    }
+                    chain[c] = ok ? 1 : 0;
+                }
+                if ((rc = put_raw<int>(h, chain, D.chain_sup))) goto done;
+            }
             if ((rc = put_state(h, {}, D.snow_in))) goto done;
             if ((rc = put_state(h, {}, D.snow_out))) goto done;
         }
