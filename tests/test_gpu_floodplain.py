@@ -12,9 +12,10 @@ import numpy as np
 import pytest
 
 import oracle as O
-from test_gpu_parity import FLUX_VARS, STATE_VARS, compare
+from test_gpu_parity import FLUX_VARS, RTOL, STATE_VARS, compare, rel_err
 from test_gpu_reservoir import RES_VARS
 from wflowhip import Model, Var, synth
+from wflowhip import _abi
 from wflowhip._abi import FLOODPLAIN_VARS
 
 pytestmark = pytest.mark.gpu
@@ -107,3 +108,66 @@ def test_floodplain_chain_chunks():
     synth.add_floodplain(c, seed=7)
     fs = run_fp(c, 3)
     assert int((fs > 0).sum()) > 0
+
+
+def test_floodplain_4096_hourly_on_basins():
+    """The full 4096^2 hourly grid (250 m, river threshold 2000 cells) with a floodplain on every
+    river cell, on the GPU against the oracle run on complete basins of it (basins exchange no
+    water, so a set of whole basins is an exact sub-problem): random small basins over 12 steps,
+    checked every 4th step, and the largest basin (about 6 M cells, holding the longest flow
+    path) over 2 steps. Every state, flux and floodplain variable of every oracle cell must match
+    within RTOL."""
+    import time
+    import torch
+
+    t0 = time.time()
+    c = synth.make_catchment(4096, cell=250.0, dt=3600.0, river_threshold=2000, seed=1)
+    synth.add_floodplain(c, seed=8)
+    gpu = Model(c)
+    node = gpu.node_order()
+    inv = np.empty_like(node)
+    inv[node] = np.arange(node.size)
+    elev = torch.as_tensor(c.arrays["elevation"], device="cuda")[torch.as_tensor(node, device="cuda")].contiguous()
+    lab = synth.basin_ids(c.arrays["ldd"], c.nx, c.ny, c.arrays["cell_index"])
+    ids, counts = np.unique(lab, return_counts=True)
+    rng = np.random.default_rng(25)
+    small, total = [], 0
+    for k in rng.permutation(len(ids)):
+        if counts[k] < 20000 and total < 150000:
+            small.append(ids[k])
+            total += counts[k]
+    riv = c.arrays["river_location"].astype(bool)
+    parts = []
+    for name, basin_set, steps in (("small", np.array(small), 12), ("largest", ids[np.argmax(counts)][None], 2)):
+        mask = np.isin(lab, basin_set)
+        sel = torch.as_tensor(inv[np.nonzero(mask)[0]], device="cuda")
+        parts.append((name, mask, sel, mask[riv], O.OracleModel(c.subset(mask)), steps))
+    _say(f"floodplain 4096 basins set up in {time.time() - t0:.0f}s: {[(p[0], int(p[1].sum())) for p in parts]}")
+    flooded = 0
+    for k in range(12):
+        P, E, T = synth.forcing_torch(c, k, elev)
+        gpu.set_forcing(P, E, T, memkind=_abi.WFH_MEM_DEVICE_NODE_ORDER)
+        gpu.update_model()
+        for name, mask, sel, rmask, ref, steps in parts:
+            if k >= steps:
+                continue
+            ref.step(*(f[sel].cpu().numpy() for f in (P, E, T)))
+            if (k + 1) % 4 != 0 and k != steps - 1:
+                continue
+            for v in STATE_VARS + FLUX_VARS + FLOODPLAIN_VARS:
+                g = gpu.get(v)
+                if v in _abi.RIVER_VARS:
+                    g = g[rmask]
+                elif v in _abi.LAYERED_VARS:
+                    g = g.reshape(-1, c.n)[:, mask].ravel()
+                else:
+                    g = g[mask]
+                e = rel_err(g, ref.get(v))
+                assert e <= RTOL, (name, k, v.name, e)
+            flooded = max(flooded, int((ref.get(Var.FLOODPLAIN_STORAGE) > 0).sum()))
+        _say(f"floodplain 4096 step {k} done ({time.time() - t0:.0f}s)")
+    assert flooded > 0
+    gpu.close()
+    for p in parts:
+        p[4].close()
+
