@@ -15,7 +15,7 @@ import pytest
 import oracle as O
 from test_gpu_parity import FLUX_VARS, STATE_VARS, compare
 from test_gpu_reservoir import RES_VARS
-from wflowhip import Model, Var, synth
+from wflowhip import Model, Var, _abi, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -100,3 +100,75 @@ def test_linked_free_weirs_daily_and_adaptive():
     c = synth.make_catchment(96, cell=250.0, dt=3600.0, river_threshold=60, seed=53)
     synth.add_reservoirs(c, 12, seed=2, curves=True, linked=True)
     run_curves(c, 3, start_day=1, adaptive=True)
+
+
+def test_curves_4096_hourly_on_basins():
+    """The full 4096^2 hourly grid (250 m, river threshold 2000 cells) with 64 reservoirs of
+    outflow types 1-4 (rating curves, storage curves, one overflowing) on the GPU against the
+    oracle run on complete basins of it (no reservoir area crosses a basin divide, which the test
+    asserts, so a set of whole basins with their reservoirs is an exact sub-problem). The basins
+    of fewer than 1 M cells that hold reservoirs run 8 steps over the year's end, the 2 M-cell
+    basin with 13 reservoirs 2 steps. Every state, flux and reservoir variable must match within
+    RTOL."""
+    import time
+    import torch
+    from test_gpu_parity import RTOL, rel_err
+
+    t0 = time.time()
+    c = synth.make_catchment(4096, cell=250.0, dt=3600.0, river_threshold=2000, seed=1)
+    synth.add_reservoirs(c, 64, seed=1, radius=2, curves=True)
+    A = c.arrays
+    lab = synth.basin_ids(A["ldd"], c.nx, c.ny, A["cell_index"])
+    outlet, cov = A["reservoir_outlet"], A["reservoir_coverage"]
+    covered = np.nonzero(cov)[0]
+    assert np.array_equal(lab[covered], lab[outlet[cov[covered] - 1]])
+    ids, counts = np.unique(lab, return_counts=True)
+    size = dict(zip(ids.tolist(), counts.tolist()))
+    held = np.unique(lab[outlet])
+    small = np.array([b for b in held if size[int(b)] < 1000000])
+    mid = np.array([b for b in held if 1000000 <= size[int(b)] < 2500000])
+    assert small.size > 0 and mid.size == 1
+    gpu = Model(c)
+    node = gpu.node_order()
+    inv = np.empty_like(node)
+    inv[node] = np.arange(node.size)
+    elev = torch.as_tensor(A["elevation"], device="cuda")[torch.as_tensor(node, device="cuda")].contiguous()
+    riv = A["river_location"].astype(bool)
+    parts = []
+    for name, basin_set, steps in (("small", small, 8), ("mid", mid, 2)):
+        mask = np.isin(lab, basin_set)
+        keep = np.nonzero(mask[outlet])[0]
+        sel = torch.as_tensor(inv[np.nonzero(mask)[0]], device="cuda")
+        parts.append((name, mask, sel, mask[riv], keep, O.OracleModel(c.subset(mask)), steps))
+    _say(f"reservoirs 4096 basins set up in {time.time() - t0:.0f}s: "
+         f"{[(p[0], int(p[1].sum()), int(p[4].size)) for p in parts]}")
+    for k in range(8):
+        day = _day(k // 3, 364)
+        gpu.set_julian_day(day)
+        P, E, T = synth.forcing_torch(c, k, elev)
+        gpu.set_forcing(P, E, T, memkind=_abi.WFH_MEM_DEVICE_NODE_ORDER)
+        gpu.update_model()
+        for name, mask, sel, rmask, keep, ref, steps in parts:
+            if k >= steps:
+                continue
+            ref.set_julian_day(day)
+            ref.step(*(f[sel].cpu().numpy() for f in (P, E, T)))
+            if (k + 1) % 4 != 0 and k != steps - 1:
+                continue
+            for v in STATE_VARS + FLUX_VARS + RES_VARS:
+                g = gpu.get(v)
+                if v in _abi.RESERVOIR_VARS:
+                    g = g[keep]
+                elif v in _abi.RIVER_VARS:
+                    g = g[rmask]
+                elif v in _abi.LAYERED_VARS:
+                    g = g.reshape(-1, c.n)[:, mask].ravel()
+                else:
+                    g = g[mask]
+                e = rel_err(g, ref.get(v))
+                assert e <= RTOL, (name, k, day, v.name, e)
+        _say(f"reservoirs 4096 step {k} day {day} done ({time.time() - t0:.0f}s)")
+    gpu.close()
+    for p in parts:
+        p[5].close()
+
