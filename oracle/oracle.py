@@ -2,6 +2,11 @@
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
 cpu_baseline leg, never by the product package.
+
+Parity unpinned (no reference vector and no closed form): the functions listed as such in the
+header of oracle/wflow_oracle.c. Among them are the standalone soil infiltration, leakage and
+layer-depth updates, the land and subsurface network sweeps, the step end to end, reservoir
+area means, and elementary functions at the ulp level against Julia's Base.Math.
 """
 
 import ctypes

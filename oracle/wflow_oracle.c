@@ -11,10 +11,27 @@
  * include/wfh_math.h, the same elementary functions the HIP kernels use, so that ulp-level
  * differences cannot change Newton or Brooks-Corey iteration counts between the two.
  *
- * Configuration covered: sbm model, kinematic wave land/river/subsurface routing with fixed
- * sub-time steps, exponential Ksat profile, no reservoirs/floodplain/glacier/water demand,
- * optional HBV snow and frozen-soil infiltration reduction, Gash (dt >= 23 h) or modified
- * Rutter interception with a static canopy.
+ * Configuration covered: sbm model, kinematic wave land/river/subsurface routing with fixed or
+ * adaptive sub-time steps, exponential and exponential-constant Ksat profiles, reservoirs of
+ * outflow types 1-4 (linear or interpolated storage curves, linked free weirs), the 1-D river
+ * floodplain, lateral snow transport, optional HBV snow and frozen-soil infiltration
+ * reduction, Gash (dt >= 23 h) or modified Rutter interception with a static canopy. No
+ * glacier, paddy or water demand.
+ *
+ * Parity unpinned (no reference vector and no closed form; checked only by line-by-line review
+ * and by invariants: per-cell balance closure, restart equal to a continuous run, and results
+ * that do not depend on the sweep order):
+ *   - wfo_soil_actual_infiltration (soil.jl:987-1017) and wfo_soil_leakage (soil.jl:1118-1136)
+ *     as standalone functions, wfo_update_ustorelayerdepth (soil.jl:1213-1259) standalone, and
+ *     the composition wfo_soil_water_flow;
+ *   - the land and subsurface network sweeps inside wfo_step (kinwave_land_update!,
+ *     kinwave_subsurface_update!) and wfo_step end to end; the reference checks these only on
+ *     its Moselle test data, which is not available;
+ *   - reservoir area means (reservoir forcing in wfo_step): a left fold here, Julia's pairwise
+ *     `mean` in the reference, so the last bits are unpinned for areas of more than a few cells;
+ *   - in wfo_quantile, the fma and the isapprox(a, b) branch of Statistics.jl's interpolation;
+ *   - exp/log/pow/cbrt (include/wfh_math.h) at the ulp level against Julia's Base.Math, which
+ *     is not vendored (they are checked against libm and exact rationals instead).
  */
 #include "wflow_oracle.h"
 
