@@ -102,21 +102,23 @@ def test_linked_free_weirs_daily_and_adaptive():
     run_curves(c, 3, start_day=1, adaptive=True)
 
 
-def test_curves_4096_hourly_on_basins():
+@pytest.mark.parametrize("linked", [False, True])
+def test_curves_4096_hourly_on_basins(linked):
     """The full 4096^2 hourly grid (250 m, river threshold 2000 cells) with 64 reservoirs of
     outflow types 1-4 (rating curves, storage curves, one overflowing) on the GPU against the
     oracle run on complete basins of it (no reservoir area crosses a basin divide, which the test
     asserts, so a set of whole basins with their reservoirs is an exact sub-problem). The basins
     of fewer than 1 M cells that hold reservoirs run 8 steps over the year's end, the 2 M-cell
-    basin with 13 reservoirs 2 steps. Every state, flux and reservoir variable must match within
-    RTOL."""
+    basin with 13 reservoirs 2 steps. With linked=True, free weirs are linked to a lower
+    reservoir downstream (two such pairs lie in the 2 M-cell basin). Every state, flux and
+    reservoir variable must match within RTOL."""
     import time
     import torch
     from test_gpu_parity import RTOL, rel_err
 
     t0 = time.time()
     c = synth.make_catchment(4096, cell=250.0, dt=3600.0, river_threshold=2000, seed=1)
-    synth.add_reservoirs(c, 64, seed=1, radius=2, curves=True)
+    synth.add_reservoirs(c, 64, seed=1, radius=2, curves=True, linked=linked)
     A = c.arrays
     lab = synth.basin_ids(A["ldd"], c.nx, c.ny, A["cell_index"])
     outlet, cov = A["reservoir_outlet"], A["reservoir_coverage"]
@@ -140,7 +142,9 @@ def test_curves_4096_hourly_on_basins():
         keep = np.nonzero(mask[outlet])[0]
         sel = torch.as_tensor(inv[np.nonzero(mask)[0]], device="cuda")
         parts.append((name, mask, sel, mask[riv], keep, O.OracleModel(c.subset(mask)), steps))
-    _say(f"reservoirs 4096 basins set up in {time.time() - t0:.0f}s: "
+    if linked:
+        assert int((A["reservoir_lower"][parts[1][4]] > 0).sum()) == 2
+    _say(f"reservoirs 4096 (linked={linked}) basins set up in {time.time() - t0:.0f}s: "
          f"{[(p[0], int(p[1].sum()), int(p[4].size)) for p in parts]}")
     for k in range(8):
         day = _day(k // 3, 364)
