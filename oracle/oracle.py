@@ -4,9 +4,9 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.
 cpu_baseline leg, never by the product package.
 
 Parity unpinned (no reference vector and no closed form): the functions listed as such in the
-header of oracle/wflow_oracle.c. Among them are the standalone soil infiltration, leakage and
-layer-depth updates, the land and subsurface network sweeps, the step end to end, reservoir
-area means, and elementary functions at the ulp level against Julia's Base.Math.
+header of oracle/wflow_oracle.c: the soil-flow composition and the step end to end beyond the
+parts pinned by tests/test_oracle_network.py, reservoir area means, the quantile's fma branch, and
+elementary functions at the ulp level against Julia's Base.Math.
 """
 
 import ctypes
@@ -93,6 +93,8 @@ def lib():
             getattr(L, f).restype = ctypes.c_int64
             getattr(L, f).argtypes = [ctypes.c_void_p]
         L.wfo_nlayers.argtypes = [ctypes.c_void_p]
+        L.wfo_set_mutation.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+        L.wfo_set_mutation.restype = ctypes.c_int32
         L.wfo_topology.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 5
         L.wfo_toposort_dfs.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         dbl_fns = {
@@ -462,6 +464,12 @@ class OracleModel:
         """update_index_hq!  reservoir.jl:379-386."""
         if lib().wfo_set_julian_day(self.h, int(day)) != 0:
             raise ValueError(f"julian day {day} outside 1..365")
+
+    def set_mutation(self, mutation):
+        """Plant a known mistake in the sweeps (tests of the network checks only):
+        1 subsurface, 2 overland, 3 river fold the upstream discharge of the sub-step before."""
+        if lib().wfo_set_mutation(self.h, int(mutation)) != 0:
+            raise ValueError(f"unknown mutation {mutation}")
 
     def last_substeps(self):
         """(subsurface, overland, river) sub-step counts of the last step."""

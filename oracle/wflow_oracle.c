@@ -18,15 +18,25 @@
  * reduction, Gash (dt >= 23 h) or modified Rutter interception with a static canopy. No
  * glacier, paddy or water demand.
  *
+ * Pinned without a reference vector (tests/test_oracle_network.py):
+ *   - the network sweeps in wfo_step (kinwave_subsurface_update!, kinwave_land_update!,
+ *     kinwave_river_update!) by network conservation: the folded inflows equal the discharge of
+ *     the upstream cells of the same sub-step summed over the network, and the network storage
+ *     changes by the lateral inflow less the outflow; a planted stale fold (wfo_set_mutation)
+ *     fails them while every per-cell balance still closes; a chain reaches the closed-form
+ *     steady state q_k = (k + 1) q_lat dx;
+ *   - wfo_soil_actual_infiltration (one layer: closed form; several: conservation and the
+ *     closed form of the bottom-up cascade), wfo_soil_leakage (each limiting regime exactly),
+ *     wfo_update_ustorelayerdepth (water content kept on a rise, field-capacity water added on
+ *     a fall), wfo_kw_ssf_newton_raphson (the linear root in exact rationals) and
+ *     wfo_kinematic_wave (inside the root's exact-rational bracket widened by its stopping
+ *     residual).
+ *
  * Parity unpinned (no reference vector and no closed form; checked only by line-by-line review
  * and by invariants: per-cell balance closure, restart equal to a continuous run, and results
  * that do not depend on the sweep order):
- *   - wfo_soil_actual_infiltration (soil.jl:987-1017) and wfo_soil_leakage (soil.jl:1118-1136)
- *     as standalone functions, wfo_update_ustorelayerdepth (soil.jl:1213-1259) standalone, and
- *     the composition wfo_soil_water_flow;
- *   - the land and subsurface network sweeps inside wfo_step (kinwave_land_update!,
- *     kinwave_subsurface_update!) and wfo_step end to end; the reference checks these only on
- *     its Moselle test data, which is not available;
+ *   - the composition wfo_soil_water_flow and wfo_step end to end beyond the pinned parts above
+ *     (the land balance closes per cell, the reference's own bound);
  *   - reservoir area means (reservoir forcing in wfo_step): a left fold here, Julia's pairwise
  *     `mean` in the reference, so the last bits are unpinned for areas of more than a few cells;
  *   - in wfo_quantile, the fma and the isapprox(a, b) branch of Statistics.jl's interpolation;
@@ -1099,6 +1109,10 @@ struct wfo_model {
     /* adaptive sub-steps: stable time step buffer, sub-step counts of the last step */
     double* stbuf;
     int32_t nsub[3];
+    /* planted mistake for the tests of the network checks (wfo_set_mutation), and the copy of the
+     * upstream discharge at the start of the sub-step that the mutated folds read */
+    int32_t mutation;
+    double* stale;
     /* lateral snow transport */
     double *snow_in, *snow_out, *snow_flux_w, *snow_cap_s, *snow_cap_w;
     /* upstream lists without reservoir outlets (filter_upstream_nodes, utils.jl:61-71) */
@@ -1683,6 +1697,7 @@ wfo_model* wfo_create(const wfh_config* cfg, const wfh_static* st, char* err, in
     m->r_store_prev = dalloc(nriv); m->bal_land = dalloc(n); m->bal_land_rel = dalloc(n); m->bal_ssf = dalloc(n);
     m->bal_ol = dalloc(n); m->bal_r = dalloc(nriv);
     m->stbuf = dalloc(n > nriv ? n : nriv);
+    m->stale = dalloc(n > nriv ? n : nriv);
     m->snow_in = dalloc(n); m->snow_out = dalloc(n); m->snow_flux_w = dalloc(n);
     m->snow_cap_s = dalloc(n); m->snow_cap_w = dalloc(n);
     free(ldd2);
@@ -1719,7 +1734,7 @@ void wfo_destroy(wfo_model* m) {
         (void**)&m->r_qin_cum, (void**)&m->r_q_avg, (void**)&m->r_qin_avg, (void**)&m->r_area,
         (void**)&m->precip, (void**)&m->pet, (void**)&m->temp, (void**)&m->total_storage,
         (void**)&m->land_store_prev, (void**)&m->ol_store_prev, (void**)&m->r_store_prev, (void**)&m->bal_land,
-        (void**)&m->bal_land_rel, (void**)&m->bal_ssf, (void**)&m->bal_ol, (void**)&m->bal_r, (void**)&m->stbuf,
+        (void**)&m->bal_land_rel, (void**)&m->bal_ssf, (void**)&m->bal_ol, (void**)&m->bal_r, (void**)&m->stbuf, (void**)&m->stale,
         (void**)&m->fup_ptr, (void**)&m->fup_idx, (void**)&m->rfup_ptr, (void**)&m->rfup_idx, (void**)&m->res,
         (void**)&m->res_land, (void**)&m->res_river, (void**)&m->land_res, (void**)&m->river_res,
         (void**)&m->cov_ptr, (void**)&m->cov_idx, (void**)&m->land_cov, (void**)&m->res_store_prev,
@@ -2021,16 +2036,21 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
                 m->rech_flux_cum[i] += fl * dts;
                 m->qnet_bnds[i] += fl;
             }
+            const double* sq_up = m->ssf_q;
+            if (m->mutation == WFO_MUT_STALE_SSF) {
+                memcpy(m->stale, m->ssf_q, (size_t)n * sizeof(double));
+                sq_up = m->stale;
+            }
             for (int64_t k = 0; k < n; ++k) { /* kinwave_subsurface_update!  :198-273 */
                 int64_t v = m->order[k];
                 double qin = 0.0, tor = 0.0;
                 for (int64_t e = m->fup_ptr[v]; e < m->fup_ptr[v + 1]; ++e) {
                     int64_t u = m->fup_idx[e];
-                    qin = qin + m->ssf_q[u] * (1.0 - m->fftr[u]);
+                    qin = qin + sq_up[u] * (1.0 - m->fftr[u]);
                 }
                 for (int64_t e = m->fup_ptr[v]; e < m->fup_ptr[v + 1]; ++e) {
                     int64_t u = m->fup_idx[e];
-                    tor = tor + m->ssf_q[u] * m->fftr[u];
+                    tor = tor + sq_up[u] * m->fftr[u];
                 }
                 m->ssf_qin[v] = qin;
                 m->ssf_tor_cum[v] += tor * dts;
@@ -2078,19 +2098,24 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
             dts = check_timestepsize(dts, t, dt);
             ++m->nsub[1];
             for (int64_t i = 0; i < n; ++i) m->ol_qin[i] = 0.0;
+            const double* oq_up = m->ol_q;
+            if (m->mutation == WFO_MUT_STALE_LAND) {
+                memcpy(m->stale, m->ol_q, (size_t)n * sizeof(double));
+                oq_up = m->stale;
+            }
             for (int64_t k = 0; k < n; ++k) { /* kinwave_land_update!  :293-341 */
                 int64_t v = m->order[k];
                 double tor = 0.0;
                 for (int64_t e = m->fup_ptr[v]; e < m->fup_ptr[v + 1]; ++e) {
                     int64_t u = m->fup_idx[e];
-                    tor = tor + m->ol_q[u] * m->fftr[u];
+                    tor = tor + oq_up[u] * m->fftr[u];
                 }
                 m->ol_tor_cum[v] += tor * dts;
                 if (m->sfw[v] > 0.0) {
                     double qin = 0.0;
                     for (int64_t e = m->fup_ptr[v]; e < m->fup_ptr[v + 1]; ++e) {
                         int64_t u = m->fup_idx[e];
-                        qin = qin + m->ol_q[u] * (1.0 - m->fftr[u]);
+                        qin = qin + oq_up[u] * (1.0 - m->fftr[u]);
                     }
                     m->ol_qin[v] = qin;
                 }
@@ -2144,10 +2169,15 @@ int32_t wfo_step(wfo_model* m, const double* precip, const double* pet, const do
             ++m->nsub[2];
             if (m->fp) river_floodplain_exchange(m, dts);
             for (int64_t r = 0; r < nriv; ++r) m->r_qin[r] = 0.0;
+            const double* rq_up = m->r_q;
+            if (m->mutation == WFO_MUT_STALE_RIVER) {
+                memcpy(m->stale, m->r_q, (size_t)nriv * sizeof(double));
+                rq_up = m->stale;
+            }
             for (int64_t k = 0; k < nriv; ++k) { /* kinwave_river_update!  :492-566 */
                 int64_t v = m->rorder[k];
                 double s = 0.0;
-                for (int64_t e = m->rfup_ptr[v]; e < m->rfup_ptr[v + 1]; ++e) s = s + m->r_q[m->rfup_idx[e]];
+                for (int64_t e = m->rfup_ptr[v]; e < m->rfup_ptr[v + 1]; ++e) s = s + rq_up[m->rfup_idx[e]];
                 m->r_qin[v] += s;
                 double inflow = 0.0 / m->r_fl[v];
                 inflow -= 0.0 / m->r_fl[v];
@@ -2247,6 +2277,11 @@ int64_t wfo_n(const wfo_model* m) { return m->n; }
 int64_t wfo_nriv(const wfo_model* m) { return m->nriv; }
 int32_t wfo_nlayers(const wfo_model* m) { return m->N; }
 int64_t wfo_kinwave_iterations(const wfo_model* m) { return m->kw_iters; }
+int32_t wfo_set_mutation(wfo_model* m, int32_t mutation) {
+    if (mutation < WFO_MUT_NONE || mutation > WFO_MUT_STALE_RIVER) return -1;
+    m->mutation = mutation;
+    return 0;
+}
 int32_t wfo_set_julian_day(wfo_model* m, int32_t julian_day) {
     if (julian_day < 1 || julian_day > 365) return -1;
     for (int64_t k = 0; k < m->nres; ++k) m->res[k].hq_col = julian_day - 1; /* update_index_hq! */

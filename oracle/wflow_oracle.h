@@ -93,6 +93,12 @@ int32_t wfo_nlayers(const wfo_model* m);
 int32_t wfo_topology(const wfo_model* m, int64_t* land_order, int64_t* land_down, int64_t* river_order,
                      int64_t* river_down, int64_t* river_land_index);
 int64_t wfo_kinwave_iterations(const wfo_model* m);
+/* A planted mistake, for tests that show their checks catch it (never set otherwise): the
+ * subsurface / overland / river sweep folds the upstream discharge of the sub-step before
+ * (the discharge at the start of the sweep) instead of the one its upstream cells have just
+ * computed. Returns 0, or -1 for an unknown code. */
+enum { WFO_MUT_NONE = 0, WFO_MUT_STALE_SSF = 1, WFO_MUT_STALE_LAND = 2, WFO_MUT_STALE_RIVER = 3 };
+int32_t wfo_set_mutation(wfo_model* m, int32_t mutation);
 /* subsurface, overland and river sub-step counts of the last wfo_step */
 void wfo_last_substeps(const wfo_model* m, int32_t* out3);
 /* FloodPlainProfile  floodplain.jl:48-147 for one river cell: nd0 flood depths (ascending, without
