@@ -301,6 +301,18 @@ int32_t wfh_set_forcing(wfh_handle* h, const double* precipitation, const double
  * one back before it launches the sub-step: the call returns when the step has completed. */
 int32_t wfh_step(wfh_handle* h);
 
+/* wfh_step without any device allocation by the library, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It has no
+ * counterpart in the reference. The routing scratch of the step (wave tickets, abort flag and
+ * one outflow slot per cell and sub-step) comes from a fixed device array of 64 MiB that every GPU
+ * the library runs on holds until the process ends, instead of the handle's own buffers; the
+ * results are those of wfh_step. Limit: (S_s + S_l) * n + S_r * max(nriv, 1) + 32 words of 8 bytes
+ * must fit in 64 MiB (S_* = sub-steps per step), else WFH_ERR_ARG. Fixed sub-steps only: adaptive
+ * sub-steps, a floodplain or linked free weirs return WFH_ERR_ARG. Every call on one GPU shares
+ * that GPU's array and clears it first, so calls that overlap on one GPU (two handles stepped on
+ * concurrently) race; serialise them. Asynchronous like wfh_step; wfh_sync reports an abort. */
+int32_t wfh_step_noalloc(wfh_handle* h);
+
 /* Julian day (1..365; February 29 counts as March 1) that selects the rating-curve column of
  * reservoirs with outflow type 1 in the following steps; 1 after wfh_create. The reference sets
  * it from the step's start time before the river update: julian_day(clock.time - clock.dt).

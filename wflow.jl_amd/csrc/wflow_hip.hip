@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -3452,32 +3453,39 @@ static hipEvent_t* timing_events(wfh_handle* h) {
 }
 
 // lateral_snow_transport! and flux_in! (between the vertical update and the routing)
-static void launch_snow_transport(wfh_handle* h) {
-    RouteArgs a = h->ra;
-    a.parity = h->parity;
-    h->parity ^= 1;
-    hipLaunchKernelGGL(k_snow_transport, dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, a);
-    hipLaunchKernelGGL(k_snow_in, dim3((unsigned)((h->n + 255) / 256)), dim3(256), 0, h->stream, h->d);
+static void launch_snow_transport(wfh_handle* h, const Dev& d, RouteArgs a, int& parity) {
+    a.parity = parity;
+    parity ^= 1;
+    hipLaunchKernelGGL(k_snow_transport, dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, d, a);
+    hipLaunchKernelGGL(k_snow_in, dim3((unsigned)((h->n + 255) / 256)), dim3(256), 0, h->stream, d);
+}
+static void launch_snow_transport(wfh_handle* h) { launch_snow_transport(h, h->d, h->ra, h->parity); }
+
+// The kernels of one fixed-sub-step step on the handle's stream, with the routing scratch
+// (slots, tickets, abort flag) that `a` and `d.abort_flag` point to; `parity` selects the ticket
+// counter of the first dataflow launch and is advanced per launch.
+template <int N>
+static void launch_step(wfh_handle* h, const Dev& d, const RouteArgs& base, int& parity, hipEvent_t* ev) {
+    const int n = (int)h->n;
+    if (ev) hipEventRecord(ev[0], h->stream);
+    if (h->nres > 0)
+        hipLaunchKernelGGL(k_res_forcing, dim3((unsigned)((h->nres + 63) / 64)), dim3(64), 0, h->stream, d);
+    if (!FUSE_VERTICAL) hipLaunchKernelGGL(k_vertical<N>, dim3((n + 255) / 256), dim3(256), 0, h->stream, d);
+    if (ev) hipEventRecord(ev[1], h->stream);
+    if (d.snow_lat) launch_snow_transport(h, d, base, parity);
+    RouteArgs a = base;
+    a.parity = parity;
+    parity ^= 1;
+    if (h->ext)
+        hipLaunchKernelGGL((k_route<N, true>), dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, d, a);
+    else
+        hipLaunchKernelGGL((k_route<N, false>), dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, d, a);
+    if (ev) hipEventRecord(ev[2], h->stream);
 }
 
 template <int N>
 static void launch_kernels(wfh_handle* h) {
-    const int n = (int)h->n;
-    hipEvent_t* ev = timing_events(h);
-    if (ev) hipEventRecord(ev[0], h->stream);
-    if (h->nres > 0)
-        hipLaunchKernelGGL(k_res_forcing, dim3((unsigned)((h->nres + 63) / 64)), dim3(64), 0, h->stream, h->d);
-    if (!FUSE_VERTICAL) hipLaunchKernelGGL(k_vertical<N>, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d);
-    if (ev) hipEventRecord(ev[1], h->stream);
-    if (h->d.snow_lat) launch_snow_transport(h);
-    RouteArgs a = h->ra;
-    a.parity = h->parity;
-    h->parity ^= 1;
-    if (h->ext)
-        hipLaunchKernelGGL((k_route<N, true>), dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, a);
-    else
-        hipLaunchKernelGGL((k_route<N, false>), dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, a);
-    if (ev) hipEventRecord(ev[2], h->stream);
+    launch_step<N>(h, h->d, h->ra, h->parity, timing_events(h));
 }
 
 // ---- adaptive sub-steps ----
@@ -3699,6 +3707,89 @@ extern "C" int32_t wfh_step(wfh_handle* h) {
         case 5: launch_kernels<5>(h); break;
         default: launch_kernels<6>(h); break;
     }
+    HIPCHK(h, hipGetLastError());
+    h->nsub[0] = h->S_s;
+    h->nsub[1] = h->S_l;
+    h->nsub[2] = h->S_r;
+    return WFH_OK;
+}
+
+// ---- allocation-free step (wfh_step_noalloc) ----
+// The routing scratch of the fixed-sub-step step (ticket counters, abort flag, one outflow slot per
+// cell and sub-step) taken from a fixed device array instead of the handle's allocations. Every
+// GPU the library runs on holds its own copy for the life of the process; its device address is
+// looked up once per device (hipGetSymbolAddress) and only that address is used from the host.
+// Layout in 64-bit words: [0], [1] ticket counters, [2] abort flag (low 32 bits), [3..31] unused,
+// then slot_ssf [S_s][n], slot_ol [S_l][n], slot_r [S_r][max(nriv, 1)].
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+constexpr size_t NOALLOC_HEAD_WORDS = 32;
+static __device__ __align__(256) uint64_t scratch[NOALLOC_SCRATCH_BYTES / sizeof(uint64_t)];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint64_t* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Every call starts from a cleared scratch: the previous call (of any handle on this GPU) leaves
+// its counters behind, and a call that aborted may leave slots filled.
+__global__ void k_scratch_init(uint64_t* s, size_t words) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+        s[i] = i < NOALLOC_HEAD_WORDS ? 0ull : SLOT_EMPTY;
+}
+// hands an abort raised in the fixed scratch on to the handle's own flag, which wfh_sync reads
+__global__ void k_flag_or(int* dst, const int* src) {
+    if (*src) *dst = 1;
+}
+}  // namespace
+
+extern "C" int32_t wfh_step_noalloc(wfh_handle* h) {
+    if (!h) return WFH_ERR_ARG;
+    if (h->broken) return fail(h, WFH_ERR_STATE, "handle is in an error state (a previous step failed)");
+    if (!h->forcing_set) return fail(h, WFH_ERR_STATE, "wfh_set_forcing must be called before wfh_step_noalloc");
+    if (h->cfg.kinematic_wave_adaptive || h->d.fp || h->d.res_linked)
+        return fail(h, WFH_ERR_ARG,
+                    "wfh_step_noalloc covers fixed sub-steps only (no adaptive sub-steps, floodplain or linked "
+                    "free weirs)");
+    const int dev = h->cfg.device;
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(h, WFH_ERR_ARG, "device ordinal out of range");
+    const size_t nslot = (size_t)(h->S_s + h->S_l) * (size_t)h->n + (size_t)h->S_r * (size_t)std::max<int64_t>(h->nriv, 1);
+    const size_t words = NOALLOC_HEAD_WORDS + nslot;
+    if (words * sizeof(uint64_t) > NOALLOC_SCRATCH_BYTES)
+        return fail(h, WFH_ERR_ARG,
+                    "wfh_step_noalloc: the model needs " + std::to_string(words * sizeof(uint64_t)) +
+                        " bytes of routing scratch, above the fixed limit of " +
+                        std::to_string(NOALLOC_SCRATCH_BYTES) + " bytes; use wfh_step");
+    HIPCHK(h, hipSetDevice(dev));
+    uint64_t* s = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            void* p = nullptr;
+            HIPCHK(h, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            if (!p) return fail(h, WFH_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+            g_scratch_ptr[dev] = (uint64_t*)p;
+        }
+        s = g_scratch_ptr[dev];
+    }
+    const int g = (int)std::min<size_t>((words + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_scratch_init, dim3(g), dim3(256), 0, h->stream, s, words);
+    RouteArgs a = h->ra;
+    a.counters = (unsigned long long*)s;
+    a.abort_flag = (int*)(s + 2);
+    a.slot_ssf = s + NOALLOC_HEAD_WORDS;
+    a.slot_ol = a.slot_ssf + (size_t)h->S_s * h->n;
+    a.slot_r = a.slot_ol + (size_t)h->S_l * h->n;
+    Dev d = h->d;
+    d.abort_flag = a.abort_flag;
+    int parity = 0;
+    switch (h->N) {
+        case 2: launch_step<2>(h, d, a, parity, nullptr); break;
+        case 3: launch_step<3>(h, d, a, parity, nullptr); break;
+        case 4: launch_step<4>(h, d, a, parity, nullptr); break;
+        case 5: launch_step<5>(h, d, a, parity, nullptr); break;
+        default: launch_step<6>(h, d, a, parity, nullptr); break;
+    }
+    hipLaunchKernelGGL(k_flag_or, dim3(1), dim3(1), 0, h->stream, h->ra.abort_flag, (const int*)a.abort_flag);
     HIPCHK(h, hipGetLastError());
     h->nsub[0] = h->S_s;
     h->nsub[1] = h->S_l;
@@ -4151,3 +4242,4 @@ extern "C" int32_t wfh_kernel_bytes(wfh_handle* h, int64_t* vertical_bytes, int6
     if (routing_bytes) *routing_bytes = k2;
     return WFH_OK;
 }
+

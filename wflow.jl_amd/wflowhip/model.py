@@ -39,6 +39,8 @@ def load_library():
     L.wfh_set_forcing.restype = i32
     L.wfh_step.argtypes = [P]
     L.wfh_step.restype = i32
+    L.wfh_step_noalloc.argtypes = [P]
+    L.wfh_step_noalloc.restype = i32
     L.wfh_get.argtypes = [P, i32, P, i64, i32]
     L.wfh_get.restype = i32
     L.wfh_set.argtypes = [P, i32, P, i64, i32]
@@ -135,6 +137,11 @@ class Model:
         """update_model! (asynchronous on the model's stream)."""
         self._check(self.lib.wfh_step(self.h))
 
+    def update_model_noalloc(self):
+        """update_model! through wfh_step_noalloc: the routing scratch comes from the library's
+        fixed per-device array, so the step allocates nothing (asynchronous like update_model)."""
+        self._check(self.lib.wfh_step_noalloc(self.h))
+
     def run_timestep(self, precipitation, potential_evaporation, temperature):
         """run_timestep! with the step's forcing supplied by the caller."""
         self.set_forcing(precipitation, potential_evaporation, temperature)
@@ -220,3 +227,4 @@ class Model:
         a, b = ctypes.c_double(), ctypes.c_double()
         self._check(self.lib.wfh_get_timing(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
