@@ -367,6 +367,48 @@ int32_t wfh_kernel_bytes(wfh_handle* h, int64_t* vertical_bytes, int64_t* routin
 const char* wfh_last_error(const wfh_handle* h);
 void wfh_destroy(wfh_handle* h);
 
+/* ---- Parameter ensembles -------------------------------------------------------------------
+ * M members of one catchment that differ in three parameter maps, each scaled by a factor per
+ * member: kv_0 (KsatVer), f and Manning n (land and river). No counterpart in the reference,
+ * which runs one parameter set per model: a member's step is update_model! (sbm_model.jl:60-92)
+ * of the model built from the scaled maps (Model(config), sbm_model.jl:7-57), and its results are
+ * those of wfh_step on that model. All members are stepped together: states are held
+ * member-innermost (cell v, member m at v * M + m) and the step's launches run from a HIP graph.
+ * Fixed sub-steps with one overland and one subsurface sub-step per step (land_dt and
+ * subsurface_dt >= dt) and at most 8 river sub-steps; no reservoirs, floodplain, snow transport
+ * or exponential-constant profile. members: a multiple of 64, at most 4096; factors: [members],
+ * finite and positive. */
+typedef struct wfh_ensemble wfh_ensemble;
+int32_t wfh_ens_create(const wfh_config* cfg, const wfh_static* st, int32_t members, const double* kv0_factor,
+                       const double* f_factor, const double* n_factor, wfh_ensemble** out);
+/* Forcing of the next step, shared by all members (as wfh_set_forcing; always copied). */
+int32_t wfh_ens_set_forcing(wfh_ensemble* x, const double* precipitation, const double* potential_evaporation,
+                            const double* temperature, int32_t memkind);
+/* One update_model! of every member (asynchronous; the graph of the current schedule). */
+int32_t wfh_ens_step(wfh_ensemble* x);
+/* Member m's field (land-index or river-index order) of a state or river output: canopy, snow,
+ * soil temperature, satwaterdepth, ustorelayerdepth, subsurface q and q_max, overland q, h and
+ * alpha, river q, h, storage, q average and alpha. Synchronises. */
+int32_t wfh_ens_get(wfh_ensemble* x, int32_t var, int32_t member, double* dst, int64_t count);
+/* Largest |land balance error| of the last step per member ([members]; water_mass_balance). */
+int32_t wfh_ens_balance(wfh_ensemble* x, double* max_abs_error);
+/* Schedule of the step: threads per block (64, 128, 256), the level width from which a level is
+ * a stage of its own, and cells per strand in such stages. Results do not depend on it. */
+int32_t wfh_ens_set_schedule(wfh_ensemble* x, int32_t block, int32_t wide, int32_t strand_cells);
+/* Autotuner: the next steps run the candidate schedules in turn, `trials` steps each, timed; the
+ * fastest is then kept (trials = 0 stops tuning). The steps are real steps of the model. */
+int32_t wfh_ens_set_autotune(wfh_ensemble* x, int32_t trials);
+int32_t wfh_ens_get_schedule(wfh_ensemble* x, int32_t* block, int32_t* wide, int32_t* strand_cells, int32_t* stages,
+                             int32_t* tuning);
+int32_t wfh_ens_sync(wfh_ensemble* x);
+/* Step timing (HIP events around the graph launch; synchronises every step while enabled). */
+int32_t wfh_ens_set_timing(wfh_ensemble* x, int32_t enable);
+int32_t wfh_ens_get_timing(wfh_ensemble* x, double* step_ms);
+/* Algorithmic HBM bytes of one ensemble step (the roofline of the measurement harness). */
+int32_t wfh_ens_step_bytes(wfh_ensemble* x, int64_t* bytes);
+const char* wfh_ens_last_error(const wfh_ensemble* x);
+void wfh_ens_destroy(wfh_ensemble* x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,7 @@
 #include "wflow_hip.h"
 #include "wfh_math.h"
 #include "wfh_physics.h"
+#include "wfh_column.h"
 
 using namespace wfh;
 
@@ -209,306 +210,88 @@ struct VertOut {
 template <int N>
 __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut<N>& o) {
     const int n = d.n;
-    const double dt = d.dt;
     // Every input of the cell is loaded here, in one block, unconditionally (parameters of
     // switched-off processes are allocated as zeros): loads spread over the branches below
     // would each be waited for at the branch's end, one memory round trip after another.
     const double P_in = d.P[i], E_in = d.E[i], T = d.T[i];
     const uint8_t fl = d.nres > 0 ? d.flags[i] : 0;
-    const double canopy0 = d.canopy[i], ss0 = d.snow_s[i], sw0 = d.snow_w[i], tsws0 = d.tsws[i];
-    const double gap = ld(d.gap, i), smax = ld(d.smax, i), kc = ld(d.kc, i), er = ld(d.er, i);
-    const double p_tti = ld(d.tti, i), p_tt = ld(d.tt, i), p_ttm = ld(d.ttm, i), p_cfmax = ld(d.cfmax, i);
-    const double p_whc = ld(d.whc, i), p_wsoil = ld(d.wsoil, i), p_cfsoil = ld(d.cfsoil, i);
+    ColState<N> s;
+    s.canopy = d.canopy[i];
+    s.ss = d.snow_s[i];
+    s.sw = d.snow_w[i];
+    s.tsws = d.tsws[i];
+    ColParams p;
+    p.gap = ld(d.gap, i), p.smax = ld(d.smax, i), p.kc = ld(d.kc, i), p.er = ld(d.er, i);
+    p.tti = ld(d.tti, i), p.tt = ld(d.tt, i), p.ttm = ld(d.ttm, i), p.cfmax = ld(d.cfmax, i);
+    p.whc = ld(d.whc, i), p.wsoil = ld(d.wsoil, i), p.cfsoil = ld(d.cfsoil, i);
     const int r = d.river_of[i];
     const double wd_land = d.ol_h[i];
     const double r_h0 = d.r_h[r >= 0 ? r : 0];
-    const double rf = ld(d.rf, i), wf = ld(d.wf, i), p_sfrac = ld(d.sfrac, i);
-    const double ts = ld(d.ts, i), tr = ld(d.tr, i), tfc = ld(d.tfc, i), dsoil = ld(d.d, i);
-    const double kv0 = ld(d.kv0, i), fk = ld(d.f, i), zexp = ld(d.zexp, i);
-    const double swd = d.swd[i], tsoil0 = d.tsoil[i];
-    const double compf = ld(d.compf, i), icap_s = ld(d.icap_s, i), icap_c = ld(d.icap_c, i);
-    const double rd = ld(d.rd, i), p_h3h = ld(d.h3h, i), p_h3l = ld(d.h3l, i);
-    const double h1 = ld(d.h1, i), h2 = ld(d.h2, i), h4 = ld(d.h4, i), ah1 = ld(d.ah1, i), hb = ld(d.hb, i);
-    const double p_rdp = ld(d.rdp, i), caph = ld(d.caph, i), capn = ld(d.capn, i), maxleak = ld(d.maxleak, i);
+    p.rf = ld(d.rf, i), p.wf = ld(d.wf, i), p.sfrac = ld(d.sfrac, i);
+    p.ts = ld(d.ts, i), p.tr = ld(d.tr, i), p.tfc = ld(d.tfc, i), p.dsoil = ld(d.d, i);
+    p.kv0 = ld(d.kv0, i), p.fk = ld(d.f, i), p.zexp = ld(d.zexp, i);
+    s.swd = d.swd[i], s.tsoil = d.tsoil[i];
+    p.compf = ld(d.compf, i), p.icap_s = ld(d.icap_s, i), p.icap_c = ld(d.icap_c, i);
+    p.rd = ld(d.rd, i), p.h3h = ld(d.h3h, i), p.h3l = ld(d.h3l, i);
+    p.h1 = ld(d.h1, i), p.h2 = ld(d.h2, i), p.h4 = ld(d.h4, i), p.ah1 = ld(d.ah1, i), p.hb = ld(d.hb, i);
+    p.rdp = ld(d.rdp, i), p.caph = ld(d.caph, i), p.capn = ld(d.capn, i), p.maxleak = ld(d.maxleak, i);
     // precipitation and potential evaporation over a reservoir's area go to the reservoir
     // (update_forcing!  io.jl:125-143)
     const double P = (fl & 8) ? 0.0 : P_in, E = (fl & 8) ? 0.0 : E_in;
-    double us[N], kvf[N], cbc[N], rfr[N];
+    ColLayers<N> L;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        us[k] = d.ustore[k * n + i];
-        kvf[k] = ld(d.kvfrac, k, i);
-        cbc[k] = ld(d.c, k, i);
-        rfr[k] = ld(d.rootfrac, k, i);
+        s.us[k] = d.ustore[k * n + i];
+        L.kvf[k] = ld(d.kvfrac, k, i);
+        L.cbc[k] = ld(d.c, k, i);
+        L.rfr[k] = ld(d.rootfrac, k, i);
     }
-
-    // storage_prev! (Wflow/src/mass_balance.jl:96-115,170-183) from last step's states
-    double canopy = canopy0;
-    double ss = d.snow ? ss0 : 0.0, sw = d.snow ? sw0 : 0.0;
-    double store_prev = 0.0;
-    if (d.balance) {
-        const double snow = d.snow ? ss + sw : 0.0 + 0.0;
-        store_prev = tsws0 + canopy + snow + 0.0 * 0.0 + 0.0;
-        d.store_prev[i] = store_prev;
-    }
-
-    // ---- interception: canopy.jl:54-98 (Gash, dt >= 23 h) / :115-143 (modified Rutter) ----
-    const double cpe = kc * E * (1.0 - gap);
-    double tf, ir, sf;
-    if (d.gash) {
-        interception_gash(smax, er, gap, P, cpe, dt, tf, ir, sf);
-    } else {
-        interception_modrut(P, cpe, gap, smax, dt, canopy, tf, ir, sf);
-    }
-
-    // ---- snow: snow.jl:123-177 ----
-    double wfs;
-    if (d.snow) {
-        double swe, sro;
-        snow_hbv(tf + sf, T, p_tti, p_tt, p_ttm, p_cfmax, p_whc, dt, ss, sw, swe, sro);
-        d.swe[i] = swe;
-        wfs = sro + 0.0 * 0.0;  // get_water_flux_surface! with snow (runoff.jl:49-58), no glacier
-    } else {
-        wfs = tf + sf;  // runoff.jl:37-46
-    }
-
-    // ---- open water runoff: runoff.jl:61-111 ----
-    const double wd_river = r >= 0 ? r_h0 : 0.0;
-    const double rr = wfh_min(1.0, rf) * wfs;
-    const double rl = wfh_min(1.0, wf) * wfs;
-    const double aor = rf * wfh_min(wd_river / dt, E);
-    const double aol = wf * wfh_min(wd_land / dt, E);
-    const double nrr = rr - aor;
-
-    // ---- soil boundary conditions: soil.jl:643-682 ----
-    const double pot_trans = wfh_max(0.0, cpe - ir);
-    double pot_sevap = p_sfrac * E;
-    pot_sevap = pot_sevap - 0.0;
-    const double wfs_soil = wfh_max(((wfs + 0.0) - rr) - rl + 0.0 / dt, 0.0);
-
-    // ---- update_soil_water_flow!  soil.jl:1150-1211 ----
-    double act[N], cum[N + 1];
-    const int nl = soil_layers<N>(dsoil, d.cfg_cum, d.cfg_thick, act, cum);
-    const double swc = dsoil * (ts - tr);
-    // update_diagnostic_vars!  soil.jl:1400-1436
-    double usd = 0.0;
-    {
-        bool first = true;
-#pragma unroll
-        for (int k = 0; k < N; ++k)
-            if (k < nl) {
-                usd = first ? us[k] : usd + us[k];
-                first = false;
-            }
-    }
-    const double zi = wfh_max(0.0, dsoil - swd / (ts - tr));
-    const double thd = drainable_porosity(ts, tfc);
-    double drainable = (dsoil - zi) * thd;
-    double ucap = swc - swd - usd;
-    double th[N];
-    const int nu = set_layerthickness<N>(zi, cum, act, th);
-    // soil_temperature!, infiltration_reduction_factor!  soil.jl:685-730
-    double tsoil = tsoil0;
-    if (d.snow) tsoil = tsoil + p_wsoil * (T - tsoil);
-    double fred = 1.0;
-    if (d.snow && d.infred) {
-        const double cf = p_cfsoil;
-        const double bb = 1.0 / (1.0 - cf);
-        fred = scurve(tsoil, 0.0 + 273.15, bb, 8.0) + cf;
-    }
-    // infiltration  soil_process.jl:16-41
-    double infilt, inf_exc;
-    {
-        const double soilinf = wfs_soil * (1.0 - compf);
-        const double pathinf = wfs_soil * compf;
-        const double ms = wfh_min(icap_s * fred, soilinf);
-        const double mp = wfh_min(icap_c * fred, pathinf);
-        infilt = wfh_min(mp + ms, wfh_max(0.0, ucap / dt));
-        inf_exc = (soilinf - ms) + (pathinf - mp);
-    }
-    // unsaturated_zone_flow!  soil.jl:764-804
-    double transfer = 0.0;
-    if (nu > 0) {
-        double z = 0.0, flow = 0.0;
-#pragma unroll
-        for (int m = 0; m < N; ++m) {
-            if (m >= nu) continue;
-            z = (m == 0) ? th[0] : z + th[m];
-            const double l_sat = th[m] * (ts - tr);
-            const double kvz = kvf[m] * kv0 * wfh_exp(-fk * (z < zexp ? z : zexp));
-            double u = (m == 0) ? us[m] + infilt * dt : us[m] + flow * dt;
-            unsatzone_flow_layer(u, kvz, l_sat, cbc[m], dt, flow);
-            us[m] = u;
-        }
-        transfer = flow;
-    }
-    // soil_evaporation!  soil.jl:814-856
-    double sevap, ses;
-    {
-        double pot = pot_sevap;
-        double seu;
-        if (nu == 0)
-            seu = 0.0;
-        else if (nu == 1)
-            seu = pot * wfh_min(1.0, us[0] / (zi * (ts - tr)));
-        else
-            seu = pot * wfh_min(1.0, us[0] / (th[0] * (ts - tr)));
-        seu = wfh_min(seu, us[0] / dt);
-        pot -= seu;
-        us[0] = us[0] - seu * dt;
-        if (nu == 0 || nu == 1) {
-            const double e = pot * wfh_min(1.0, (act[0] - zi) / act[0]);
-            ses = wfh_min(e, (act[0] - zi) * thd / dt);
-        } else {
-            ses = 0.0;
-        }
-        sevap = seu + ses;
-        drainable -= ses * dt;
-    }
-    // transpiration!  soil.jl:865-975
-    double ae_u = 0.0, ae_s, transp;
-    {
-        const double h3 = feddes_h3(p_h3h, p_h3l, pot_trans);
-        double sum_rf = 0.0, rf_low = 0.0;
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-            if (k >= nu) continue;
-            double rfu;
-            if (k == nu - 1 && zi < rd) {
-                const double rootlength = wfh_min(act[k], rd - cum[k]);
-                rfu = rfr[k] * (th[k] / rootlength);
-            } else {
-                rfu = rfr[k];
-            }
-            sum_rf += rfu;
-            rf_low = rfu;
-        }
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-            if (k >= nu) continue;
-            const double rfu = (k < nu - 1) ? rfr[k] : rf_low;
-            const double rfs = rd > 0.0 ? wfh_max((1.0 / sum_rf), 1.0) * rfu : 0.0;
-            const double vwc = wfh_max(us[k] / th[k], 1e-7);
-            const double head = head_brooks_corey(vwc, ts, tr, cbc[k], hb);
-            const double alpha = rwu_reduction_feddes(head, h1, h2, h3, h4, ah1);
-            const double availcap = wfh_min(1.0, wfh_max(0.0, (rd - cum[k]) / th[k]));
-            const double maxextr = us[k] * availcap / dt;
-            const double layer = wfh_min(alpha * rfs * pot_trans, maxextr);
-            us[k] = us[k] - layer * dt;
-            ae_u += layer;
-        }
-        const double wetroots = scurve(zi, rd, 1.0, p_rdp);
-        const double alpha0 = rwu_reduction_feddes(0.0, h1, h2, h3, h4, ah1);
-        const double rest = pot_trans - ae_u;
-        ae_s = wfh_min(rest * wetroots * alpha0, drainable / dt);
-        drainable -= ae_s * dt;
-        transp = ae_u + ae_s;
-    }
-    // actual_infiltration!  soil.jl:987-1017
-    double act_inf;
-    {
-        double excess = 0.0;
-#pragma unroll
-        for (int k = N - 1; k >= 0; --k) {
-            if (k >= nu) continue;
-            excess = wfh_max(0.0, us[k] - th[k] * (ts - tr));
-            us[k] = us[k] - excess;
-            if (k > 0) us[k - 1] = us[k - 1] + excess;
-        }
-        act_inf = infilt - excess / dt;
-    }
-    const double sat_exc = (wfs_soil - act_inf) - inf_exc;
-    // unsaturated_store_depth! + capacity  soil.jl:1194-1196
-    {
-        bool first = true;
-#pragma unroll
-        for (int k = 0; k < N; ++k)
-            if (k < nl) {
-                usd = first ? us[k] : usd + us[k];
-                first = false;
-            }
-        if (nl <= 0) usd = 0.0;
-    }
-    ucap = swc - swd - usd;
-    // capillary_flux!  soil.jl:1050-1111
-    double capflux = 0.0;
-    if (nu > 0) {
-        double kvf_nu = 0.0;
-#pragma unroll
-        for (int k = 0; k < N; ++k)
-            if (k == nu - 1) kvf_nu = kvf[k];
-        const double ksat = kvf_nu * kv0 * wfh_exp(-fk * (zi < zexp ? zi : zexp));
-        const double mx = wfh_max(0.0, wfh_min(wfh_min(wfh_min(ksat, ae_u), ucap / dt), drainable / dt));
-        double cap;
-        if (zi > rd)
-            cap = mx * wfh_pow(1.0 - wfh_min(zi, caph) / (caph), capn);
-        else
-            cap = 0.0;
-        double net = cap;
-#pragma unroll
-        for (int k = N - 1; k >= 0; --k) {
-            if (k >= nu) continue;
-            const double toadd = wfh_min(net, wfh_max((th[k] * (ts - tr) - us[k]) / dt, 0.0));
-            us[k] = us[k] + toadd * dt;
-            net -= toadd;
-            capflux += toadd;
-        }
-    }
-    // leakage!  soil.jl:1118-1136
-    double leak;
-    {
-        double kvf_nl = 0.0;
-#pragma unroll
-        for (int k = 0; k < N; ++k)
-            if (k == nl - 1) kvf_nl = kvf[k];
-        const double deepksat = kvf_nl * kv0 * wfh_exp(-fk * (dsoil < zexp ? dsoil : zexp));
-        const double deeptransfer = wfh_min(drainable / dt, deepksat);
-        leak = wfh_max(0.0, wfh_min(maxleak, deeptransfer));
-    }
-    const double recharge = (transfer - capflux - leak - ae_s - ses);
-    double aet = sevap + transp + aor + aol + 0.0;
-    aet = aet + ir;  // sbm.jl:130
+    const ColConfig cfg{d.dt, d.gash != 0, d.snow != 0, d.infred != 0, d.balance != 0, d.cfg_cum, d.cfg_thick};
+    ColFlux f;
+    sbm_column<N>(P, E, T, wd_land, r >= 0 ? r_h0 : 0.0, p, L, cfg, s, f);
 
     // ---- write back ----
-    d.canopy[i] = canopy;
+    if (d.balance) d.store_prev[i] = f.store_prev;
+    d.canopy[i] = s.canopy;
     if (d.snow) {
-        d.snow_s[i] = ss;
-        d.snow_w[i] = sw;
+        d.swe[i] = f.swe;
+        d.snow_s[i] = s.ss;
+        d.snow_w[i] = s.sw;
     }
-    d.tsoil[i] = tsoil;
+    d.tsoil[i] = s.tsoil;
 #pragma unroll
-    for (int k = 0; k < N; ++k) d.ustore[k * n + i] = us[k];
-    d.zi[i] = zi;
-    d.interception[i] = ir;
-    d.transfer[i] = transfer;
-    d.recharge[i] = recharge;
-    d.act_inf[i] = act_inf;
-    d.inf_excess[i] = inf_exc;
-    d.sat_excess[i] = sat_exc;
-    d.transpiration[i] = transp;
-    d.soil_evap[i] = sevap;
-    d.capflux[i] = capflux;
-    d.leakage[i] = leak;
-    d.aet[i] = aet;
-    d.net_runoff_river[i] = nrr;
-    d.runoff_land[i] = rl;
-    d.aeow_l[i] = aol;
+    for (int k = 0; k < N; ++k) d.ustore[k * n + i] = s.us[k];
+    d.zi[i] = f.zi;
+    d.interception[i] = f.interception;
+    d.transfer[i] = f.transfer;
+    d.recharge[i] = f.recharge;
+    d.act_inf[i] = f.act_inf;
+    d.inf_excess[i] = f.inf_exc;
+    d.sat_excess[i] = f.sat_exc;
+    d.transpiration[i] = f.transp;
+    d.soil_evap[i] = f.sevap;
+    d.capflux[i] = f.capflux;
+    d.leakage[i] = f.leak;
+    d.aet[i] = f.aet;
+    d.net_runoff_river[i] = f.nrr;
+    d.runoff_land[i] = f.rl;
+    d.aeow_l[i] = f.aol;
 #pragma unroll
-    for (int k = 0; k < N; ++k) o.us[k] = us[k];
-    o.zi = zi;
-    o.recharge = recharge;
-    o.sat_exc = sat_exc;
-    o.runoff_l = rl;
-    o.inf_exc = inf_exc;
-    o.aeow_l = aol;
-    o.nrr = nrr;
-    o.aet = aet;
-    o.leak = leak;
-    o.canopy = canopy;
-    o.snow_s = ss;
-    o.snow_w = sw;
-    o.store_prev = store_prev;
-    o.P = P;
+    for (int k = 0; k < N; ++k) o.us[k] = s.us[k];
+    o.zi = f.zi;
+    o.recharge = f.recharge;
+    o.sat_exc = f.sat_exc;
+    o.runoff_l = f.rl;
+    o.inf_exc = f.inf_exc;
+    o.aeow_l = f.aol;
+    o.nrr = f.nrr;
+    o.aet = f.aet;
+    o.leak = f.leak;
+    o.canopy = s.canopy;
+    o.snow_s = s.ss;
+    o.snow_w = s.sw;
+    o.store_prev = f.store_prev;
+    o.P = f.P;
 }
 
 template <int N>
@@ -4243,3 +4026,7 @@ extern "C" int32_t wfh_kernel_bytes(wfh_handle* h, int64_t* vertical_bytes, int6
     return WFH_OK;
 }
 
+// ============================================================================================
+// Parameter ensembles (member-innermost layout, staged launches replayed from a HIP graph)
+// ============================================================================================
+#include "wfh_ensemble.h"

@@ -228,3 +228,130 @@ class Model:
         self._check(self.lib.wfh_get_timing(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+
+def _bind_ensemble(L):
+    P = ctypes.c_void_p
+    i32, i64 = ctypes.c_int32, ctypes.c_int64
+    L.wfh_ens_create.argtypes = [ctypes.POINTER(_abi.WfhConfig), ctypes.POINTER(_abi.WfhStatic), i32, P, P, P,
+                                 ctypes.POINTER(P)]
+    L.wfh_ens_create.restype = i32
+    for name, args in (("wfh_ens_set_forcing", [P, P, P, P, i32]), ("wfh_ens_step", [P]),
+                       ("wfh_ens_get", [P, i32, i32, P, i64]), ("wfh_ens_balance", [P, P]),
+                       ("wfh_ens_set_schedule", [P, i32, i32, i32]), ("wfh_ens_set_autotune", [P, i32]),
+                       ("wfh_ens_get_schedule", [P, P, P, P, P, P]), ("wfh_ens_sync", [P]),
+                       ("wfh_ens_set_timing", [P, i32]), ("wfh_ens_get_timing", [P, P]),
+                       ("wfh_ens_step_bytes", [P, P])):
+        getattr(L, name).argtypes = args
+        getattr(L, name).restype = i32
+    L.wfh_ens_last_error.argtypes = [P]
+    L.wfh_ens_last_error.restype = ctypes.c_char_p
+    L.wfh_ens_destroy.argtypes = [P]
+    L.wfh_ens_destroy.restype = None
+
+
+class Ensemble:
+    """M members of one catchment whose kv_0, f and Manning n maps are scaled by a factor per
+    member, stepped together (member-innermost layout, graph-replayed stages). A member's
+    results are those of Model on the catchment with its scaled maps (member_catchment)."""
+
+    def __init__(self, catchment, kv0_factor, f_factor, n_factor, water_mass_balance=True, device=0, **cfg_kw):
+        self.lib = load_library()
+        if not hasattr(self.lib, "_ens_bound"):
+            _bind_ensemble(self.lib)
+            self.lib._ens_bound = True
+        self.catchment = catchment
+        self.factors = [np.ascontiguousarray(a, dtype=np.float64) for a in (kv0_factor, f_factor, n_factor)]
+        self.members = int(self.factors[0].size)
+        self.cfg = catchment.config(water_mass_balance=water_mass_balance, device=device, **cfg_kw)
+        self._static = catchment.static()
+        h = ctypes.c_void_p()
+        rc = self.lib.wfh_ens_create(ctypes.byref(self.cfg), ctypes.byref(self._static), self.members,
+                                     *(a.ctypes.data for a in self.factors), ctypes.byref(h))
+        if rc != 0:
+            raise WflowError(rc, self.lib.wfh_ens_last_error(None).decode())
+        self.h = h
+        self.n, self.nriv, self.nlayers = catchment.n, catchment.nriv, catchment.nlayers
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.wfh_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise WflowError(rc, self.lib.wfh_ens_last_error(self.h).decode())
+
+    def set_forcing(self, precipitation, potential_evaporation, temperature, memkind=None):
+        arrs = (precipitation, potential_evaporation, temperature)
+        if isinstance(precipitation, np.ndarray):
+            arrs = tuple(np.ascontiguousarray(a, dtype=np.float64) for a in arrs)
+            ptrs = [a.ctypes.data for a in arrs]
+            kind = _abi.WFH_MEM_HOST if memkind is None else memkind
+        else:
+            ptrs = [a.data_ptr() for a in arrs]
+            kind = _abi.WFH_MEM_DEVICE if memkind is None else memkind
+        self._check(self.lib.wfh_ens_set_forcing(self.h, *ptrs, kind))
+
+    def update_model(self):
+        self._check(self.lib.wfh_ens_step(self.h))
+
+    def run_timestep(self, precipitation, potential_evaporation, temperature):
+        self.set_forcing(precipitation, potential_evaporation, temperature)
+        self.update_model()
+
+    def sync(self):
+        self._check(self.lib.wfh_ens_sync(self.h))
+
+    def get(self, var, member):
+        cnt = _abi.var_count(var, self.n, self.nriv, self.nlayers)
+        out = np.empty(cnt, np.float64)
+        self._check(self.lib.wfh_ens_get(self.h, int(var), int(member), out.ctypes.data, cnt))
+        return out
+
+    def balance(self):
+        out = np.empty(self.members, np.float64)
+        self._check(self.lib.wfh_ens_balance(self.h, out.ctypes.data))
+        return out
+
+    def set_schedule(self, block, wide, strand_cells):
+        self._check(self.lib.wfh_ens_set_schedule(self.h, int(block), int(wide), int(strand_cells)))
+
+    def set_autotune(self, trials=1):
+        self._check(self.lib.wfh_ens_set_autotune(self.h, int(trials)))
+
+    def schedule(self):
+        v = [ctypes.c_int32() for _ in range(5)]
+        self._check(self.lib.wfh_ens_get_schedule(self.h, *(ctypes.addressof(x) for x in v)))
+        return dict(zip(("block", "wide", "strand_cells", "stages", "tuning"), (x.value for x in v)))
+
+    def set_timing(self, enable=True):
+        self._check(self.lib.wfh_ens_set_timing(self.h, int(enable)))
+
+    def timing(self):
+        a = ctypes.c_double()
+        self._check(self.lib.wfh_ens_get_timing(self.h, ctypes.byref(a)))
+        return a.value
+
+    def step_bytes(self):
+        a = ctypes.c_int64()
+        self._check(self.lib.wfh_ens_step_bytes(self.h, ctypes.addressof(a)))
+        return a.value
+
+
+def member_catchment(catchment, kv0_factor, f_factor, n_factor):
+    """The catchment of one ensemble member: kv_0, f and Manning n (land and river) scaled."""
+    import copy
+
+    c = copy.copy(catchment)
+    c.arrays = dict(catchment.arrays)
+    c.arrays["kv_0"] = catchment.arrays["kv_0"] * float(kv0_factor)
+    c.arrays["f"] = catchment.arrays["f"] * float(f_factor)
+    c.arrays["land_manning_n"] = catchment.arrays["land_manning_n"] * float(n_factor)
+    c.arrays["river_manning_n"] = catchment.arrays["river_manning_n"] * float(n_factor)
+    return c
