@@ -31,9 +31,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("grid", "batch"), default="grid",
+    ap.add_argument("--workload", choices=("grid", "batch", "ensemble"), default="grid",
                     help="grid: one nx^2 catchment at dx 250 m; batch: 64 catchments of log-uniform size "
-                         "(1e4 .. 2e6 cells, dx 1 km, river threshold 100 cells) in one model")
+                         "(1e4 .. 2e6 cells, dx 1 km, river threshold 100 cells) in one model; ensemble: "
+                         "--ens-members parameter members of one --ens-nx^2 catchment (kv_0, f, n lognormal "
+                         "sigma 0.3), member-innermost, autotuned schedule")
+    ap.add_argument("--ens-members", type=int, default=256)
+    ap.add_argument("--ens-nx", type=int, default=2048)
+    ap.add_argument("--ens-tune-trials", type=int, default=1)
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--nx", type=int, default=4096)
     ap.add_argument("--frames", type=int, default=24, help="distinct forcing frames kept in HBM")
@@ -141,6 +146,74 @@ def cpu_baseline(catchment, frames_host, args):
     }
 
 
+def ensemble_run(args, t0):
+    """Parameter ensemble of one catchment (wfh_ens_*): every member's update_model! in one graph
+    replay per step; the first steps of the warm-up run the autotuner's candidate schedules."""
+    import numpy as np
+    import torch
+
+    from wflowhip import Ensemble, _abi, synth
+
+    c = synth.make_catchment(args.ens_nx, cell=250.0, dt=3600.0, river_threshold=2000, seed=4)
+    M = args.ens_members
+    rng = np.random.default_rng(44)
+    fa, fb, fc = np.exp(0.3 * rng.standard_normal((3, M)))
+    workload = (f"{M}-member parameter ensemble of one {args.ens_nx}x{args.ens_nx} catchment (kv_0, f, Manning n "
+                "scaled per member, lognormal sigma 0.3), dx 250 m, hourly, river threshold 2000 cells, sbm + "
+                "kinematic wave (land 1, river 4, subsurface 1 sub-steps), member-innermost, staged launches "
+                "replayed from a HIP graph, autotuned schedule")
+    print(f"# {workload}: n={c.n} nriv={c.nriv} longest path={c.max_depth} ({time.time() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
+    ens = Ensemble(c, fa, fb, fc, water_mass_balance=True)
+    print(f"# ensemble created ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    from wflowhip import Model
+    base = Model(c)  # node order only
+    node_t = torch.as_tensor(base.node_order(), device="cuda")
+    base.close()
+    elev = torch.as_tensor(c.arrays["elevation"], device="cuda")[node_t].contiguous()
+    nframes = max(1, min(args.frames, args.steps + args.warmup))
+    frames = [synth.forcing_torch(c, k, elev) for k in range(nframes)]
+    torch.cuda.synchronize()
+    ens.set_autotune(args.ens_tune_trials)
+    k = 0
+    while ens.schedule()["tuning"] or k < args.warmup:
+        ens.set_forcing(*frames[k % nframes], memkind=_abi.WFH_MEM_DEVICE_NODE_ORDER)
+        ens.update_model()
+        k += 1
+    ens.sync()
+    sched = ens.schedule()
+    print(f"# {k} warm-up steps (autotuner): schedule {sched} ({time.time() - t0:.1f}s)", file=sys.stderr,
+          flush=True)
+    ens.set_timing(True)
+    t1 = time.perf_counter()
+    for j in range(args.steps):
+        ens.set_forcing(*frames[(k + j) % nframes], memkind=_abi.WFH_MEM_DEVICE_NODE_ORDER)
+        ens.update_model()
+    ens.sync()
+    elapsed = time.perf_counter() - t1
+    step_ms = ens.timing()
+    nbytes = ens.step_bytes()
+    bal = float(ens.balance().max())
+    print(f"# {args.steps} steps in {elapsed:.3f}s ({step_ms:.2f} ms per graph replay); max |land balance "
+          f"error| {bal:.2e}", file=sys.stderr, flush=True)
+    achieved = nbytes / (step_ms * 1e-3) / 1e9
+    result = {
+        "metric": "member-cell-timesteps/s FP64 SBM+kinwave (batched ensemble GPU step)",
+        "value": M * c.n * args.steps / elapsed,
+        "unit": "cell-timesteps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": k, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (fractal DEM, seeded parameters and member factors, GPU-generated forcing)",
+        "config": {"workload": workload, "members": M, "cells": c.n, "river_cells": c.nriv,
+                   "longest_flow_path": c.max_depth, "schedule": sched, "step_bytes": nbytes,
+                   "max_abs_land_balance_error": bal},
+        "roofline": {"bound": "hbm", "kernel": "k_ens_stage (all stages of a step)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+    }
+    ens.close()
+    return result, c, frames, node_t
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -154,6 +227,19 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     t0 = time.time()
+    if args.workload == "ensemble":
+        result, c, frames, node_t = ensemble_run(args, t0)
+        if not args.no_cpu_baseline:
+            node = node_t.cpu().numpy()
+            inv = np.empty_like(node)
+            inv[node] = np.arange(node.size)
+            frames_host = [tuple(f.cpu().numpy()[inv] for f in frames[k]) for k in range(args.cpu_sample_steps)]
+            result["cpu_baseline"] = cpu_baseline(c, frames_host, args)
+            result["cpu_baseline"]["sample"] += " (member 0, the unperturbed catchment; a member is one model)"
+        else:
+            result["cpu_baseline"] = None
+        print(json.dumps(result), flush=True)
+        return
     if args.workload == "grid":
         c = synth.make_catchment(args.nx, cell=250.0, dt=3600.0, river_threshold=2000, seed=1)
         workload = (f"{args.nx}x{args.nx} cells, dx 250 m, hourly, river threshold 2000 cells, "

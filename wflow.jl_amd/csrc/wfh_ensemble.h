@@ -22,6 +22,10 @@ namespace {
 
 constexpr int ENS_MAX_SR = 8;       // river sub-steps per step
 constexpr int ENS_MAX_MEMBERS = 4096;
+constexpr int ENS_PRE = 10;          // fluxes of the vertical update that the routing part reads
+// What a stage launch does for each of its cells: everything, the vertical column only (into
+// the pre-buffer), or the routing only (the column taken from the pre-buffer).
+enum { ENS_FUSED = 0, ENS_COLUMN = 1, ENS_ROUTE = 2 };
 
 struct EnsDev {
     int n, nriv, M, G;  // cells, river cells, members, member groups of 64
@@ -51,15 +55,22 @@ struct EnsDev {
     double* rq_sub;                // [S_r - 1][nriv * M]: discharge of the earlier river sub-steps
     unsigned long long* bal_max;   // [M]: largest |land balance error| of the step (bits of a double)
     const int *strand_ptr, *strand_cell;
+    // the vertical update of chain-stage cells, done beforehand in a parallel pass: node -> slot
+    // (or -1), and the fluxes the routing part reads, [ENS_PRE][npre * M]
+    const int* pre_slot;
+    double* pre;
+    size_t pre_stride;
 };
 static_assert(sizeof(EnsDev) + 64 <= 4096, "EnsDev too large for a kernel argument");
 
 __device__ __forceinline__ double ens_ld(const Prm& a, int v) { return a.p[v & a.mask]; }
 
-// The complete update of cell v (node order) for member m: the same operations, in the same
-// order, as k_vertical followed by k_route's families for one cell (one subsurface and one
-// overland sub-step, S_r river sub-steps). Returns |land balance error| (0 without the balance).
-template <int N>
+// The update of cell v (node order) for member m: the same operations, in the same order, as
+// k_vertical followed by k_route's families for one cell (one subsurface and one overland
+// sub-step, S_r river sub-steps). MODE ENS_COLUMN stops after the vertical column (states
+// written, the routing inputs into the pre-buffer), ENS_ROUTE starts from them. Returns |land
+// balance error| (0 without the balance or in ENS_COLUMN).
+template <int N, int MODE>
 __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const int m) {
     const size_t M = (size_t)e.M;
     const size_t i = (size_t)v * M + m;
@@ -67,50 +78,77 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
     const double dt = e.dt;
     const double fa = e.fa[m], fb = e.fb[m];
     // ---- loads: member states (vector), shared parameters (wave-uniform) ----
-    const double P = e.P[v], E = e.E[v], T = e.T[v];
+    const double P = e.P[v];
     ColState<N> s;
     s.canopy = e.canopy[i];
     s.ss = e.snow_s[i];
     s.sw = e.snow_w[i];
-    s.tsws = e.tsws[i];
-    s.swd = e.swd[i];
-    s.tsoil = e.tsoil[i];
 #pragma unroll
     for (int k = 0; k < N; ++k) s.us[k] = e.ustore[k * nM + i];
-    const double sq0 = e.ssf_q[i], oq0 = e.ol_q[i], oh0 = e.ol_h[i];
-    const double qmax = e.qmax[i], alpha = e.alpha[i];
+    const double oh0 = e.ol_h[i];
     const int r = e.river_of[v];
     const bool riv = r >= 0;
     const size_t ri = (size_t)(riv ? r : 0) * M + m;
-    const double rq0 = e.r_q[ri], rh0 = e.r_h[ri], r_store_prev = e.r_storage[ri], ralpha = e.r_alpha[ri];
+    const double rh0 = e.r_h[ri];
     ColParams p;
-    p.gap = ens_ld(e.gap, v), p.smax = ens_ld(e.smax, v), p.kc = ens_ld(e.kc, v), p.er = ens_ld(e.er, v);
-    p.tti = ens_ld(e.tti, v), p.tt = ens_ld(e.tt, v), p.ttm = ens_ld(e.ttm, v), p.cfmax = ens_ld(e.cfmax, v);
-    p.whc = ens_ld(e.whc, v), p.wsoil = ens_ld(e.wsoil, v), p.cfsoil = ens_ld(e.cfsoil, v);
-    p.rf = ens_ld(e.rf, v), p.wf = ens_ld(e.wf, v), p.sfrac = ens_ld(e.sfrac, v);
     p.ts = ens_ld(e.ts, v), p.tr = ens_ld(e.tr, v), p.tfc = ens_ld(e.tfc, v), p.dsoil = ens_ld(e.d, v);
     p.kv0 = ens_ld(e.kv0, v) * fa, p.fk = ens_ld(e.f, v) * fb, p.zexp = INFINITY;
-    p.compf = ens_ld(e.compf, v), p.icap_c = ens_ld(e.icap_c, v);
-    p.rd = ens_ld(e.rd, v), p.h3h = ens_ld(e.h3h, v), p.h3l = ens_ld(e.h3l, v);
-    p.h1 = ens_ld(e.h1, v), p.h2 = ens_ld(e.h2, v), p.h4 = ens_ld(e.h4, v), p.ah1 = ens_ld(e.ah1, v);
-    p.hb = ens_ld(e.hb, v), p.rdp = ens_ld(e.rdp, v), p.caph = ens_ld(e.caph, v), p.capn = ens_ld(e.capn, v);
-    p.maxleak = ens_ld(e.maxleak, v);
-    ColLayers<N> L;
+    ColFlux cf;
+    if (MODE != ENS_ROUTE) {
+        const double E = e.E[v], T = e.T[v];
+        s.tsws = e.tsws[i];
+        s.swd = e.swd[i];
+        s.tsoil = e.tsoil[i];
+        p.gap = ens_ld(e.gap, v), p.smax = ens_ld(e.smax, v), p.kc = ens_ld(e.kc, v), p.er = ens_ld(e.er, v);
+        p.tti = ens_ld(e.tti, v), p.tt = ens_ld(e.tt, v), p.ttm = ens_ld(e.ttm, v), p.cfmax = ens_ld(e.cfmax, v);
+        p.whc = ens_ld(e.whc, v), p.wsoil = ens_ld(e.wsoil, v), p.cfsoil = ens_ld(e.cfsoil, v);
+        p.rf = ens_ld(e.rf, v), p.wf = ens_ld(e.wf, v), p.sfrac = ens_ld(e.sfrac, v);
+        p.compf = ens_ld(e.compf, v), p.icap_c = ens_ld(e.icap_c, v);
+        p.rd = ens_ld(e.rd, v), p.h3h = ens_ld(e.h3h, v), p.h3l = ens_ld(e.h3l, v);
+        p.h1 = ens_ld(e.h1, v), p.h2 = ens_ld(e.h2, v), p.h4 = ens_ld(e.h4, v), p.ah1 = ens_ld(e.ah1, v);
+        p.hb = ens_ld(e.hb, v), p.rdp = ens_ld(e.rdp, v), p.caph = ens_ld(e.caph, v), p.capn = ens_ld(e.capn, v);
+        p.maxleak = ens_ld(e.maxleak, v);
+        ColLayers<N> L;
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-        L.kvf[k] = ld(e.kvfrac, k, v);
-        L.cbc[k] = ld(e.c, k, v);
-        L.rfr[k] = ld(e.rootfrac, k, v);
+        for (int k = 0; k < N; ++k) {
+            L.kvf[k] = ld(e.kvfrac, k, v);
+            L.cbc[k] = ld(e.c, k, v);
+            L.rfr[k] = ld(e.rootfrac, k, v);
+        }
+        p.icap_s = p.kv0 * L.kvf[0];  // infiltcap_soil = kv_0 kvfrac[1] (soil.jl)
+        // ---- vertical column ----
+        const ColConfig cfg{dt, e.gash != 0, e.snow != 0, e.infred != 0, e.balance != 0, e.cfg_cum, e.cfg_thick};
+        sbm_column<N>(P, E, T, oh0, riv ? rh0 : 0.0, p, L, cfg, s, cf);
     }
-    p.icap_s = p.kv0 * L.kvf[0];  // infiltcap_soil = kv_0 kvfrac[1] (soil.jl)
+    if (MODE == ENS_COLUMN) {
+        e.canopy[i] = s.canopy;
+        if (e.snow) {
+            e.snow_s[i] = s.ss;
+            e.snow_w[i] = s.sw;
+        }
+        e.tsoil[i] = s.tsoil;
+#pragma unroll
+        for (int k = 0; k < N; ++k) e.ustore[k * nM + i] = s.us[k];
+        double* q = e.pre + (size_t)e.pre_slot[v] * M + m;
+        const size_t ps = e.pre_stride;
+        q[0] = cf.zi, q[ps] = cf.recharge, q[2 * ps] = cf.sat_exc, q[3 * ps] = cf.rl, q[4 * ps] = cf.inf_exc;
+        q[5 * ps] = cf.aol, q[6 * ps] = cf.nrr, q[7 * ps] = cf.aet, q[8 * ps] = cf.leak, q[9 * ps] = cf.store_prev;
+        return 0.0;
+    }
+    if (MODE == ENS_ROUTE) {
+        const double* q = e.pre + (size_t)e.pre_slot[v] * M + m;
+        const size_t ps = e.pre_stride;
+        cf.zi = q[0], cf.recharge = q[ps], cf.sat_exc = q[2 * ps], cf.rl = q[3 * ps], cf.inf_exc = q[4 * ps];
+        cf.aol = q[5 * ps], cf.nrr = q[6 * ps], cf.aet = q[7 * ps], cf.leak = q[8 * ps], cf.store_prev = q[9 * ps];
+        cf.P = P;
+    }
+    // routing inputs, loaded after the column (fewer registers live across it)
+    const double sq0 = e.ssf_q[i], oq0 = e.ol_q[i];
+    const double qmax = e.qmax[i], alpha = e.alpha[i];
+    const double rq0 = e.r_q[ri], r_store_prev = e.r_storage[ri], ralpha = e.r_alpha[ri];
     const double area = ens_ld(e.area, v), flen = ens_ld(e.fl, v), fw = ens_ld(e.fw, v), slope = ens_ld(e.slope, v);
     const double sy = ens_ld(e.sy, v), sfw = ens_ld(e.sfw, v);
     const double kh0 = ens_ld(e.khfrac, v) * p.kv0;
-
-    // ---- vertical column ----
-    const ColConfig cfg{dt, e.gash != 0, e.snow != 0, e.infred != 0, e.balance != 0, e.cfg_cum, e.cfg_thick};
-    ColFlux cf;
-    sbm_column<N>(P, E, T, oh0, riv ? rh0 : 0.0, p, L, cfg, s, cf);
 
     // ---- lateral subsurface flow (one sub-step of length dt) ----
     const double ts = p.ts, tr = p.tr, dsoil = p.dsoil;
@@ -175,12 +213,14 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
     for (int k = 0; k < N; ++k) e.ustore[k * nM + i] = col.ustore[k];
     e.swd[i] = swd;
     e.tsws[i] = swd + usd;
-    e.canopy[i] = s.canopy;
-    if (e.snow) {
-        e.snow_s[i] = s.ss;
-        e.snow_w[i] = s.sw;
+    if (MODE == ENS_FUSED) {
+        e.canopy[i] = s.canopy;
+        if (e.snow) {
+            e.snow_s[i] = s.ss;
+            e.snow_w[i] = s.sw;
+        }
+        e.tsoil[i] = s.tsoil;
     }
-    e.tsoil[i] = s.tsoil;
     double bal = 0.0;
     if (e.balance) {  // land balance  mass_balance.jl:215-253
         const double t_in = s_qin_avg / area + (cf.P + 0.0) + 0.0 + 0.0 / area;
@@ -253,8 +293,8 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
     return bal;
 }
 
-template <int N>
-__global__ void __launch_bounds__(256) k_ens_stage(EnsDev e, int s0, int ns) {
+template <int N, int MODE>
+__global__ void __launch_bounds__(256, 2) k_ens_stage(EnsDev e, int s0, int ns) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     if (w >= ns * e.G) return;
@@ -265,7 +305,7 @@ __global__ void __launch_bounds__(256) k_ens_stage(EnsDev e, int s0, int ns) {
     double worst = 0.0;
     for (int c = c0; c < c1; ++c) {
         const int v = __builtin_amdgcn_readfirstlane(e.strand_cell[c]);
-        worst = wfh_max(worst, ens_cell<N>(e, v, m));
+        worst = wfh_max(worst, ens_cell<N, MODE>(e, v, m));
     }
     if (e.balance && worst > 0.0) atomicMax(&e.bal_max[m], (unsigned long long)wfh_bits(worst));
 }
@@ -341,8 +381,13 @@ struct EnsSchedule {
     int strand_cells = 4;  // cells per strand in those stages
     std::vector<int> strand_ptr, strand_cell;  // host copy
     std::vector<std::pair<int, int>> stages;   // (first strand, strand count)
+    std::vector<int> modes;                    // ENS_FUSED / ENS_COLUMN / ENS_ROUTE per stage
+    std::vector<int> pre_slot;                 // node -> slot of a chain-stage cell, or -1
+    int npre = 0;
+    bool built = false;
     int* d_ptr = nullptr;
     int* d_cell = nullptr;
+    int* d_pre_slot = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     double ms = -1.0;  // measured step time (autotuner)
@@ -359,6 +404,8 @@ struct wfh_ensemble {
     std::vector<int> down;                    // node -> downstream node or -1
     int max_level = 0;
     std::vector<EnsSchedule> cand;            // schedules (the autotuner's candidates)
+    double* pre = nullptr;                    // pre-buffer shared by the schedules, [ENS_PRE][pre_cap * M]
+    int pre_cap = 0;
     int cur = 0;                              // schedule in use
     bool tuning = false;
     int tune_next = 0, tune_trials = 1, tune_done = 0;
@@ -392,14 +439,19 @@ T* ealloc(wfh_ensemble* x, size_t count) {
     return (T*)p;
 }
 
-// The stages of one schedule (see the header comment of this file).
+// The stages of one schedule (see the header comment of this file). The vertical column of the
+// cells of chain stages runs first, in a parallel stage of its own (ENS_COLUMN), so that the
+// chains carry the routing only.
 void ens_build_schedule(const wfh_ensemble* x, EnsSchedule& s) {
+    if (s.built) return;
+    s.built = true;
     const int n = x->e.n;
     std::vector<std::vector<int>> bylevel(x->max_level + 1);
     for (int v = 0; v < n; ++v) bylevel[x->level[v]].push_back(v);
     s.strand_ptr.assign(1, 0);
     s.strand_cell.clear();
     s.stages.clear();
+    s.modes.clear();
     std::vector<int> stage_of(n, -1);
     int L = 0;
     const int K = std::max(1, s.strand_cells);
@@ -416,6 +468,7 @@ void ens_build_schedule(const wfh_ensemble* x, EnsSchedule& s) {
                 s.strand_ptr.push_back((int)s.strand_cell.size());
             }
             ++L;
+            s.modes.push_back(ENS_FUSED);
         } else {
             // narrow levels L .. L2-1 while every cell has at most one upstream cell in the stage
             int L2 = L;
@@ -443,8 +496,30 @@ void ens_build_schedule(const wfh_ensemble* x, EnsSchedule& s) {
                     s.strand_ptr.push_back((int)s.strand_cell.size());
                 }
             L = L2;
+            s.modes.push_back(ENS_ROUTE);
         }
         s.stages.push_back({first, (int)s.strand_ptr.size() - 1 - first});
+    }
+    // the column pass over the chain stages' cells, first
+    s.pre_slot.assign(n, -1);
+    std::vector<int> pre;
+    for (size_t k = 0; k < s.stages.size(); ++k) {
+        if (s.modes[k] != ENS_ROUTE) continue;
+        for (int t = s.stages[k].first; t < s.stages[k].first + s.stages[k].second; ++t)
+            for (int c = s.strand_ptr[t]; c < s.strand_ptr[t + 1]; ++c) {
+                s.pre_slot[s.strand_cell[c]] = (int)pre.size();
+                pre.push_back(s.strand_cell[c]);
+            }
+    }
+    s.npre = (int)pre.size();
+    if (s.npre > 0) {
+        const int first = (int)s.strand_ptr.size() - 1;
+        for (size_t a = 0; a < pre.size(); a += K) {
+            for (size_t b = a; b < std::min(pre.size(), a + (size_t)K); ++b) s.strand_cell.push_back(pre[b]);
+            s.strand_ptr.push_back((int)s.strand_cell.size());
+        }
+        s.stages.insert(s.stages.begin(), {first, (int)s.strand_ptr.size() - 1 - first});
+        s.modes.insert(s.modes.begin(), ENS_COLUMN);
     }
 }
 
@@ -453,21 +528,33 @@ void ens_launch_stages(wfh_ensemble* x, const EnsSchedule& s, hipStream_t st) {
     EnsDev e = x->e;
     e.strand_ptr = s.d_ptr;
     e.strand_cell = s.d_cell;
+    e.pre_slot = s.d_pre_slot;
+    e.pre = x->pre;
+    e.pre_stride = (size_t)x->pre_cap * x->M;
     if (e.balance) hipLaunchKernelGGL(k_ens_zero_u64, dim3((x->M + 255) / 256), dim3(256), 0, st, e.bal_max, x->M);
     const int wpb = s.block / WAVE;
-    for (const auto& sg : s.stages) {
+    for (size_t k = 0; k < s.stages.size(); ++k) {
+        const auto& sg = s.stages[k];
         const int waves = sg.second * e.G;
-        hipLaunchKernelGGL(k_ens_stage<N>, dim3((waves + wpb - 1) / wpb), dim3(s.block), 0, st, e, sg.first,
-                           sg.second);
+        const dim3 grid((waves + wpb - 1) / wpb), block(s.block);
+        if (s.modes[k] == ENS_COLUMN)
+            hipLaunchKernelGGL((k_ens_stage<N, ENS_COLUMN>), grid, block, 0, st, e, sg.first, sg.second);
+        else if (s.modes[k] == ENS_ROUTE)
+            hipLaunchKernelGGL((k_ens_stage<N, ENS_ROUTE>), grid, block, 0, st, e, sg.first, sg.second);
+        else
+            hipLaunchKernelGGL((k_ens_stage<N, ENS_FUSED>), grid, block, 0, st, e, sg.first, sg.second);
     }
 }
 
 int32_t ens_prepare(wfh_ensemble* x, EnsSchedule& s) {
     if (s.d_ptr) return WFH_OK;
     ens_build_schedule(x, s);
+    if (s.npre > x->pre_cap) return efail(x, WFH_ERR_STATE, "internal error: pre-buffer smaller than a schedule");
     s.d_ptr = ealloc<int>(x, s.strand_ptr.size());
     s.d_cell = ealloc<int>(x, s.strand_cell.size());
-    if (!s.d_ptr || !s.d_cell) return efail(x, WFH_ERR_HIP, "hipMalloc schedule");
+    s.d_pre_slot = ealloc<int>(x, s.pre_slot.size());
+    if (!s.d_ptr || !s.d_cell || !s.d_pre_slot) return efail(x, WFH_ERR_HIP, "hipMalloc schedule");
+    EHIP(x, hipMemcpy(s.d_pre_slot, s.pre_slot.data(), s.pre_slot.size() * sizeof(int), hipMemcpyHostToDevice));
     EHIP(x, hipMemcpy(s.d_ptr, s.strand_ptr.data(), s.strand_ptr.size() * sizeof(int), hipMemcpyHostToDevice));
     EHIP(x, hipMemcpy(s.d_cell, s.strand_cell.data(), s.strand_cell.size() * sizeof(int), hipMemcpyHostToDevice));
     // the step's launches, recorded once into a graph
@@ -681,6 +768,12 @@ extern "C" int32_t wfh_ens_create(const wfh_config* cfg, const wfh_static* st, i
                 x->cand.push_back(c);
             }
     x->cur = 0;
+    for (EnsSchedule& c : x->cand) {
+        ens_build_schedule(x, c);
+        x->pre_cap = std::max(x->pre_cap, c.npre);
+    }
+    if (x->pre_cap > 0 && !(x->pre = ealloc<double>(x, (size_t)ENS_PRE * x->pre_cap * M)))
+        return bail(WFH_ERR_HIP, "ensemble: hipMalloc pre-buffer");
     rc = ens_prepare(x, x->cand[0]);
     if (rc) return bail(rc, x->err);
     *out = x;
@@ -783,6 +876,10 @@ extern "C" int32_t wfh_ens_set_schedule(wfh_ensemble* x, int32_t block, int32_t 
     c.block = block;
     c.wide = wide;
     c.strand_cells = strand_cells;
+    ens_build_schedule(x, c);
+    if (c.npre > x->pre_cap)
+        return efail(x, WFH_ERR_ARG, "this schedule has more chain-stage cells than the pre-buffer holds; use a "
+                                     "larger `wide` threshold");
     x->cand.push_back(c);
     x->cur = (int)x->cand.size() - 1;
     x->tuning = false;
