@@ -39,6 +39,15 @@ def parse():
     ap.add_argument("--ens-members", type=int, default=256)
     ap.add_argument("--ens-nx", type=int, default=2048)
     ap.add_argument("--ens-tune-trials", type=int, default=1)
+    ap.add_argument("--learned", action="store_true",
+                    help="ensemble workload: also time the learned Newton warm start (trained first)")
+    ap.add_argument("--learn-harvest-steps", type=int, default=6)
+    ap.add_argument("--learn-epochs", type=int, default=4)
+    ap.add_argument("--learn-gate", type=float, default=1e-12,
+                    help="a wave consults the network where some member's previous discharge is below this "
+                         "(inf: every solve)")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="grid workload: skip the batched (ensemble) and batched+learned lines attached to it")
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--nx", type=int, default=4096)
     ap.add_argument("--frames", type=int, default=24, help="distinct forcing frames kept in HBM")
@@ -146,14 +155,51 @@ def cpu_baseline(catchment, frames_host, args):
     }
 
 
-def ensemble_run(args, t0):
+def _ens_steps(ens, frames, k0, steps, memkind):
+    for j in range(steps):
+        ens.set_forcing(*frames[(k0 + j) % len(frames)], memkind=memkind)
+        ens.update_model()
+    ens.sync()
+
+
+def train_warm_start(args, log):
+    """The learned Newton start: rows harvested from plain solves of another catchment's ensemble
+    (a 1024^2 catchment, seed 41, 64 members, hourly), then trained on the GPU."""
+    import numpy as np
+
+    from wflowhip import learned as LW, synth
+
+    tc = synth.make_catchment(1024, cell=250.0, dt=3600.0, river_threshold=2000, seed=41)
+    f = np.exp(0.3 * np.random.default_rng(41).standard_normal((3, 64)))
+    t0 = time.time()
+    rows = LW.harvest(tc, f, args.learn_harvest_steps, lambda k: synth.forcing_numpy(tc, 48 + k), every=16,
+                      cap=4_000_000)
+    log(f"# harvested {rows.shape[0]} rows in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    net, losses = LW.fit(rows, epochs=args.learn_epochs)
+    log(f"# trained {args.learn_epochs} epochs in {time.time() - t0:.1f}s, mse {losses[0]:.3f} -> {losses[-1]:.3f}")
+    return net, {"harvested_rows": int(rows.shape[0]), "epochs": args.learn_epochs,
+                 "mse_first_last": [losses[0], losses[-1]],
+                 "source": "plain solves of a 64-member ensemble of another 1024^2 catchment (seed 41), "
+                           f"{args.learn_harvest_steps} hourly steps, every 16th solve"}
+
+
+def ensemble_run(args, t0, learned=False):
     """Parameter ensemble of one catchment (wfh_ens_*): every member's update_model! in one graph
-    replay per step; the first steps of the warm-up run the autotuner's candidate schedules."""
+    replay per step; the first steps of the warm-up run the autotuner's candidate schedules. With
+    `learned`, the same ensemble then runs as many steps with the learned warm start."""
     import numpy as np
     import torch
 
     from wflowhip import Ensemble, _abi, synth
+    from wflowhip import learned as LW
 
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    net, train_info = (None, None)
+    if learned:
+        net, train_info = train_warm_start(args, log)
     c = synth.make_catchment(args.ens_nx, cell=250.0, dt=3600.0, river_threshold=2000, seed=4)
     M = args.ens_members
     rng = np.random.default_rng(44)
@@ -162,10 +208,9 @@ def ensemble_run(args, t0):
                 "scaled per member, lognormal sigma 0.3), dx 250 m, hourly, river threshold 2000 cells, sbm + "
                 "kinematic wave (land 1, river 4, subsurface 1 sub-steps), member-innermost, staged launches "
                 "replayed from a HIP graph, autotuned schedule")
-    print(f"# {workload}: n={c.n} nriv={c.nriv} longest path={c.max_depth} ({time.time() - t0:.1f}s)",
-          file=sys.stderr, flush=True)
+    log(f"# {workload}: n={c.n} nriv={c.nriv} longest path={c.max_depth} ({time.time() - t0:.1f}s)")
     ens = Ensemble(c, fa, fb, fc, water_mass_balance=True)
-    print(f"# ensemble created ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    log(f"# ensemble created ({time.time() - t0:.1f}s)")
     from wflowhip import Model
     base = Model(c)  # node order only
     node_t = torch.as_tensor(base.node_order(), device="cuda")
@@ -174,28 +219,39 @@ def ensemble_run(args, t0):
     nframes = max(1, min(args.frames, args.steps + args.warmup))
     frames = [synth.forcing_torch(c, k, elev) for k in range(nframes)]
     torch.cuda.synchronize()
+    NODE = _abi.WFH_MEM_DEVICE_NODE_ORDER
     ens.set_autotune(args.ens_tune_trials)
     k = 0
     while ens.schedule()["tuning"] or k < args.warmup:
-        ens.set_forcing(*frames[k % nframes], memkind=_abi.WFH_MEM_DEVICE_NODE_ORDER)
-        ens.update_model()
+        _ens_steps(ens, frames, k, 1, NODE)
         k += 1
-    ens.sync()
     sched = ens.schedule()
-    print(f"# {k} warm-up steps (autotuner): schedule {sched} ({time.time() - t0:.1f}s)", file=sys.stderr,
-          flush=True)
-    ens.set_timing(True)
-    t1 = time.perf_counter()
-    for j in range(args.steps):
-        ens.set_forcing(*frames[(k + j) % nframes], memkind=_abi.WFH_MEM_DEVICE_NODE_ORDER)
-        ens.update_model()
-    ens.sync()
-    elapsed = time.perf_counter() - t1
-    step_ms = ens.timing()
+    log(f"# {k} warm-up steps (autotuner): schedule {sched} ({time.time() - t0:.1f}s)")
+
+    def timed(label):
+        nonlocal k
+        _ens_steps(ens, frames, k, 1, NODE)  # records the graph of this variant
+        k += 1
+        ens.set_timing(True)
+        t1 = time.perf_counter()
+        _ens_steps(ens, frames, k, args.steps, NODE)
+        elapsed = time.perf_counter() - t1
+        k += args.steps
+        step_ms = ens.timing()
+        ens.set_timing(False)
+        bal = float(ens.balance().max())
+        log(f"# {label}: {args.steps} steps in {elapsed:.3f}s ({step_ms:.2f} ms per graph replay); max |land "
+            f"balance error| {bal:.2e}")
+        # Newton-update histogram over two further (untimed) steps
+        ens.kw_histogram(True)
+        _ens_steps(ens, frames, k, 2, NODE)
+        k += 2
+        hist, used = ens.kw_histogram(False)
+        upd = [LW.mean_updates(hist[0]), LW.mean_updates(hist[1])]
+        return elapsed, step_ms, bal, hist, used, upd
+
+    elapsed, step_ms, bal, hist, used, upd = timed("plain")
     nbytes = ens.step_bytes()
-    bal = float(ens.balance().max())
-    print(f"# {args.steps} steps in {elapsed:.3f}s ({step_ms:.2f} ms per graph replay); max |land balance "
-          f"error| {bal:.2e}", file=sys.stderr, flush=True)
     achieved = nbytes / (step_ms * 1e-3) / 1e9
     result = {
         "metric": "member-cell-timesteps/s FP64 SBM+kinwave (batched ensemble GPU step)",
@@ -206,11 +262,62 @@ def ensemble_run(args, t0):
         "data": "synthetic (fractal DEM, seeded parameters and member factors, GPU-generated forcing)",
         "config": {"workload": workload, "members": M, "cells": c.n, "river_cells": c.nriv,
                    "longest_flow_path": c.max_depth, "schedule": sched, "step_bytes": nbytes,
-                   "max_abs_land_balance_error": bal},
+                   "max_abs_land_balance_error": bal,
+                   "newton_updates_per_solve": {"overland": upd[0], "river": upd[1]},
+                   "newton_update_histogram": {"overland": hist[0].tolist(), "river": hist[1].tolist()}},
         "roofline": {"bound": "hbm", "kernel": "k_ens_stage (all stages of a step)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
     }
     ens.close()
+    if learned:
+        # plain and learned side by side over the same model hours: two ensembles of half the
+        # members each (both fit in HBM together), stepped alternately on the same forcing
+        Mh = M // 2
+        pair = [Ensemble(c, fa[:Mh], fb[:Mh], fc[:Mh], water_mass_balance=True) for _ in range(2)]
+        pair[1].set_learned(net, args.learn_gate)
+        for e_ in pair:
+            e_.set_schedule(sched["block"], sched["wide"], sched["strand_cells"])
+        for j in range(args.warmup):
+            for e_ in pair:
+                _ens_steps(e_, frames, j, 1, NODE)
+        ms = [0.0, 0.0]
+        for e_ in pair:
+            e_.set_timing(True)
+        for j in range(args.warmup, args.warmup + args.steps):
+            for i_, e_ in enumerate(pair):
+                _ens_steps(e_, frames, j, 1, NODE)
+        ms = [e_.timing() for e_ in pair]
+        hists = []
+        for e_ in pair:
+            e_.set_timing(False)
+            e_.kw_histogram(True)
+        for j in range(args.warmup + args.steps, args.warmup + args.steps + 2):
+            for e_ in pair:
+                _ens_steps(e_, frames, j, 1, NODE)
+        hists = [e_.kw_histogram(False) for e_ in pair]
+        bal_l = float(pair[1].balance().max())
+        log(f"# side by side, {Mh} members each, same hours: plain {ms[0]:.2f} ms, learned {ms[1]:.2f} ms per step")
+        upd = [[LW.mean_updates(h[0][f_]) for f_ in (0, 1)] for h in hists]
+        result["learned"] = {
+            "members": Mh, "ms_per_step_plain": ms[0], "ms_per_step_learned": ms[1],
+            "value": Mh * c.n / (ms[1] * 1e-3), "value_plain": Mh * c.n / (ms[0] * 1e-3),
+            "ratio_vs_plain_ensemble": ms[0] / ms[1],
+            "max_abs_land_balance_error": bal_l,
+            "newton_updates_per_solve": {"plain": {"overland": upd[0][0], "river": upd[0][1]},
+                                         "learned": {"overland": upd[1][0], "river": upd[1][1]}},
+            "newton_update_histogram": {"plain": {"overland": hists[0][0][0].tolist(),
+                                                  "river": hists[0][0][1].tolist()},
+                                        "learned": {"overland": hists[1][0][0].tolist(),
+                                                    "river": hists[1][0][1].tolist()}},
+            "learned_starts_taken": {"overland": int(hists[1][1][0]), "river": int(hists[1][1][1]),
+                                     "solves": [int(hists[1][0][0].sum()), int(hists[1][0][1].sum())]},
+            "training": train_info, "gate_q_prev": args.learn_gate,
+            "note": "two ensembles of half the members stepped alternately over the same model hours "
+                    "(graph-replay time per step from HIP events)",
+        }
+        for e_ in pair:
+            e_.close()
+        net.close()
     return result, c, frames, node_t
 
 
@@ -228,7 +335,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     t0 = time.time()
     if args.workload == "ensemble":
-        result, c, frames, node_t = ensemble_run(args, t0)
+        result, c, frames, node_t = ensemble_run(args, t0, learned=args.learned)
         if not args.no_cpu_baseline:
             node = node_t.cpu().numpy()
             inv = np.empty_like(node)
@@ -388,6 +495,28 @@ def main():
     else:
         result["cpu_baseline"] = None
     m.close()
+    if args.workload == "grid" and not args.no_variants and not (args.adaptive or args.floodplain or args.reservoirs
+                                                                  or args.snow_transport or args.host_forcing):
+        # the metric's comparison: the batched form (parameter ensemble, configs' combined workload)
+        # and the batched form with the learned warm start, against this plain line
+        del frames
+        torch.cuda.empty_cache()
+        ens, _, _, _ = ensemble_run(args, t0, learned=True)
+        plain = result["value"]
+        result["variants"] = {
+            "batched": {"value": ens["value"], "unit": ens["unit"], "ms_per_step": ens["ms_per_step"],
+                        "ratio_vs_plain": ens["value"] / plain, "workload": ens["config"]["workload"],
+                        "schedule": ens["config"]["schedule"], "roofline": ens["roofline"],
+                        "newton_updates_per_solve": ens["config"]["newton_updates_per_solve"]},
+            "batched_learned": {"value": ens["learned"]["value"], "unit": ens["unit"],
+                                "members": ens["learned"]["members"],
+                                "ms_per_step": ens["learned"]["ms_per_step_learned"],
+                                "ratio_vs_plain": ens["learned"]["value"] / plain,
+                                "ratio_vs_batched": ens["learned"]["ratio_vs_plain_ensemble"],
+                                "newton_updates_per_solve": ens["learned"]["newton_updates_per_solve"],
+                                "learned_starts_taken": ens["learned"]["learned_starts_taken"],
+                                "training": ens["learned"]["training"]},
+        }
     print(json.dumps(result), flush=True)
 
 

@@ -409,6 +409,43 @@ int32_t wfh_ens_step_bytes(wfh_ensemble* x, int64_t* bytes);
 const char* wfh_ens_last_error(const wfh_ensemble* x);
 void wfh_ens_destroy(wfh_ensemble* x);
 
+/* ---- Learned Newton warm start ---------------------------------------------------------------
+ * No counterpart in the reference: kinematic_wave (surface_process.jl:24-70) starts its Newton
+ * iteration from u_prev = q_prev^0.2 (or cbrt(C / alpha)). These entry points train and use a
+ * network that proposes a start instead: 8 features (log q_in, log q_prev, signed log of the
+ * lateral inflow as discharge, log alpha, beta, log dx, log dt, log slope) -> log10 q - 5 log10
+ * (reference start u), an 8-64-64-1 perceptron with BF16 weights and activations and FP32
+ * accumulation on the matrix cores. The iteration and its stopping rule stay the reference's;
+ * the proposed start is taken only where its residual is below the reference start's, so the
+ * result is the reference's Newton from a different start (within the same 1e-12 residual). */
+typedef struct wfh_mlp wfh_mlp;
+int32_t wfh_mlp_create(int32_t device, uint64_t seed, wfh_mlp** out);
+/* feature mean / standard deviation [8] and target mean / standard deviation */
+int32_t wfh_mlp_set_normalization(wfh_mlp* m, const float* x_mean, const float* x_std, float y_mean, float y_std);
+/* flat parameters [4737]: W1 [64][8], b1 [64], W2 [64][64], b2 [64], w3 [64], b3 (host floats) */
+int32_t wfh_mlp_set_params(wfh_mlp* m, const float* params);
+int32_t wfh_mlp_get_params(wfh_mlp* m, float* params);
+/* One pass over n device samples (x [n][8], t [n] normalised targets) in minibatches of `batch`
+ * (multiple of 64), an Adam step each; lr = 0: the gradient of the mean squared error over all n
+ * (no update). loss: mean squared error of the pass; grad [4737] (host, may be NULL): the last
+ * minibatch's gradient. */
+int32_t wfh_mlp_train(wfh_mlp* m, const float* x, const float* t, int64_t n, int64_t batch, float lr, double* loss,
+                      float* grad);
+/* network outputs (normalised target space) for device x [n][8] into device y [n] */
+int32_t wfh_mlp_predict(wfh_mlp* m, const float* x, int64_t n, float* y);
+const char* wfh_mlp_last_error(const wfh_mlp* m);
+void wfh_mlp_destroy(wfh_mlp* m);
+/* The ensemble's kinematic-wave solves with the network's start (mlp NULL: off); a wave consults
+ * the network where some member's previous discharge is below gate (INFINITY: always). */
+int32_t wfh_ens_set_learned(wfh_ensemble* x, wfh_mlp* mlp, double gate);
+/* Harvest training rows (features [8], target, Newton updates: 10 floats) of one plain solve in
+ * `every` into the device buffer rows [cap][10] during the following steps (rows NULL: stop). */
+int32_t wfh_ens_set_harvest(wfh_ensemble* x, float* rows, int64_t cap, int32_t every);
+int32_t wfh_ens_harvest_count(wfh_ensemble* x, int64_t* rows);
+/* Newton updates per solve since enabled: [2][16] bins (overland, river) + learned starts taken
+ * [2]; enable 1 restarts, 0 stops, other values keep; out (34 values) may be NULL. */
+int32_t wfh_ens_kw_histogram(wfh_ensemble* x, int32_t enable, uint64_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,18 +60,133 @@ struct EnsDev {
     const int* pre_slot;
     double* pre;
     size_t pre_stride;
+    // learned warm start of the kinematic-wave solves (LEARN instantiation): the network, and the
+    // gate: a wave consults it only where some member's previous discharge is below mlp_gate
+    const MlpOps* mlp;
+    double mlp_gate;
+    // optional instrumentation: Newton updates per solve ([2 families][16 bins] + learned starts
+    // used [2]), and harvesting of (features, target, updates) rows of sampled solves
+    unsigned long long* kw_hist;
+    float* harvest;
+    unsigned long long* harvest_n;
+    long long harvest_cap;
+    int harvest_every;
 };
 static_assert(sizeof(EnsDev) + 64 <= 4096, "EnsDev too large for a kernel argument");
 
 __device__ __forceinline__ double ens_ld(const Prm& a, int v) { return a.p[v & a.mask]; }
+
+constexpr int ENS_HIST_BINS = 16;
+__device__ __forceinline__ int kw_bin(int updates) {
+    if (updates <= 8) return updates;
+    if (updates >= 3000) return 15;
+    if (updates <= 16) return 9;
+    if (updates <= 32) return 10;
+    if (updates <= 64) return 11;
+    if (updates <= 128) return 12;
+    if (updates <= 256) return 13;
+    return 14;
+}
+
+// Features of a kinematic-wave solve for the warm-start network: q_in, previous q, lateral
+// inflow (as discharge), alpha, beta, dx, dt, slope (logarithms where they span decades).
+__device__ __forceinline__ void kw_features(const double q_in, const KwPrep& p, const double alpha, const double dt,
+                                            const double dx, const double slope, float* f) {
+    const float ql = (float)(p.q_lat * dx);
+    f[0] = log10f((float)q_in + 1e-9f);
+    f[1] = log10f((float)p.q_prev + 1e-9f);
+    f[2] = copysignf(log10f(1.0f + fabsf(ql) * 1e6f), ql);
+    f[3] = log10f((float)alpha);
+    f[4] = 0.6f;
+    f[5] = log10f((float)dx);
+    f[6] = log10f((float)dt);
+    f[7] = log10f((float)slope);
+}
+
+// The reference's start of the solve (u_prev, or cbrt(C / alpha) without previous flow).
+__device__ __forceinline__ double kw_ref_start(const double q_in, const KwPrep& p, const double alpha) {
+    const double C = p.dt_dx * q_in + p.a_u3 + p.lat;
+    return p.u_prev > 0.0 ? p.u_prev : wfh_cbrt(C / alpha);
+}
+
+// Kernel variants: plain solves; plain solves with the instrumentation (Newton-update histogram,
+// harvest of training rows); the learned warm start (with the histogram).
+enum { ENS_PLAIN = 0, ENS_INSTR = 1, ENS_LEARN = 2 };
+
+// One kinematic-wave solve of the ensemble (family fam: 0 overland, 1 river). ENS_PLAIN:
+// kw_solve. ENS_LEARN: every lane of the wave calls this (the network runs across the wave, 64
+// members at once) when some lane's previous discharge is below the gate; the network's start is
+// taken where its residual beats the reference start's (kw_solve_from).
+// The learned solve as a call of its own (taken only by waves the gate lets through): features,
+// the network, its proposed start, then the reference's Newton iteration (kw_solve_from).
+struct KwLearned {
+    double q, area;
+    int updates, used;
+};
+__device__ __noinline__ KwLearned kw_learned_solve(const MlpOps* W, const double q_in, const double dt_dx,
+                                                   const double u_prev, const double a_u3, const double lat,
+                                                   const double q_prev, const double q_lat, const double alpha,
+                                                   const double dt, const double dx, const double slope) {
+    KwPrep p;
+    p.dt_dx = dt_dx, p.u_prev = u_prev, p.a_u3 = a_u3, p.lat = lat, p.q_prev = q_prev, p.q_lat = q_lat;
+    float f[8];
+    kw_features(q_in, p, alpha, dt, dx, slope, f);
+    const float y = mlp_forward(*W, f, nullptr);
+    const float target = W->y_mean + W->y_std * y;
+    const double u0 = kw_ref_start(q_in, p, alpha) * (double)exp10f(0.2f * target);
+    KwLearned r;
+    bool used;
+    r.updates = kw_solve_from(q_in, p, alpha, u0, r.q, r.area, used);
+    r.used = used;
+    return r;
+}
+
+template <int VAR>
+__device__ __forceinline__ void ens_kw(const EnsDev& e, const MlpOps* W, const int fam, const unsigned key,
+                                       const double q_in, const KwPrep& p, const double alpha, const double dt,
+                                       const double dx, const double slope, double& q, double& area,
+                                       unsigned* hist) {
+    int updates = 0;
+    bool used = false;
+    if (VAR == ENS_LEARN && __any(p.q_prev < e.mlp_gate)) {
+        const KwLearned r =
+            kw_learned_solve(W, q_in, p.dt_dx, p.u_prev, p.a_u3, p.lat, p.q_prev, p.q_lat, alpha, dt, dx, slope);
+        q = r.q;
+        area = r.area;
+        updates = r.updates;
+        used = r.used != 0;
+    } else if (VAR != ENS_PLAIN && (hist || e.harvest)) {
+        updates = kw_solve_from(q_in, p, alpha, NaN_, q, area, used);
+    } else {
+        kw_solve(q_in, p, alpha, q, area);
+        return;
+    }
+    if (hist) {
+        atomicAdd(&hist[fam * ENS_HIST_BINS + kw_bin(updates)], 1u);
+        if (used) atomicAdd(&hist[2 * ENS_HIST_BINS + fam], 1u);
+    }
+    if (VAR == ENS_INSTR && e.harvest && q > 0.0 && (key % (unsigned)e.harvest_every) == 0u) {
+        const unsigned long long row = atomicAdd(e.harvest_n, 1ull);
+        if ((long long)row < e.harvest_cap) {
+            float f[8];
+            kw_features(q_in, p, alpha, dt, dx, slope, f);
+            float* o = e.harvest + row * 10;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) o[j] = f[j];
+            o[8] = log10f((float)q) - 5.0f * log10f((float)kw_ref_start(q_in, p, alpha));
+            o[9] = (float)updates;
+        }
+    }
+}
 
 // The update of cell v (node order) for member m: the same operations, in the same order, as
 // k_vertical followed by k_route's families for one cell (one subsurface and one overland
 // sub-step, S_r river sub-steps). MODE ENS_COLUMN stops after the vertical column (states
 // written, the routing inputs into the pre-buffer), ENS_ROUTE starts from them. Returns |land
 // balance error| (0 without the balance or in ENS_COLUMN).
-template <int N, int MODE>
-__device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const int m) {
+template <int N, int MODE, int VAR>
+__device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const int m, const MlpOps* W,
+                                           unsigned* hist) {
     const size_t M = (size_t)e.M;
     const size_t i = (size_t)v * M + m;
     const size_t nM = (size_t)e.n * M;
@@ -251,7 +366,8 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
         double qin = 0.0;
         if (sfw > 0.0) qin = qin_all;
         double qn, ar;
-        kw_solve(qin, op, alpha, qn, ar);
+        ens_kw<VAR>(e, W, 0, (unsigned)v * 2654435761u ^ (unsigned)m * 40503u, qin, op, alpha, dts, flen, slope, qn,
+                      ar, hist);
         double oh = oh0;
         if (sfw > 0.0) oh = ar / sfw;
         e.ol_q[i] = qn;
@@ -278,7 +394,8 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
             const KwPrep rp = kw_prepare(rq, rqlat_all, ralpha, dts, rfl);
             const double qin = 0.0 + sumq;
             double qn, ar;
-            kw_solve(qin, rp, ralpha, qn, ar);
+            ens_kw<VAR>(e, W, 1, ((unsigned)v * 2654435761u ^ (unsigned)m * 40503u) + 97u * (unsigned)(k + 1), qin, rp,
+                          ralpha, dts, rfl, slope, qn, ar, hist);
             rq = qn;
             rh = ar / rfw;
             r_storage = rfl * ar;
@@ -293,21 +410,36 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
     return bal;
 }
 
-template <int N, int MODE>
+template <int N, int MODE, int VAR>
 __global__ void __launch_bounds__(256, 2) k_ens_stage(EnsDev e, int s0, int ns) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    if (w >= ns * e.G) return;
-    const int st = __builtin_amdgcn_readfirstlane(s0 + w / e.G);
-    const int g = __builtin_amdgcn_readfirstlane(w % e.G);
-    const int m = g * WAVE + lane;
-    const int c0 = e.strand_ptr[st], c1 = e.strand_ptr[st + 1];
-    double worst = 0.0;
-    for (int c = c0; c < c1; ++c) {
-        const int v = __builtin_amdgcn_readfirstlane(e.strand_cell[c]);
-        worst = wfh_max(worst, ens_cell<N, MODE>(e, v, m));
+    // the network's operands in LDS (before any wave leaves: every wave takes the barrier)
+    __shared__ unsigned hist_s[2 * ENS_HIST_BINS + 2];
+    const MlpOps* W = VAR == ENS_LEARN ? e.mlp : nullptr;  // read from L2 by the (rare) network calls
+    unsigned* hist = nullptr;
+    if (VAR != ENS_PLAIN && e.kw_hist) {  // (before any wave leaves: every wave takes the barrier)
+        for (int k = threadIdx.x; k < 2 * ENS_HIST_BINS + 2; k += blockDim.x) hist_s[k] = 0u;
+        hist = hist_s;
+        __syncthreads();
     }
-    if (e.balance && worst > 0.0) atomicMax(&e.bal_max[m], (unsigned long long)wfh_bits(worst));
+    if (w < ns * e.G) {
+        const int st = __builtin_amdgcn_readfirstlane(s0 + w / e.G);
+        const int g = __builtin_amdgcn_readfirstlane(w % e.G);
+        const int m = g * WAVE + lane;
+        const int c0 = e.strand_ptr[st], c1 = e.strand_ptr[st + 1];
+        double worst = 0.0;
+        for (int c = c0; c < c1; ++c) {
+            const int v = __builtin_amdgcn_readfirstlane(e.strand_cell[c]);
+            worst = wfh_max(worst, ens_cell<N, MODE, VAR>(e, v, m, W, hist));
+        }
+        if (e.balance && worst > 0.0) atomicMax(&e.bal_max[m], (unsigned long long)wfh_bits(worst));
+    }
+    if (hist) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < 2 * ENS_HIST_BINS + 2; k += blockDim.x)
+            if (hist_s[k]) atomicAdd(&e.kw_hist[k], (unsigned long long)hist_s[k]);
+    }
 }
 
 __global__ void k_ens_zero_u64(unsigned long long* p, int n) {
@@ -414,6 +546,8 @@ struct wfh_ensemble {
     double timed_ms = 0.0;
     int64_t timed_steps = 0;
     bool forcing_set = false;
+    unsigned long long* hist_buf = nullptr;   // Newton update histogram (allocated when first enabled)
+    unsigned long long* harvest_ctr = nullptr;
     int* d_index_node = nullptr;              // land index -> node (for per-member reads)
     double *fP = nullptr, *fE = nullptr, *fT = nullptr;
 };
@@ -537,26 +671,37 @@ void ens_launch_stages(wfh_ensemble* x, const EnsSchedule& s, hipStream_t st) {
         const auto& sg = s.stages[k];
         const int waves = sg.second * e.G;
         const dim3 grid((waves + wpb - 1) / wpb), block(s.block);
+        const int var = e.mlp ? ENS_LEARN : (e.kw_hist || e.harvest) ? ENS_INSTR : ENS_PLAIN;
         if (s.modes[k] == ENS_COLUMN)
-            hipLaunchKernelGGL((k_ens_stage<N, ENS_COLUMN>), grid, block, 0, st, e, sg.first, sg.second);
-        else if (s.modes[k] == ENS_ROUTE)
-            hipLaunchKernelGGL((k_ens_stage<N, ENS_ROUTE>), grid, block, 0, st, e, sg.first, sg.second);
-        else
-            hipLaunchKernelGGL((k_ens_stage<N, ENS_FUSED>), grid, block, 0, st, e, sg.first, sg.second);
+            hipLaunchKernelGGL((k_ens_stage<N, ENS_COLUMN, ENS_PLAIN>), grid, block, 0, st, e, sg.first, sg.second);
+        else if (s.modes[k] == ENS_ROUTE) {
+            auto kr = var == ENS_LEARN   ? k_ens_stage<N, ENS_ROUTE, ENS_LEARN>
+                      : var == ENS_INSTR ? k_ens_stage<N, ENS_ROUTE, ENS_INSTR>
+                                         : k_ens_stage<N, ENS_ROUTE, ENS_PLAIN>;
+            hipLaunchKernelGGL(kr, grid, block, 0, st, e, sg.first, sg.second);
+        } else {
+            auto kf = var == ENS_LEARN   ? k_ens_stage<N, ENS_FUSED, ENS_LEARN>
+                      : var == ENS_INSTR ? k_ens_stage<N, ENS_FUSED, ENS_INSTR>
+                                         : k_ens_stage<N, ENS_FUSED, ENS_PLAIN>;
+            hipLaunchKernelGGL(kf, grid, block, 0, st, e, sg.first, sg.second);
+        }
     }
 }
 
 int32_t ens_prepare(wfh_ensemble* x, EnsSchedule& s) {
-    if (s.d_ptr) return WFH_OK;
+    if (s.exec) return WFH_OK;
     ens_build_schedule(x, s);
     if (s.npre > x->pre_cap) return efail(x, WFH_ERR_STATE, "internal error: pre-buffer smaller than a schedule");
-    s.d_ptr = ealloc<int>(x, s.strand_ptr.size());
-    s.d_cell = ealloc<int>(x, s.strand_cell.size());
-    s.d_pre_slot = ealloc<int>(x, s.pre_slot.size());
-    if (!s.d_ptr || !s.d_cell || !s.d_pre_slot) return efail(x, WFH_ERR_HIP, "hipMalloc schedule");
-    EHIP(x, hipMemcpy(s.d_pre_slot, s.pre_slot.data(), s.pre_slot.size() * sizeof(int), hipMemcpyHostToDevice));
-    EHIP(x, hipMemcpy(s.d_ptr, s.strand_ptr.data(), s.strand_ptr.size() * sizeof(int), hipMemcpyHostToDevice));
-    EHIP(x, hipMemcpy(s.d_cell, s.strand_cell.data(), s.strand_cell.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!s.d_ptr) {
+        s.d_ptr = ealloc<int>(x, s.strand_ptr.size());
+        s.d_cell = ealloc<int>(x, s.strand_cell.size());
+        s.d_pre_slot = ealloc<int>(x, s.pre_slot.size());
+        if (!s.d_ptr || !s.d_cell || !s.d_pre_slot) return efail(x, WFH_ERR_HIP, "hipMalloc schedule");
+        EHIP(x, hipMemcpy(s.d_pre_slot, s.pre_slot.data(), s.pre_slot.size() * sizeof(int), hipMemcpyHostToDevice));
+        EHIP(x, hipMemcpy(s.d_ptr, s.strand_ptr.data(), s.strand_ptr.size() * sizeof(int), hipMemcpyHostToDevice));
+        EHIP(x, hipMemcpy(s.d_cell, s.strand_cell.data(), s.strand_cell.size() * sizeof(int),
+                          hipMemcpyHostToDevice));
+    }
     // the step's launches, recorded once into a graph
     hipStream_t st = x->base->stream;
     EHIP(x, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -570,6 +715,17 @@ int32_t ens_prepare(wfh_ensemble* x, EnsSchedule& s) {
     EHIP(x, hipStreamEndCapture(st, &s.graph));
     EHIP(x, hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0));
     return WFH_OK;
+}
+
+// The recorded graphs bake in the kernel arguments: drop them when an option changes.
+void ens_invalidate(wfh_ensemble* x) {
+    hipStreamSynchronize(x->base->stream);
+    for (EnsSchedule& c : x->cand) {
+        if (c.exec) hipGraphExecDestroy(c.exec);
+        if (c.graph) hipGraphDestroy(c.graph);
+        c.exec = nullptr;
+        c.graph = nullptr;
+    }
 }
 
 }  // namespace
@@ -736,6 +892,13 @@ extern "C" int32_t wfh_ens_create(const wfh_config* cfg, const wfh_static* st, i
     e.fa = fa;
     e.fb = fb;
     e.khfrac = Prm{khd, 0xFFFFFFFFu};
+    e.mlp = nullptr;
+    e.mlp_gate = INFINITY;
+    e.kw_hist = nullptr;
+    e.harvest = nullptr;
+    e.harvest_n = nullptr;
+    e.harvest_cap = 0;
+    e.harvest_every = 1;
     e.P = x->fP;
     e.E = x->fE;
     e.T = x->fT;
@@ -1008,6 +1171,77 @@ extern "C" int32_t wfh_ens_step_bytes(wfh_ensemble* x, int64_t* bytes) {
     b += 8 * nr * M * (4 + 4 + x->e.S_r * 1) + 8 * Er * M * x->e.S_r;  // river states, sub-step q, folds
     b += n * (8 * 40 + 4 * 4);                                     // shared parameters and topology
     *bytes = b;
+    return WFH_OK;
+}
+
+// The learned warm start of the kinematic-wave solves (NULL: off): consulted by a wave where some
+// member's previous discharge is below `gate` (INFINITY: every solve).
+extern "C" int32_t wfh_ens_set_learned(wfh_ensemble* x, wfh_mlp* mlp, double gate) {
+    if (!x) return WFH_ERR_ARG;
+    if (mlp && mlp->device != x->base->cfg.device) return efail(x, WFH_ERR_ARG, "network on another device");
+    if (!(gate >= 0.0)) return efail(x, WFH_ERR_ARG, "gate must be >= 0");
+    ens_invalidate(x);
+    x->e.mlp = mlp ? mlp->ops : nullptr;
+    x->e.mlp_gate = gate;
+    return WFH_OK;
+}
+
+// Harvest (features [8], target, Newton updates) rows of the plain solves of the following steps
+// into the device buffer rows [cap][10], one solve in `every` (by a hash of cell, member and
+// sub-step); NULL stops. The target is log10(q) - 5 log10(reference start u).
+extern "C" int32_t wfh_ens_set_harvest(wfh_ensemble* x, float* rows, int64_t cap, int32_t every) {
+    if (!x) return WFH_ERR_ARG;
+    if (rows && (cap <= 0 || every < 1)) return efail(x, WFH_ERR_ARG, "cap must be > 0 and every >= 1");
+    EHIP(x, hipSetDevice(x->base->cfg.device));
+    if (!x->harvest_ctr && !(x->harvest_ctr = ealloc<unsigned long long>(x, 1)))
+        return efail(x, WFH_ERR_HIP, "hipMalloc");
+    ens_invalidate(x);
+    hipLaunchKernelGGL(k_ens_zero_u64, dim3(1), dim3(64), 0, x->base->stream, x->harvest_ctr, 1);
+    EHIP(x, hipGetLastError());
+    x->e.harvest = rows;
+    x->e.harvest_n = x->harvest_ctr;
+    x->e.harvest_cap = cap;
+    x->e.harvest_every = every;
+    return WFH_OK;
+}
+
+extern "C" int32_t wfh_ens_harvest_count(wfh_ensemble* x, int64_t* rows) {
+    if (!x || !rows) return WFH_ERR_ARG;
+    unsigned long long v = 0;
+    EHIP(x, hipSetDevice(x->base->cfg.device));
+    EHIP(x, hipStreamSynchronize(x->base->stream));
+    if (x->harvest_ctr) EHIP(x, hipMemcpy(&v, x->harvest_ctr, sizeof v, hipMemcpyDeviceToHost));
+    *rows = (int64_t)std::min<unsigned long long>(v, (unsigned long long)std::max<int64_t>(x->e.harvest_cap, 0));
+    return WFH_OK;
+}
+
+// Histogram of Newton updates per kinematic-wave solve over the steps since it was enabled:
+// hist [2][16] (overland, river; bins 0..8 updates, 9-16, 17-32, 33-64, 65-128, 129-256,
+// 257-2999, 3000) then learned starts taken [2]. enable = 1 (re)starts counting, 0 stops; out
+// (34 values) may be NULL.
+extern "C" int32_t wfh_ens_kw_histogram(wfh_ensemble* x, int32_t enable, uint64_t* out) {
+    if (!x) return WFH_ERR_ARG;
+    EHIP(x, hipSetDevice(x->base->cfg.device));
+    const int nb = 2 * ENS_HIST_BINS + 2;
+    if (out) {
+        EHIP(x, hipStreamSynchronize(x->base->stream));
+        if (x->hist_buf)
+            EHIP(x, hipMemcpy(out, x->hist_buf, nb * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        else
+            std::fill(out, out + nb, (uint64_t)0);
+    }
+    if (enable == 1) {
+        if (!x->hist_buf && !(x->hist_buf = ealloc<unsigned long long>(x, nb))) return efail(x, WFH_ERR_HIP, "hipMalloc");
+        hipLaunchKernelGGL(k_ens_zero_u64, dim3(1), dim3(64), 0, x->base->stream, x->hist_buf, nb);
+        EHIP(x, hipGetLastError());
+        if (!x->e.kw_hist) {
+            ens_invalidate(x);
+            x->e.kw_hist = x->hist_buf;
+        }
+    } else if (enable == 0 && x->e.kw_hist) {
+        ens_invalidate(x);
+        x->e.kw_hist = nullptr;
+    }
     return WFH_OK;
 }
 

@@ -290,6 +290,71 @@ WFH_HD void kw_solve(double q_in, const KwPrep& p, double alpha, double& q, doub
     area = alpha * u3;
 }
 
+// The Newton iteration of kw_solve from the start u (the same updates, stopping rule and cycle
+// shortcut); returns the final u (before the KIN_WAVE_MIN_FLOW floor) and the number of updates
+// the reference would take (3000 when the shortcut jumps to the limit).
+WFH_HD double kw_newton(double u, const double C, const double dt_dx, const double alpha, int& updates) {
+    const double c1 = 5.0 * dt_dx, c2 = 3.0 * alpha;
+    double u_pp = NaN_;
+    updates = 0;
+    for (int k = 0; k < 3000; ++k) {
+        const double u2 = u * u;
+        const double u3 = u2 * u;
+        const double f_u = u3 * (dt_dx * u2 + alpha) - C;
+        if (wfh_fabs(f_u) <= 1.0e-12) break;
+        const double df = u2 * (c1 * u2 + c2);
+        const double u_c = u;
+        u -= f_u / df;
+        if (wfh_isnan(u) || u <= 0.0) u = WFH_KIN_WAVE_MIN_FLOW_QROOT;
+        const int n = k + 1;
+        updates = n;
+        if (u == u_c) {
+            updates = 3000;
+            break;
+        }
+        if (u == u_pp) {
+            if ((3000 - n) & 1) u = u_c;
+            updates = 3000;
+            break;
+        }
+        u_pp = u_c;
+    }
+    return u;
+}
+
+// kw_solve with a proposed start u0 (the learned warm start): u0 is used when it is finite,
+// positive and its residual |f(u0)| is smaller than that of the reference's start; otherwise the
+// reference's start (u_prev, or cbrt(C / alpha)) is kept. The iteration and its stopping rule
+// are the reference's either way. With u0 = NaN the result is kw_solve's, bit for bit. Returns
+// the Newton updates taken; `used` tells whether u0 was the start.
+WFH_HD int kw_solve_from(double q_in, const KwPrep& p, double alpha, const double u0, double& q, double& area,
+                         bool& used) {
+    used = false;
+    if (q_in + p.q_prev + p.q_lat == 0.0) {
+        q = 0.0;
+        area = 0.0;
+        return 0;
+    }
+    const double dt_dx = p.dt_dx;
+    const double C = dt_dx * q_in + p.a_u3 + p.lat;
+    double u = p.u_prev > 0.0 ? p.u_prev : wfh_cbrt(C / alpha);
+    if (u0 > 0.0 && u0 < INFINITY) {
+        const double r2 = u * u, f_ref = r2 * u * (dt_dx * r2 + alpha) - C;
+        const double l2 = u0 * u0, f_l = l2 * u0 * (dt_dx * l2 + alpha) - C;
+        if (wfh_fabs(f_l) < wfh_fabs(f_ref)) {
+            u = u0;
+            used = true;
+        }
+    }
+    int updates;
+    u = kw_newton(u, C, dt_dx, alpha, updates);
+    u = wfh_max(u, WFH_KIN_WAVE_MIN_FLOW_QROOT);
+    const double u3 = u * u * u;
+    q = u3 * u * u;
+    area = alpha * u3;
+    return updates;
+}
+
 WFH_HD void kinematic_wave(double q_in, double q_prev, double q_lat, double alpha, double dt, double dx,
                            double& q, double& area) {
     const KwPrep p = kw_prepare(q_prev, q_lat, alpha, dt, dx);

@@ -4029,4 +4029,5 @@ extern "C" int32_t wfh_kernel_bytes(wfh_handle* h, int64_t* vertical_bytes, int6
 // ============================================================================================
 // Parameter ensembles (member-innermost layout, staged launches replayed from a HIP graph)
 // ============================================================================================
+#include "wfh_mlp.h"
 #include "wfh_ensemble.h"

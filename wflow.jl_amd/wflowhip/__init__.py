@@ -11,4 +11,4 @@ Model raises if the library is missing or no GPU is visible.
 """
 
 from ._abi import Var  # noqa: F401
-from .model import Ensemble, Model, load_library, library_path, member_catchment  # noqa: F401
+from .model import Ensemble, Mlp, Model, load_library, library_path, member_catchment  # noqa: F401

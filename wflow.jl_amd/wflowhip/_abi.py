@@ -187,7 +187,9 @@ EXPORTS = ["wfh_create", "wfh_set_forcing", "wfh_step", "wfh_get", "wfh_set", "w
            "wfh_ens_create", "wfh_ens_set_forcing", "wfh_ens_step", "wfh_ens_get", "wfh_ens_balance",
            "wfh_ens_set_schedule", "wfh_ens_set_autotune", "wfh_ens_get_schedule", "wfh_ens_sync",
            "wfh_ens_set_timing", "wfh_ens_get_timing", "wfh_ens_step_bytes", "wfh_ens_last_error",
-           "wfh_ens_destroy"]
+           "wfh_ens_destroy", "wfh_mlp_create", "wfh_mlp_set_normalization", "wfh_mlp_set_params",
+           "wfh_mlp_get_params", "wfh_mlp_train", "wfh_mlp_predict", "wfh_mlp_last_error", "wfh_mlp_destroy",
+           "wfh_ens_set_learned", "wfh_ens_set_harvest", "wfh_ens_harvest_count", "wfh_ens_kw_histogram"]
 
 
 def var_count(var, n, nriv, nlayers, nres=0):

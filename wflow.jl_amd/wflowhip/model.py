@@ -355,3 +355,149 @@ def member_catchment(catchment, kv0_factor, f_factor, n_factor):
     c.arrays["land_manning_n"] = catchment.arrays["land_manning_n"] * float(n_factor)
     c.arrays["river_manning_n"] = catchment.arrays["river_manning_n"] * float(n_factor)
     return c
+
+
+MLP_NPARAM = 64 * 8 + 64 + 64 * 64 + 64 + 64 + 1
+
+
+def _bind_mlp(L):
+    P = ctypes.c_void_p
+    i32, i64 = ctypes.c_int32, ctypes.c_int64
+    L.wfh_mlp_create.argtypes = [i32, ctypes.c_uint64, ctypes.POINTER(P)]
+    L.wfh_mlp_create.restype = i32
+    L.wfh_mlp_set_normalization.argtypes = [P, P, P, ctypes.c_float, ctypes.c_float]
+    L.wfh_mlp_set_normalization.restype = i32
+    for name in ("wfh_mlp_set_params", "wfh_mlp_get_params"):
+        getattr(L, name).argtypes = [P, P]
+        getattr(L, name).restype = i32
+    L.wfh_mlp_train.argtypes = [P, P, P, i64, i64, ctypes.c_float, P, P]
+    L.wfh_mlp_train.restype = i32
+    L.wfh_mlp_predict.argtypes = [P, P, i64, P]
+    L.wfh_mlp_predict.restype = i32
+    L.wfh_mlp_last_error.argtypes = [P]
+    L.wfh_mlp_last_error.restype = ctypes.c_char_p
+    L.wfh_mlp_destroy.argtypes = [P]
+    L.wfh_mlp_destroy.restype = None
+    L.wfh_ens_set_learned.argtypes = [P, P, ctypes.c_double]
+    L.wfh_ens_set_learned.restype = i32
+    L.wfh_ens_set_harvest.argtypes = [P, P, i64, i32]
+    L.wfh_ens_set_harvest.restype = i32
+    L.wfh_ens_harvest_count.argtypes = [P, P]
+    L.wfh_ens_harvest_count.restype = i32
+    L.wfh_ens_kw_histogram.argtypes = [P, i32, P]
+    L.wfh_ens_kw_histogram.restype = i32
+
+
+def _lib_with_mlp():
+    L = load_library()
+    if not hasattr(L, "_mlp_bound"):
+        _bind_mlp(L)
+        L._mlp_bound = True
+    return L
+
+
+class Mlp:
+    """The learned Newton warm start network (8-64-64-1, BF16 on the matrix cores): features of a
+    kinematic-wave solve -> log10 q - 5 log10 (reference start u). Trained on the GPU with the
+    library's own forward and backward kernels. Data are torch tensors on the device."""
+
+    def __init__(self, device=0, seed=0):
+        self.lib = _lib_with_mlp()
+        h = ctypes.c_void_p()
+        rc = self.lib.wfh_mlp_create(int(device), int(seed), ctypes.byref(h))
+        if rc != 0:
+            raise WflowError(rc, self.lib.wfh_mlp_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.wfh_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise WflowError(rc, self.lib.wfh_mlp_last_error(self.h).decode())
+
+    def set_normalization(self, x_mean, x_std, y_mean, y_std):
+        xm = np.ascontiguousarray(x_mean, np.float32)
+        xs = np.ascontiguousarray(x_std, np.float32)
+        self._check(self.lib.wfh_mlp_set_normalization(self.h, xm.ctypes.data, xs.ctypes.data, float(y_mean),
+                                                        float(y_std)))
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == MLP_NPARAM
+        self._check(self.lib.wfh_mlp_set_params(self.h, p.ctypes.data))
+
+    def params(self):
+        p = np.empty(MLP_NPARAM, np.float32)
+        self._check(self.lib.wfh_mlp_get_params(self.h, p.ctypes.data))
+        return p
+
+    def train_epoch(self, x, t, batch=8192, lr=1e-3):
+        """One pass of Adam steps over device tensors x [n, 8] (float32), t [n] (normalised);
+        returns the pass's mean squared error."""
+        assert x.is_cuda and x.dtype.itemsize == 4 and x.is_contiguous() and t.is_contiguous()
+        loss = ctypes.c_double()
+        self._check(self.lib.wfh_mlp_train(self.h, x.data_ptr(), t.data_ptr(), int(t.numel()), int(batch), float(lr),
+                                           ctypes.byref(loss), None))
+        return loss.value
+
+    def gradient(self, x, t):
+        """Gradient of the mean squared error over all samples (no update), and the loss."""
+        g = np.empty(MLP_NPARAM, np.float32)
+        loss = ctypes.c_double()
+        self._check(self.lib.wfh_mlp_train(self.h, x.data_ptr(), t.data_ptr(), int(t.numel()), 64, 0.0,
+                                           ctypes.byref(loss), g.ctypes.data))
+        return g, loss.value
+
+    def predict(self, x, out):
+        self._check(self.lib.wfh_mlp_predict(self.h, x.data_ptr(), int(out.numel()), out.data_ptr()))
+        return out
+
+
+def _ens_learned_methods():
+    def set_learned(self, mlp, gate=float("inf")):
+        L = _lib_with_mlp()
+        self._check(L.wfh_ens_set_learned(self.h, mlp.h if mlp is not None else None, float(gate)))
+        self._mlp = mlp
+
+    def set_harvest(self, rows, every=1):
+        L = _lib_with_mlp()
+        if rows is None:
+            self._check(L.wfh_ens_set_harvest(self.h, None, 0, 1))
+        else:
+            assert rows.is_cuda and rows.dtype.itemsize == 4 and rows.dim() == 2 and rows.shape[1] == 10
+            self._check(L.wfh_ens_set_harvest(self.h, rows.data_ptr(), int(rows.shape[0]), int(every)))
+        self._harvest = rows
+
+    def harvest_count(self):
+        L = _lib_with_mlp()
+        v = ctypes.c_int64()
+        self._check(L.wfh_ens_harvest_count(self.h, ctypes.addressof(v)))
+        return v.value
+
+    def kw_histogram(self, enable=None):
+        """(counts [2][16], learned starts [2]); enable True restarts counting, False stops."""
+        L = _lib_with_mlp()
+        out = np.zeros(34, np.uint64)
+        flag = 2 if enable is None else int(bool(enable))
+        self._check(L.wfh_ens_kw_histogram(self.h, flag, out.ctypes.data))
+        return out[:32].reshape(2, 16), out[32:]
+
+    Ensemble.set_learned = set_learned
+    Ensemble.set_harvest = set_harvest
+    Ensemble.harvest_count = harvest_count
+    Ensemble.kw_histogram = kw_histogram
+
+
+_ens_learned_methods()
+
+# Upper edges of the Newton-update bins of Ensemble.kw_histogram
+KW_BIN_EDGES = [0, 1, 2, 3, 4, 5, 6, 7, 8, 16, 32, 64, 128, 256, 2999, 3000]
