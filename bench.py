@@ -51,6 +51,10 @@ def parse():
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--nx", type=int, default=4096)
     ap.add_argument("--frames", type=int, default=24, help="distinct forcing frames kept in HBM")
+    ap.add_argument("--spinup", type=int, default=-1,
+                    help="model steps run before the warm-up (each with its own forcing frame), so that the "
+                         "timed steps end at the configuration's last hour (720 for the grid, 480 for the batch); "
+                         "-1: that default, 0: a cold start")
     ap.add_argument("--cpu-sample-cells", type=int, default=200_000, help="cells per CPU thread")
     ap.add_argument("--cpu-threads", type=int, default=16, help="CPU threads of the baseline (at most the cores)")
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
@@ -78,27 +82,34 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(catchment, frames_host, args):
+STATE_VARS_RESTART = None  # filled in main (Var members)
+
+
+def cpu_baseline(catchment, frames_host, args, states=None, hour0=0):
     """The oracle on complete basins of the same catchment; basins exchange no water, so a set of
-    whole basins is an exact sub-problem. The basins are split into one group per thread (up to
-    16, the host share of one GPU), each group stepped by its own oracle model in its own thread
-    (the C oracle releases the interpreter lock), the way the reference spreads independent
-    subbasins over its threads. The single-thread rate of one group is reported beside it."""
+    whole basins is an exact sub-problem. The largest basin (the main stem) is always in the
+    sample; further basins are drawn until every thread (up to 16, the host share of one GPU) has
+    about --cpu-sample-cells cells. Each group of basins is stepped by its own oracle model in its
+    own thread (the C oracle releases the interpreter lock), the way the reference spreads
+    independent subbasins over its threads. With `states` (land / river order arrays of the GPU
+    model at the timed region's start) the oracle starts from the same state and steps the same
+    forcing frames as the timed GPU steps. The single-thread rate of one group is reported too."""
     import threading
 
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    from wflowhip import synth
+    from wflowhip import Var, _abi, synth
 
     threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
     lab = synth.basin_ids(catchment.arrays["ldd"], catchment.nx, catchment.ny, catchment.arrays["cell_index"])
     ids, counts = np.unique(lab, return_counts=True)
+    big = int(np.argmax(counts))
     rng = np.random.default_rng(5)
-    order = rng.permutation(len(ids))
-    want = args.cpu_sample_cells * threads
-    take, total = [], 0
+    order = [k for k in rng.permutation(len(ids)) if k != big]
+    want = args.cpu_sample_cells * max(1, threads - 1)
+    take, total = [big], 0
     for k in order:
         if total + counts[k] > want and total > 0:
             continue
@@ -106,32 +117,46 @@ def cpu_baseline(catchment, frames_host, args):
         total += counts[k]
         if total >= 0.9 * want:
             break
-    # greedy split into groups of similar size
-    groups = [[] for _ in range(threads)]
+    # the largest basin alone in its group, the rest split greedily into groups of similar size
+    groups = [[ids[big]]] + [[] for _ in range(threads - 1)]
     sizes = [0] * threads
-    for k in sorted(take, key=lambda k: -counts[k]):
-        g = int(np.argmin(sizes))
+    for k in sorted(take[1:], key=lambda k: -counts[k]):
+        g = 1 + int(np.argmin(sizes[1:])) if threads > 1 else 0
         groups[g].append(ids[k])
         sizes[g] += counts[k]
     groups = [g for g in groups if g]
     masks = [np.isin(lab, np.array(g)) for g in groups]
     kw = dict(adaptive=args.adaptive, snow_transport=args.snow_transport)
     subs = [catchment.subset(mk) for mk in masks]
-    mdl_nres = [sc.nres for sc in subs]
-    models = [O.OracleModel(sc, **kw) for sc in subs]
-    frames = [[tuple(f[mk] for f in frames_host[k % len(frames_host)]) for k in range(args.cpu_sample_steps)]
-              for mk in masks]
+    riv = catchment.arrays["river_location"].astype(bool)
+    rmasks = [mk[riv] for mk in masks]
+
+    def make(j):
+        mdl = O.OracleModel(subs[j], **kw)
+        if states is not None:
+            for var, a in states.items():
+                if _abi.Var(var) in _abi.RIVER_VARS:
+                    mdl.set(var, a[rmasks[j]])
+                elif _abi.Var(var) in _abi.LAYERED_VARS:
+                    mdl.set(var, a.reshape(-1, catchment.n)[:, masks[j]].ravel())
+                else:
+                    mdl.set(var, a[masks[j]])
+        return mdl
+
+    models = [make(j) for j in range(len(subs))]
+    frames = [[tuple(f[mk] for f in frames_host[k]) for k in range(len(frames_host))] for mk in masks]
 
     def run(j):
-        for k in range(args.cpu_sample_steps):
+        for k in range(len(frames_host)):
             models[j].step(*frames[j][k])
 
-    # single-thread rate on the first group, then all groups at once
+    # single-thread rate on the last (random-basin) group, then all groups at once
+    js = len(models) - 1
     t0 = time.perf_counter()
-    run(0)
+    run(js)
     t_single = time.perf_counter() - t0
-    models[0].close()
-    models[0] = O.OracleModel(catchment.subset(masks[0]), **kw)
+    models[js].close()
+    models[js] = make(js)
     ths = [threading.Thread(target=run, args=(j,)) for j in range(len(models))]
     t0 = time.perf_counter()
     for t in ths:
@@ -142,15 +167,18 @@ def cpu_baseline(catchment, frames_host, args):
     for mdl in models:
         mdl.close()
     cells = int(sum(int(mk.sum()) for mk in masks))
+    steps = len(frames_host)
     return {
-        "value": cells * args.cpu_sample_steps / wall,
+        "value": cells * steps / wall,
         "unit": "cell-timesteps/s",
         "cores": len(models),
         "kind": "port",
-        "single_thread_value": int(masks[0].sum()) * args.cpu_sample_steps / t_single,
-        "sample": f"{len(take)} complete basins ({cells} cells, {sum(mdl_nres)} reservoirs) of the same workload in "
-                  f"{len(models)} groups, one thread each, {args.cpu_sample_steps} steps, C oracle (-O2), "
-                  f"update_model! only"
+        "single_thread_value": int(masks[js].sum()) * steps / t_single,
+        "sample": f"{len(take)} complete basins ({cells} cells, including the largest basin of "
+                  f"{int(counts[big])} cells, which holds the main stem) of the same workload in {len(models)} "
+                  f"groups, one thread each, {steps} steps, C oracle (-O2), update_model! only"
+                  + (f", started from the GPU model's state at model hour {hour0} and stepped over the same "
+                     f"forcing as the timed GPU steps" if states is not None else ", cold start")
                   + (" (adaptive sub-steps of each group itself)" if args.adaptive else ""),
     }
 
@@ -378,8 +406,22 @@ def main():
     node = m.node_order()
     node_t = torch.as_tensor(node, device="cuda")
     elev = torch.as_tensor(c.arrays["elevation"], device="cuda")[node_t].contiguous()
-    nframes = max(1, min(args.frames, args.steps + args.warmup))
-    frames = [synth.forcing_torch(c, k, elev) for k in range(nframes)]
+    # spin-up: the configuration's run up to the timed hours (each step its own forcing frame)
+    plain_cfg = not (args.adaptive or args.floodplain or args.reservoirs or args.snow_transport or args.host_forcing)
+    run_len = 720 if args.workload == "grid" else 480
+    h0 = args.spinup if args.spinup >= 0 else (max(0, run_len - args.warmup - args.steps) if plain_cfg else 0)
+    for k in range(h0):
+        fr = synth.forcing_torch(c, k, elev)
+        torch.cuda.synchronize()
+        m.set_forcing(*fr, memkind=_abi.WFH_MEM_DEVICE_NODE_ORDER)
+        m.update_model()
+        m.sync()
+        if (k + 1) % 100 == 0:
+            print(f"# spin-up: {k + 1} of {h0} steps ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    if h0:
+        workload += f", timed over model hours {h0 + args.warmup}-{h0 + args.warmup + args.steps} after a {h0}-step spin-up"
+    nframes = max(1, min(max(args.frames, args.steps + args.warmup), 64))
+    frames = [synth.forcing_torch(c, h0 + k, elev) for k in range(nframes)]
     torch.cuda.synchronize()
     print(f"# {nframes} forcing frames on the GPU ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
 
@@ -400,6 +442,12 @@ def main():
         step(k)
     m.sync()
     torch.cuda.synchronize()
+    states = None
+    if not args.no_cpu_baseline and plain_cfg:  # the state the CPU baseline starts from
+        states = {int(v): m.get(v) for v in (Var.CANOPY_STORAGE, Var.SNOW_STORAGE, Var.SNOW_WATER,
+                                              Var.SOIL_TEMPERATURE, Var.SATWATERDEPTH, Var.USTOREDEPTH,
+                                              Var.USTORELAYERDEPTH, Var.SSF_Q, Var.LAND_Q, Var.LAND_H,
+                                              Var.LAND_STORAGE, Var.RIVER_Q, Var.RIVER_H, Var.RIVER_STORAGE)}
     m.set_timing(True)
     t1 = time.perf_counter()
     subs = []
@@ -489,9 +537,10 @@ def main():
         frames_host = []
         inv = np.empty_like(node)
         inv[node] = np.arange(node.size)
-        for k in range(min(args.cpu_sample_steps, nframes)):
+        k0 = args.warmup if states is not None else 0
+        for k in range(k0, min(k0 + args.cpu_sample_steps, nframes)):
             frames_host.append(tuple(f.cpu().numpy()[inv] for f in frames[k]))
-        result["cpu_baseline"] = cpu_baseline(c, frames_host, args)
+        result["cpu_baseline"] = cpu_baseline(c, frames_host, args, states, h0 + k0)
     else:
         result["cpu_baseline"] = None
     m.close()

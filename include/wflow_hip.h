@@ -313,6 +313,22 @@ int32_t wfh_step(wfh_handle* h);
  * concurrently) race; serialise them. Asynchronous like wfh_step; wfh_sync reports an abort. */
 int32_t wfh_step_noalloc(wfh_handle* h);
 
+/* Graph mode (enable = 1): the fixed-sub-step step's kernel launches are recorded into a HIP
+ * graph once (per starting ticket parity) and every following wfh_step replays it. No
+ * counterpart in the reference. Forcing bound in node order (WFH_MEM_DEVICE_NODE_ORDER) is then
+ * copied into the library's buffers. Results are wfh_step's. Adaptive, floodplain and
+ * linked-weir models (one launch per sub-step) return WFH_ERR_ARG. */
+int32_t wfh_set_graph(wfh_handle* h, int32_t enable);
+/* Autotuner of the routing schedule (no counterpart in the reference): the next trials x 4 steps
+ * run the candidate persistent routing grids (all resident blocks, 3/4, 1/2, 1/4 of them) in
+ * turn, each timed with HIP events (those steps synchronise); the fastest is then kept. The steps
+ * are real steps of the model and every grid gives the same results. trials = 0 stops. */
+int32_t wfh_autotune(wfh_handle* h, int32_t trials);
+/* The routing grid in use and its maximum (blocks of 256 threads), whether the autotuner is
+ * still running and whether graph mode is on; any pointer may be NULL. */
+int32_t wfh_get_schedule(wfh_handle* h, int32_t* route_blocks, int32_t* max_blocks, int32_t* tuning,
+                         int32_t* graph_mode);
+
 /* Julian day (1..365; February 29 counts as March 1) that selects the rating-curve column of
  * reservoirs with outflow type 1 in the following steps; 1 after wfh_create. The reference sets
  * it from the step's start time before the river update: julian_day(clock.time - clock.dt).

@@ -2446,6 +2446,15 @@ int32_t wfo_set(wfo_model* m, int32_t var, const double* src, int64_t count) {
     case WFH_LAND_H: if (count != n) return WFH_ERR_ARG; memcpy(m->ol_h, src, 8 * (size_t)n); return 0;
     case WFH_RIVER_Q: if (count != nr) return WFH_ERR_ARG; memcpy(m->r_q, src, 8 * (size_t)nr); return 0;
     case WFH_RIVER_H: if (count != nr) return WFH_ERR_ARG; memcpy(m->r_h, src, 8 * (size_t)nr); return 0;
+    case WFH_RIVER_STORAGE: if (count != nr) return WFH_ERR_ARG; memcpy(m->r_storage, src, 8 * (size_t)nr); return 0;
+    case WFH_LAND_STORAGE: if (count != n) return WFH_ERR_ARG; memcpy(m->ol_storage, src, 8 * (size_t)n); return 0;
+    case WFH_USTOREDEPTH: /* with the total soil water storage it completes (after SATWATERDEPTH) */
+        if (count != n) return WFH_ERR_ARG;
+        for (int64_t i = 0; i < n; ++i) {
+            m->soil[i].ustoredepth = src[i];
+            m->soil[i].total_soil_water_storage = m->soil[i].satwaterdepth + src[i];
+        }
+        return 0;
     case WFH_FLOODPLAIN_STORAGE:
         if (count != nr || !m->fp) return WFH_ERR_ARG;
         memcpy(m->fp_storage, src, 8 * (size_t)nr);

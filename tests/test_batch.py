@@ -84,3 +84,52 @@ def test_batch_gpu_equals_member_oracles():
             refs[m].step(*synth.forcing_numpy(c, k, seed=7 * m))
     check_members(g, b, ms, refs)
     g.close()
+
+
+@pytest.mark.gpu
+def test_batch_64_catchments_real_sizes():
+    """The 64-catchment batch at its real sizes (log-uniform 1e4 .. 2e6 cells, seeded as in
+    bench.py --workload batch, 21.9 M cells, dx 1 km, hourly) in one model on the GPU, each
+    member against its own oracle run: the smallest, median and largest members over 24 steps,
+    every other member over the first 6 (members run in parallel threads)."""
+    import math
+    import sys
+    import time
+    from concurrent.futures import ThreadPoolExecutor
+
+    from wflowhip import Model
+
+    def say(msg):
+        sys.__stderr__.write(f"[batch 64] {msg}\n")
+        sys.__stderr__.flush()
+
+    t0 = time.time()
+    rng = np.random.default_rng(2)
+    sizes = np.exp(rng.uniform(np.log(1e4), np.log(2e6), 64))
+    ms = [synth.make_catchment(int(round(math.sqrt(s))), cell=1000.0, dt=3600.0, river_threshold=100, seed=1000 + m)
+          for m, s in enumerate(sizes)]
+    b = synth.make_batch(ms)
+    g = Model(b)
+    order = np.argsort([c.n for c in ms])
+    long_run = {int(order[0]), int(order[len(order) // 2]), int(order[-1])}
+    refs = [O.OracleModel(c) for c in ms]
+    say(f"{b.n} cells in 64 members ({ms[order[0]].n} .. {ms[order[-1]].n}); set up in {time.time() - t0:.0f}s")
+    pool = ThreadPoolExecutor(16)
+    for k in range(24):
+        active = [m for m in range(64) if k < 6 or m in long_run]
+        fs = list(pool.map(lambda m: synth.forcing_numpy(ms[m], k, seed=7 * m), range(64)))
+        g.run_timestep(*(np.ascontiguousarray(np.concatenate([f[i] for f in fs])) for i in range(3)))
+        list(pool.map(lambda m: refs[m].step(*fs[m]), active))
+        if k == 5 or k == 23:
+            checked = active
+            vals = {v: g.get(v) for v in CELL_VARS + RIVER_VARS}
+            for m in checked:
+                for v in CELL_VARS + RIVER_VARS:
+                    got = pick(b, m, v, vals[v])
+                    exp = refs[m].get(v)
+                    same = (got == exp) | (np.isnan(got) & np.isnan(exp))
+                    assert same.all(), (k, m, v.name)
+            say(f"step {k + 1}: members {sorted(checked) if len(checked) < 64 else 'all 64'} equal to their oracle "
+                f"runs ({time.time() - t0:.0f}s)")
+    pool.shutdown()
+    g.close()

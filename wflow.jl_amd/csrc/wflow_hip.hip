@@ -1935,6 +1935,16 @@ struct wfh_handle {
     bool forcing_set = false, broken = false;
     int parity = 0;
     int route_grid = 0;
+    int route_grid_max = 0;  // resident blocks of the routing kernel on this device
+    // graph mode: the fixed-sub-step step's launches recorded into a graph per starting ticket
+    // parity and replayed; the autotuner times candidate routing grids on real steps
+    bool graph_mode = false;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    std::vector<int> tune_grids;
+    std::vector<double> tune_ms;
+    int tune_next = 0, tune_left = 0;
+    hipEvent_t tune_ev[2] = {nullptr, nullptr};
     int nchunks = 0;
     int max_level = 0;
     int thin_level = 0;
@@ -3126,6 +3136,7 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         if (cus < 1) cus = 1;
         const int want = (h->nchunks + (ROUTE_BLOCK / WAVE) - 1) / (ROUTE_BLOCK / WAVE);
         h->route_grid = std::max(1, std::min(want, per_cu * cus));
+        h->route_grid_max = h->route_grid;
         if (adaptive || fp || linked) {
             // river positions: river cells sorted by node, a topological order of the river graph
             // because every river edge is a land edge and node order is topological
@@ -3458,6 +3469,13 @@ extern "C" int32_t wfh_set_forcing(wfh_handle* h, const double* P, const double*
         h->d.P = h->fP;
         h->d.E = h->fE;
         h->d.T = h->fT;
+    } else if (memkind == WFH_MEM_DEVICE_NODE_ORDER && h->graph_mode) {
+        HIPCHK(h, hipMemcpyAsync(h->fP, P, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->fE, E, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->fT, T, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        h->d.P = h->fP;
+        h->d.E = h->fE;
+        h->d.T = h->fT;
     } else if (memkind == WFH_MEM_DEVICE_NODE_ORDER) {
         h->d.P = P;
         h->d.E = E;
@@ -3483,17 +3501,118 @@ extern "C" int32_t wfh_step(wfh_handle* h) {
             default: return launch_adaptive<6>(h);
         }
     }
-    switch (h->N) {
-        case 2: launch_kernels<2>(h); break;
-        case 3: launch_kernels<3>(h); break;
-        case 4: launch_kernels<4>(h); break;
-        case 5: launch_kernels<5>(h); break;
-        default: launch_kernels<6>(h); break;
+    const bool tuning = h->tune_left > 0;
+    if (tuning) {
+        h->route_grid = h->tune_grids[h->tune_next];
+        HIPCHK(h, hipEventRecord(h->tune_ev[0], h->stream));
+    }
+    if (h->graph_mode && !tuning) {
+        const int p0 = h->parity;
+        if (!h->graph_exec[p0]) {
+            int p = p0;
+            HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            switch (h->N) {
+                case 2: launch_step<2>(h, h->d, h->ra, p, nullptr); break;
+                case 3: launch_step<3>(h, h->d, h->ra, p, nullptr); break;
+                case 4: launch_step<4>(h, h->d, h->ra, p, nullptr); break;
+                case 5: launch_step<5>(h, h->d, h->ra, p, nullptr); break;
+                default: launch_step<6>(h, h->d, h->ra, p, nullptr); break;
+            }
+            HIPCHK(h, hipStreamEndCapture(h->stream, &h->graph[p0]));
+            HIPCHK(h, hipGraphInstantiate(&h->graph_exec[p0], h->graph[p0], nullptr, nullptr, 0));
+        }
+        HIPCHK(h, hipGraphLaunch(h->graph_exec[p0], h->stream));
+        h->parity ^= h->d.snow_lat ? 0 : 1;  // as launch_step advances it (one or two dataflow launches)
+    } else {
+        switch (h->N) {
+            case 2: launch_kernels<2>(h); break;
+            case 3: launch_kernels<3>(h); break;
+            case 4: launch_kernels<4>(h); break;
+            case 5: launch_kernels<5>(h); break;
+            default: launch_kernels<6>(h); break;
+        }
     }
     HIPCHK(h, hipGetLastError());
+    if (tuning) {
+        HIPCHK(h, hipEventRecord(h->tune_ev[1], h->stream));
+        HIPCHK(h, hipEventSynchronize(h->tune_ev[1]));
+        float ms = 0.0f;
+        HIPCHK(h, hipEventElapsedTime(&ms, h->tune_ev[0], h->tune_ev[1]));
+        double& best = h->tune_ms[h->tune_next];
+        best = best < 0.0 ? ms : std::min(best, (double)ms);
+        h->tune_next = (h->tune_next + 1) % (int)h->tune_grids.size();
+        if (--h->tune_left == 0) {
+            int b = 0;
+            for (int k = 1; k < (int)h->tune_grids.size(); ++k)
+                if (h->tune_ms[k] < h->tune_ms[b]) b = k;
+            h->route_grid = h->tune_grids[b];
+        }
+    }
     h->nsub[0] = h->S_s;
     h->nsub[1] = h->S_l;
     h->nsub[2] = h->S_r;
+    return WFH_OK;
+}
+
+static void drop_graphs(wfh_handle* h) {
+    for (int k = 0; k < 2; ++k) {
+        if (h->graph_exec[k]) hipGraphExecDestroy(h->graph_exec[k]);
+        if (h->graph[k]) hipGraphDestroy(h->graph[k]);
+        h->graph_exec[k] = nullptr;
+        h->graph[k] = nullptr;
+    }
+}
+
+// Graph mode (fixed sub-steps): the step's launches are recorded once per starting ticket parity
+// and replayed by every following wfh_step. Forcing bound in node order is then copied into the
+// library's buffers (a graph keeps the pointers it was recorded with).
+extern "C" int32_t wfh_set_graph(wfh_handle* h, int32_t enable) {
+    if (!h) return WFH_ERR_ARG;
+    if (enable && (h->cfg.kinematic_wave_adaptive || h->d.fp || h->d.res_linked))
+        return fail(h, WFH_ERR_ARG, "graph mode covers fixed sub-steps only (no adaptive, floodplain or linked weirs)");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    drop_graphs(h);
+    h->graph_mode = enable != 0;
+    if (h->graph_mode && h->forcing_set && (h->d.P != h->fP || h->d.E != h->fE || h->d.T != h->fT))
+        return fail(h, WFH_ERR_STATE, "set the forcing again after enabling graph mode");
+    return WFH_OK;
+}
+
+// Autotuner of the routing kernel's persistent grid: the next `trials` x candidates steps run the
+// candidate grids (all resident blocks, 3/4, 1/2, 1/4) in turn, each step timed; the fastest is
+// kept. Every grid computes the same results (the schedule changes which wave takes which chunk,
+// not the operations). trials = 0 stops tuning and keeps the current grid.
+extern "C" int32_t wfh_autotune(wfh_handle* h, int32_t trials) {
+    if (!h) return WFH_ERR_ARG;
+    if (trials < 0) return fail(h, WFH_ERR_ARG, "trials must be >= 0");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (!h->tune_ev[0]) {
+        HIPCHK(h, hipEventCreate(&h->tune_ev[0]));
+        HIPCHK(h, hipEventCreate(&h->tune_ev[1]));
+    }
+    drop_graphs(h);
+    const int g = h->route_grid_max;
+    h->tune_grids.clear();
+    for (int f : {4, 3, 2, 1}) {
+        const int gg = std::max(1, g * f / 4);
+        if (std::find(h->tune_grids.begin(), h->tune_grids.end(), gg) == h->tune_grids.end()) h->tune_grids.push_back(gg);
+    }
+    h->tune_ms.assign(h->tune_grids.size(), -1.0);
+    h->tune_next = 0;
+    h->tune_left = trials * (int)h->tune_grids.size();
+    return WFH_OK;
+}
+
+// The routing grid in use (blocks of 256 threads) and whether the autotuner is still running.
+extern "C" int32_t wfh_get_schedule(wfh_handle* h, int32_t* route_blocks, int32_t* max_blocks, int32_t* tuning,
+                                    int32_t* graph_mode) {
+    if (!h) return WFH_ERR_ARG;
+    if (route_blocks) *route_blocks = h->route_grid;
+    if (max_blocks) *max_blocks = h->route_grid_max;
+    if (tuning) *tuning = h->tune_left > 0;
+    if (graph_mode) *graph_mode = h->graph_mode;
     return WFH_OK;
 }
 
@@ -3592,6 +3711,10 @@ extern "C" int32_t wfh_last_substeps(wfh_handle* h, int32_t* subsurface, int32_t
 extern "C" int32_t wfh_set_julian_day(wfh_handle* h, int32_t julian_day) {
     if (!h) return WFH_ERR_ARG;
     if (julian_day < 1 || julian_day > 365) return fail(h, WFH_ERR_ARG, "julian day outside 1..365");
+    if (h->d.res_hq_col != julian_day - 1) {
+        if (h->graph_mode) hipStreamSynchronize(h->stream);
+        drop_graphs(h);
+    }
     h->d.res_hq_col = julian_day - 1;
     return WFH_OK;
 }
@@ -3961,6 +4084,9 @@ extern "C" const char* wfh_last_error(const wfh_handle* h) {
 extern "C" void wfh_destroy(wfh_handle* h) {
     if (!h) return;
     if (h->stream) hipStreamSynchronize(h->stream);
+    drop_graphs(h);
+    for (hipEvent_t ev : h->tune_ev)
+        if (ev) hipEventDestroy(ev);
     for (void* p : h->allocs) hipFree(p);
     if (h->sel_host) hipHostFree(h->sel_host);
     for (hipEvent_t ev : h->ev)

@@ -41,6 +41,12 @@ def load_library():
     L.wfh_step.restype = i32
     L.wfh_step_noalloc.argtypes = [P]
     L.wfh_step_noalloc.restype = i32
+    L.wfh_set_graph.argtypes = [P, i32]
+    L.wfh_set_graph.restype = i32
+    L.wfh_autotune.argtypes = [P, i32]
+    L.wfh_autotune.restype = i32
+    L.wfh_get_schedule.argtypes = [P, P, P, P, P]
+    L.wfh_get_schedule.restype = i32
     L.wfh_get.argtypes = [P, i32, P, i64, i32]
     L.wfh_get.restype = i32
     L.wfh_set.argtypes = [P, i32, P, i64, i32]
@@ -141,6 +147,19 @@ class Model:
         """update_model! through wfh_step_noalloc: the routing scratch comes from the library's
         fixed per-device array, so the step allocates nothing (asynchronous like update_model)."""
         self._check(self.lib.wfh_step_noalloc(self.h))
+
+    def set_graph(self, enable=True):
+        """Replay the step's launches from a HIP graph (fixed sub-steps)."""
+        self._check(self.lib.wfh_set_graph(self.h, int(enable)))
+
+    def autotune(self, trials=1):
+        """Time the candidate routing grids on the next trials x 4 steps and keep the fastest."""
+        self._check(self.lib.wfh_autotune(self.h, int(trials)))
+
+    def schedule(self):
+        v = [ctypes.c_int32() for _ in range(4)]
+        self._check(self.lib.wfh_get_schedule(self.h, *(ctypes.addressof(x) for x in v)))
+        return dict(zip(("route_blocks", "max_blocks", "tuning", "graph_mode"), (x.value for x in v)))
 
     def run_timestep(self, precipitation, potential_evaporation, temperature):
         """run_timestep! with the step's forcing supplied by the caller."""
