@@ -68,6 +68,12 @@ WFH_FN double wfh_fabs(double x) { return wfh_from_bits(wfh_bits(x) & 0x7fffffff
 #define WFH_CEIL(x) ceil(x)
 #endif
 
+/* 1 when any of a, b, c is NaN or infinite (all exponent bits set) */
+WFH_FN int wfh_nonfinite3(double a, double b, double c) {
+    const uint64_t e = 0x7ff0000000000000ULL;
+    return ((wfh_bits(a) & e) == e) | ((wfh_bits(b) & e) == e) | ((wfh_bits(c) & e) == e);
+}
+
 /* 2^k for -1022 <= k <= 1023 */
 WFH_FN double wfh_pow2i(int k) { return wfh_from_bits((uint64_t)(k + 1023) << 52); }
 

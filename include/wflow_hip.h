@@ -42,7 +42,7 @@ enum wfh_status {
     WFH_ERR_ARG = -1,       /* invalid argument (null pointer, size, enum)            */
     WFH_ERR_HIP = -2,       /* a HIP runtime call failed                               */
     WFH_ERR_CYCLE = -3,     /* the drainage network has a cycle (routing/utils.jl:26) */
-    WFH_ERR_NONFINITE = -4, /* reserved: non-finite input rejected                     */
+    WFH_ERR_NONFINITE = -4, /* non-finite forcing in a step (reported by wfh_sync)       */
     WFH_ERR_DEVICE = -5,    /* a kernel reported a failure (dependency wait timed out) */
     WFH_ERR_STATE = -6      /* call out of order (e.g. wfh_step before wfh_set_forcing) */
 };
@@ -289,8 +289,16 @@ typedef struct wfh_handle wfh_handle;
 int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_handle** out);
 
 /* Forcing for the next step (precipitation [m s-1], potential evaporation [m s-1],
- * temperature [K]); memkind WFH_MEM_HOST copies, WFH_MEM_DEVICE binds caller device buffers
- * that must stay valid until the step has completed.
+ * temperature [K]). memkind WFH_MEM_HOST: host arrays in land-index order, uploaded (on a copy
+ * stream, overlapping the previous step) and permuted on the device; the call returns once the
+ * caller's arrays may be reused. WFH_MEM_DEVICE: device arrays in land-index order, gathered into
+ * the library's buffers on the handle's stream (they must be ready, and stay valid until the next
+ * step has started). WFH_MEM_DEVICE_NODE_ORDER: device arrays in the library's node order, bound
+ * without a copy: they are read by every following wfh_step until the next wfh_set_forcing, so
+ * they must stay alive and unmodified until then (in graph mode they are copied instead).
+ * A non-finite value (NaN or infinity) makes the step's results unusable: the vertical kernel
+ * flags it and wfh_sync (or wfh_get) returns WFH_ERR_NONFINITE; the handle then accepts no further
+ * step until wfh_load_state restores a saved state.
  * Replaces: update_forcing!/load_dynamic_input!  Wflow/src/io.jl:108-178  */
 int32_t wfh_set_forcing(wfh_handle* h, const double* precipitation, const double* potential_evaporation,
                         const double* temperature, int32_t memkind);

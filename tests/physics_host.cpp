@@ -10,3 +10,5 @@ extern "C" void wfh_host_kinematic_wave(const double* in, double* out, long n) {
         wfh::kinematic_wave(r[0], r[1], r[2], r[3], r[4], r[5], out[2 * i], out[2 * i + 1]);
     }
 }
+
+extern "C" int wfh_host_nonfinite3(double a, double b, double c) { return wfh_nonfinite3(a, b, c); }

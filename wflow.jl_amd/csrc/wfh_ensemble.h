@@ -499,6 +499,19 @@ __global__ void k_ens_init_riv(EnsDev e, EnsInit b) {
     const_cast<double*>(e.r_alpha)[idx] = at * b.wp_pow[r];
 }
 
+// flags a non-finite forcing value in the ensemble's own copy and replaces the cell's three
+// values by finite stand-ins, so that no NaN or infinity reaches the column's iterations (the
+// step's results are reported unusable)
+__global__ void k_ens_check_forcing(double* P, double* E, double* T, int n, int* flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && wfh_nonfinite3(P[i], E[i], T[i])) {
+        __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        P[i] = 0.0;
+        E[i] = 0.0;
+        T[i] = 283.15;
+    }
+}
+
 // one member's field of a member-innermost array, into land-index order
 __global__ void k_ens_take(double* out, const double* src, const int* node_of_index, int count, int M, int m) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -546,6 +559,7 @@ struct wfh_ensemble {
     double timed_ms = 0.0;
     int64_t timed_steps = 0;
     bool forcing_set = false;
+    bool broken = false;                      // non-finite forcing was stepped
     unsigned long long* hist_buf = nullptr;   // Newton update histogram (allocated when first enabled)
     unsigned long long* harvest_ctr = nullptr;
     int* d_index_node = nullptr;              // land index -> node (for per-member reads)
@@ -969,12 +983,16 @@ extern "C" int32_t wfh_ens_set_forcing(wfh_ensemble* x, const double* P, const d
     } else {
         return efail(x, WFH_ERR_ARG, "bad memkind");
     }
+    hipLaunchKernelGGL(k_ens_check_forcing, dim3((n + 255) / 256), dim3(256), 0, s, x->fP, x->fE, x->fT, n,
+                       x->base->d.nonfinite_flag);
+    EHIP(x, hipGetLastError());
     x->forcing_set = true;
     return WFH_OK;
 }
 
 extern "C" int32_t wfh_ens_step(wfh_ensemble* x) {
     if (!x) return WFH_ERR_ARG;
+    if (x->broken) return efail(x, WFH_ERR_STATE, "the ensemble had non-finite forcing; its states are not usable");
     if (!x->forcing_set) return efail(x, WFH_ERR_STATE, "wfh_ens_set_forcing must be called before wfh_ens_step");
     hipStream_t s = x->base->stream;
     EHIP(x, hipSetDevice(x->base->cfg.device));
@@ -1066,6 +1084,12 @@ extern "C" int32_t wfh_ens_get_schedule(wfh_ensemble* x, int32_t* block, int32_t
 extern "C" int32_t wfh_ens_sync(wfh_ensemble* x) {
     if (!x) return WFH_ERR_ARG;
     EHIP(x, hipStreamSynchronize(x->base->stream));
+    int flag = 0;
+    EHIP(x, hipMemcpy(&flag, x->base->d.nonfinite_flag, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) {
+        x->broken = true;
+        return efail(x, WFH_ERR_NONFINITE, "non-finite forcing (precipitation, potential evaporation or temperature)");
+    }
     return WFH_OK;
 }
 
@@ -1102,7 +1126,8 @@ extern "C" int32_t wfh_ens_get(wfh_ensemble* x, int32_t var, int32_t member, dou
     if (count != need) return efail(x, WFH_ERR_ARG, "count does not match the variable");
     hipStream_t s = x->base->stream;
     EHIP(x, hipSetDevice(x->base->cfg.device));
-    EHIP(x, hipStreamSynchronize(s));
+    int32_t rc = wfh_ens_sync(x);
+    if (rc) return rc;
     if (need == 0) return WFH_OK;
     double* tmp = nullptr;
     EHIP(x, hipMalloc(&tmp, need * sizeof(double)));

@@ -126,6 +126,7 @@ struct Dev {
     const double *res_area, *res_demand, *res_maxrel, *res_maxsto, *res_tff, *res_tmf, *res_thr, *res_rcc;
     const double* res_rce;     // [nres] free weir rating curve exponent
     int* abort_flag;           // the routing kernels' abort flag (RouteArgs::abort_flag)
+    int* nonfinite_flag;       // set by the vertical kernel when a cell's forcing is not finite
     const int* chain_land;     // [nchunks] 1: a chain chunk of the land lists (see run_chain)
     const int* chain_rl;       // [nchunks] 1: all 64 cells are river cells and a chain of the river lists
     const int* chain_sup;      // [nchunks] 1: a chain chunk of the snow transport's lists (sup_ptr)
@@ -213,7 +214,13 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
     // Every input of the cell is loaded here, in one block, unconditionally (parameters of
     // switched-off processes are allocated as zeros): loads spread over the branches below
     // would each be waited for at the branch's end, one memory round trip after another.
-    const double P_in = d.P[i], E_in = d.E[i], T = d.T[i];
+    const double P_raw = d.P[i], E_raw = d.E[i], T_raw = d.T[i];
+    // Non-finite forcing: flagged for wfh_sync, and the column runs on finite stand-ins instead
+    // (the step's results are reported unusable), so that no NaN or infinity reaches the
+    // column's iterations (the Brooks-Corey sub-steps are counted from the water amounts).
+    const bool bad = wfh_nonfinite3(P_raw, E_raw, T_raw);
+    if (bad) __hip_atomic_store(d.nonfinite_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double P_in = bad ? 0.0 : P_raw, E_in = bad ? 0.0 : E_raw, T = bad ? 283.15 : T_raw;
     const uint8_t fl = d.nres > 0 ? d.flags[i] : 0;
     ColState<N> s;
     s.canopy = d.canopy[i];
@@ -1933,6 +1940,7 @@ struct wfh_handle {
     size_t timed_steps = 0;
     bool timing = false;
     bool forcing_set = false, broken = false;
+    bool broken_nonfinite = false;  // broken by non-finite forcing only: a loaded state repairs it
     int parity = 0;
     int route_grid = 0;
     int route_grid_max = 0;  // resident blocks of the routing kernel on this device
@@ -3098,8 +3106,9 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         A.slot_r = dalloc<uint64_t>(h, (size_t)h->S_r * std::max<int64_t>(nriv, 1), e);
         if (e != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipMalloc slots"); goto done; }
         A.counters = dalloc<unsigned long long>(h, 2, e);
-        A.abort_flag = dalloc<int>(h, 1, e);
+        A.abort_flag = dalloc<int>(h, 2, e);
         h->d.abort_flag = A.abort_flag;
+        h->d.nonfinite_flag = A.abort_flag + 1;
         if (e != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipMalloc counters"); goto done; }
         A.S_s = h->S_s;
         A.S_l = h->S_l;
@@ -3117,7 +3126,7 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         if ((rc = launch_fill(h, A.slot_ol, SLOT_EMPTY, (size_t)h->S_l * n))) goto done;
         if ((rc = launch_fill(h, A.slot_r, SLOT_EMPTY, (size_t)h->S_r * std::max<int64_t>(nriv, 1)))) goto done;
         if ((rc = launch_fill(h, (uint64_t*)A.counters, 0, 2))) goto done;
-        if (hipMemsetAsync(A.abort_flag, 0, sizeof(int), h->stream) != hipSuccess) {
+        if (hipMemsetAsync(A.abort_flag, 0, 2 * sizeof(int), h->stream) != hipSuccess) {
             rc = fail(h, WFH_ERR_HIP, "hipMemsetAsync");
             goto done;
         }
@@ -3288,8 +3297,15 @@ constexpr int MAX_ADAPTIVE_SUBSTEPS = 1000000;  // per family and step: a guard,
 static int32_t read_sel(wfh_handle* h) {
     HIPCHK(h, hipMemcpyAsync(h->sel_host, h->sel, sizeof(SelCtl), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    int flag = 0;
-    HIPCHK(h, hipMemcpy(&flag, h->ra.abort_flag, sizeof(int), hipMemcpyDeviceToHost));
+    int flags[2] = {0, 0};
+    HIPCHK(h, hipMemcpy(flags, h->ra.abort_flag, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    if (flags[1]) {
+        h->broken = h->broken_nonfinite = true;
+        return fail(h, WFH_ERR_NONFINITE,
+                    "non-finite forcing (precipitation, potential evaporation or temperature) in a step; the "
+                    "states are not usable: load a saved state to continue");
+    }
+    const int flag = flags[0];
     if (flag) {
         h->broken = true;
         return fail(h, WFH_ERR_DEVICE, "routing kernel aborted: a dependency wait exceeded its spin limit");
@@ -3722,9 +3738,15 @@ extern "C" int32_t wfh_set_julian_day(wfh_handle* h, int32_t julian_day) {
 extern "C" int32_t wfh_sync(wfh_handle* h) {
     if (!h) return WFH_ERR_ARG;
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    int flag = 0;
-    HIPCHK(h, hipMemcpy(&flag, h->ra.abort_flag, sizeof(int), hipMemcpyDeviceToHost));
-    if (flag) {
+    int flag[2] = {0, 0};
+    HIPCHK(h, hipMemcpy(flag, h->ra.abort_flag, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    if (flag[1]) {
+        h->broken = h->broken_nonfinite = true;
+        return fail(h, WFH_ERR_NONFINITE,
+                    "non-finite forcing (precipitation, potential evaporation or temperature) in a step; the "
+                    "states are not usable: load a saved state to continue");
+    }
+    if (flag[0]) {
         h->broken = true;
         return fail(h, WFH_ERR_DEVICE, "routing kernel aborted: a dependency wait exceeded its spin limit");
     }
@@ -4047,6 +4069,11 @@ extern "C" int32_t wfh_load_state(wfh_handle* h, const void* buf, int64_t bytes)
     for (auto& s : state_blocks(h)) {
         if (s.second) HIPCHK(h, hipMemcpy(s.first, p, s.second * sizeof(double), hipMemcpyHostToDevice));
         p += s.second * sizeof(double);
+    }
+    if (h->broken && h->broken_nonfinite) {  // the routing itself completed: its scratch is clean
+        const int zero = 0;
+        HIPCHK(h, hipMemcpy(h->d.nonfinite_flag, &zero, sizeof(int), hipMemcpyHostToDevice));
+        h->broken = h->broken_nonfinite = false;
     }
     return WFH_OK;
 }
