@@ -336,6 +336,14 @@ int32_t wfh_autotune(wfh_handle* h, int32_t trials);
  * still running and whether graph mode is on; any pointer may be NULL. */
 int32_t wfh_get_schedule(wfh_handle* h, int32_t* route_blocks, int32_t* max_blocks, int32_t* tuning,
                          int32_t* graph_mode);
+/* Testing only (no counterpart in the reference): sets the routing kernel's spin limit (idle polls
+ * before a waiting wave raises the abort flag; 2000000 after wfh_create, at least 256) and, for
+ * node >= 0, redirects the first upstream edge of that cell (node order, wfh_get_node_order) to
+ * the outflow slot of a cell of another 64-cell chunk that publishes nothing, so that the cell's
+ * wait can never be met. The next step's routing kernel then ends by the abort and wfh_sync
+ * returns WFH_ERR_DEVICE; the handle is for destroying afterwards. For testing the bounded waits
+ * from the host; node < 0 only sets the limit. */
+int32_t wfh_test_route_stall(wfh_handle* h, int64_t spin_limit, int64_t node);
 
 /* Julian day (1..365; February 29 counts as March 1) that selects the rating-curve column of
  * reservoirs with outflow type 1 in the following steps; 1 after wfh_create. The reference sets

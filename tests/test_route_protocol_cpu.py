@@ -1,0 +1,258 @@
+"""CPU emulation of the routing kernel's dependency protocol (wave tickets, single-consumer
+outflow slots, in-wave exchange table with back-pressure, bounded idle polls and the abort flag),
+under randomised interleavings of the waves, a limited number of resident waves, and outflow
+stores that become visible late.
+
+What it shows about the protocol (not about the compiled kernel):
+  - with tickets handed out in chunk order, every run drains for any interleaving and any number
+    of resident waves down to one: no wave waits on a chunk that no running wave can complete;
+  - the results equal the sequential sweep bit for bit, and every slot is empty again at the end
+    (the next launch starts clean);
+  - a wait that can never be met (a redirected edge) ends by the spin limit: the abort flag is
+    raised and every wave leaves, each after at most 256 idle polls once the flag is up;
+  - the check is not vacuous: handing tickets out in reverse chunk order with one resident wave
+    deadlocks the protocol, and the emulation sees it end by the spin limit.
+
+One emulated round is one pass of the kernel's run_family loop: take the in-wave values that are
+ready, poll the cross-chunk slots, fold the ready prefix of each lane's upstream list in list
+order, then compute every lane whose inputs are complete (a lane that feeds a lane of its own
+chunk waits until that lane has taken its previous sub-step: the back-pressure of the exchange
+table, which holds one value per lane)."""
+import random
+
+import pytest
+
+EMPTY = object()
+
+
+class Net:
+    def __init__(self, n, seed, W, chain_from=None):
+        rng = random.Random(seed)
+        self.n, self.W = n, W
+        down = []
+        for v in range(n):
+            if v == n - 1 or rng.random() < 0.08:
+                down.append(-1)
+            elif chain_from is not None and v >= chain_from:
+                down.append(v + 1)
+            else:
+                down.append(min(n - 1, v + 1 + int(rng.expovariate(1.0 / (2 * W)))))
+        self.down = down
+        self.ups = [[] for _ in range(n)]
+        for v in range(n):
+            if down[v] >= 0:
+                self.ups[down[v]].append(v)  # ascending upstream index: the fold order
+        self.lat = [rng.uniform(0.0, 1.0) for _ in range(n)]
+
+
+def f(fam, q_old, acc, lat):
+    return 0.5 * q_old + 0.25 * acc + lat * (1.0 + fam)
+
+
+def sequential(net, S, fams=2):
+    q = [[0.0] * net.n for _ in range(fams)]
+    for fam in range(fams):
+        for k in range(S):
+            for v in range(net.n):
+                acc = 0.0
+                for u in net.ups[v]:
+                    acc = acc + q[fam][u]
+                lat = net.lat[v] + (q[fam - 1][v] if fam else 0.0)
+                q[fam][v] = f(fam, q[fam][v], acc, lat)
+    return q
+
+
+class Emu:
+    def __init__(self, net, S, R, G, seed, spin_limit=4000, fams=2, reverse_tickets=False, delay=3, orphan=None):
+        self.net, self.S, self.R, self.G, self.fams = net, S, R, G, fams
+        self.rng = random.Random(seed)
+        self.spin_limit, self.delay = spin_limit, delay
+        self.nchunks = (net.n + net.W - 1) // net.W
+        self.reverse = reverse_tickets
+        self.slots = [[[EMPTY] * net.n for _ in range(S)] for _ in range(fams)]
+        self.pending = []  # outflow stores not yet visible: (visible_at, fam, k, p, value)
+        self.counter = 0
+        self.abort = False
+        self.round = 0
+        self.q = [[0.0] * net.n for _ in range(fams)]
+        self.ups = [list(u) for u in net.ups]
+        if orphan is not None:  # (cell, cell of another chunk that publishes nothing)
+            v, src = orphan
+            self.ups[v][0] = src
+        self.idle_after_abort = []
+        self.computed = 0
+
+    def chunk_of(self, p):
+        return p // self.net.W
+
+    def publishes(self, p):
+        d = self.net.down[p]
+        return d >= 0 and self.chunk_of(d) != self.chunk_of(p)
+
+    def flush(self):
+        keep = []
+        for item in self.pending:
+            if item[0] <= self.round:
+                _, fam, k, p, val = item
+                assert self.slots[fam][k][p] is EMPTY  # single producer, consumed before reuse
+                self.slots[fam][k][p] = val
+            else:
+                keep.append(item)
+        self.pending = keep
+
+    def wave(self):
+        net, W, S = self.net, self.net.W, self.S
+        while True:
+            t = self.counter
+            self.counter += 1
+            if t >= self.nchunks:
+                return
+            c = self.nchunks - 1 - t if self.reverse else t
+            lanes = [p for p in range(c * W, min((c + 1) * W, net.n))]
+            lane_of = {p: i for i, p in enumerate(lanes)}
+            for fam in range(self.fams):
+                kk = [0] * len(lanes)
+                qv = [self.q[fam][p] for p in lanes]
+                buf = [dict() for _ in lanes]
+                nxt = [0] * len(lanes)
+                acc = [0.0] * len(lanes)
+                cons = [lane_of.get(net.down[p], -1) for p in lanes]
+                spin, seen_abort_at = 0, None
+                while any(k < S for k in kk):
+                    table = list(zip(kk, qv))
+                    got = False
+                    for i, p in enumerate(lanes):
+                        if kk[i] >= S:
+                            continue
+                        for j, u in enumerate(self.ups[p]):
+                            if j in buf[i] or j < nxt[i]:
+                                continue
+                            if self.chunk_of(u) == c:
+                                uk, uq = table[lane_of[u]]
+                                if uk == kk[i] + 1:
+                                    buf[i][j] = uq
+                            else:
+                                v = self.slots[fam][kk[i]][u]
+                                if v is not EMPTY:
+                                    self.slots[fam][kk[i]][u] = EMPTY
+                                    buf[i][j] = v
+                                    got = True
+                        while nxt[i] in buf[i]:
+                            acc[i] = acc[i] + buf[i].pop(nxt[i])
+                            nxt[i] += 1
+                    done_any = False
+                    for _ in range(len(lanes) * S):
+                        can = [kk[i] < S and nxt[i] == len(self.ups[p])
+                               and (S == 1 or cons[i] < 0 or kk[cons[i]] >= kk[i])
+                               for i, p in enumerate(lanes)]
+                        if not any(can):
+                            break
+                        done_any = True
+                        for i, p in enumerate(lanes):
+                            if can[i]:
+                                lat = net.lat[p] + (self.q[fam - 1][p] if fam else 0.0)
+                                qv[i] = f(fam, qv[i], acc[i], lat)
+                                self.computed += 1
+                                if self.publishes(p):
+                                    when = self.round + self.rng.randint(0, self.delay)
+                                    self.pending.append((when, fam, kk[i], p, qv[i]))
+                                kk[i] += 1
+                                nxt[i], acc[i] = 0, 0.0
+                        table = list(zip(kk, qv))
+                        for i, p in enumerate(lanes):
+                            if kk[i] >= S:
+                                continue
+                            for j, u in enumerate(self.ups[p]):
+                                if j in buf[i] or j < nxt[i] or self.chunk_of(u) != c:
+                                    continue
+                                uk, uq = table[lane_of[u]]
+                                if uk == kk[i] + 1:
+                                    buf[i][j] = uq
+                            while nxt[i] in buf[i]:
+                                acc[i] = acc[i] + buf[i].pop(nxt[i])
+                                nxt[i] += 1
+                    if done_any or got:
+                        spin = 0
+                    else:
+                        spin += 1
+                        if self.abort and seen_abort_at is None:
+                            seen_abort_at = spin
+                        if spin % 256 == 0 and self.abort:
+                            self.idle_after_abort.append(spin - (seen_abort_at or spin) + 1)
+                            return
+                        if spin > self.spin_limit:
+                            if self.abort:  # raised by another wave meanwhile
+                                self.idle_after_abort.append(spin - seen_abort_at + 1)
+                            self.abort = True
+                            return
+                    yield
+                for i, p in enumerate(lanes):
+                    self.q[fam][p] = qv[i]
+
+    def run(self, max_rounds=5_000_000):
+        waiting = list(range(self.G))
+        active = []
+        while waiting or active:
+            while waiting and len(active) < self.R:
+                waiting.pop(0)
+                active.append(self.wave())
+            g = self.rng.choice(active)
+            try:
+                next(g)
+            except StopIteration:
+                active.remove(g)
+            self.round += 1
+            self.flush()
+            assert self.round < max_rounds, "the emulated kernel did not end"
+        return not self.abort
+
+
+CASES = [(n, W, S, R, G, seed) for seed, (n, W, S, R, G) in enumerate([
+    (300, 8, 1, 1, 1), (300, 8, 1, 3, 7), (300, 8, 2, 2, 2), (300, 8, 3, 5, 11),
+    (500, 16, 2, 4, 4), (500, 4, 3, 8, 20), (257, 8, 4, 2, 5), (400, 8, 2, 16, 16)])]
+
+
+@pytest.mark.parametrize("n,W,S,R,G,seed", CASES)
+def test_protocol_drains_and_matches_the_sweep(n, W, S, R, G, seed):
+    net = Net(n, seed, W, chain_from=n // 2)
+    ref = sequential(net, S)
+    for sched in range(3):
+        e = Emu(net, S, R, G, seed=1000 * seed + sched)
+        assert e.run(), "aborted"
+        assert e.q == ref
+        assert e.computed == 2 * S * n
+        assert not e.pending
+        assert all(v is EMPTY for fam in e.slots for row in fam for v in row)
+
+
+def _orphan(net):
+    """A cell with an upstream cell in another chunk and a downstream cell in another chunk (whose
+    wave will then wait too), and a cell of another chunk that publishes nothing (its outflow
+    stays in its own chunk, or it has no downstream)."""
+    W = net.W
+    v = next(p for p in range(net.n) if net.ups[p] and net.ups[p][0] // W != p // W
+             and net.down[p] >= 0 and net.down[p] // W != p // W)
+    src = next(p for p in range(net.n) if p // W != v // W and
+               (net.down[p] < 0 or net.down[p] // W == p // W))
+    return v, src
+
+
+@pytest.mark.parametrize("R,G", [(1, 1), (3, 9), (8, 8)])
+def test_a_wait_that_is_never_met_ends_by_the_limit_and_every_wave_leaves(R, G):
+    net = Net(400, 7, 8)
+    e = Emu(net, 2, R, G, seed=R, spin_limit=3000, orphan=_orphan(net))
+    assert not e.run()
+    assert e.abort
+    # every wave that was idle when the flag went up left within 256 idle polls of seeing it
+    # (with one resident wave the stalled wave is the only one: its chunk's consumers never start)
+    assert max(e.idle_after_abort, default=0) <= 256
+    assert e.idle_after_abort or R == 1
+
+
+def test_out_of_order_tickets_deadlock_and_the_limit_ends_them():
+    net = Net(300, 3, 8)
+    ok = Emu(net, 1, 1, 1, seed=0, spin_limit=2000).run()
+    assert ok
+    e = Emu(net, 1, 1, 1, seed=0, spin_limit=2000, reverse_tickets=True)
+    assert not e.run()
+    assert e.abort

@@ -48,7 +48,10 @@ namespace {
 constexpr uint64_t SLOT_EMPTY = 0xFFF40000DEADBEEFULL;  // signalling NaN, never produced by arithmetic
 constexpr int WAVE = 64;
 constexpr int ROUTE_BLOCK = 256;
-constexpr long SPIN_LIMIT = 2000000;  // polls without progress (each with a sleep) before a wave gives up
+// Polls without progress (each with a sleep) before a wave gives up (RouteArgs::spin_limit). A
+// wave idle this long raises the abort flag; every other wave reads the flag after at most 256
+// idle polls of its own and leaves the kernel.
+constexpr long SPIN_LIMIT = 2000000;
 // true: the routing kernel performs each cell's vertical update itself at the start of the
 // cell's chunk (one launch per step) instead of a separate vertical kernel before it. Measured
 // on MI355X (4096^2 hourly and the 64-catchment batch) the separate kernel is faster: it runs
@@ -160,6 +163,7 @@ struct RouteArgs {
     uint64_t* slot_r;    // [S_r][nriv]
     unsigned long long* counters;  // [2] wave tickets, alternating per launch
     int* abort_flag;
+    long spin_limit;  // SPIN_LIMIT (wfh_test_route_stall lowers it)
     int parity;
     int nchunks;
     int S_s, S_l, S_r;
@@ -524,7 +528,7 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
             ++spin;
             if ((spin & 255) == 0 && __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                 return false;
-            if (spin > SPIN_LIMIT) {
+            if (spin > a.spin_limit) {
                 __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 return false;
             }
@@ -581,7 +585,7 @@ __device__ __forceinline__ bool run_chain(const RouteArgs& a, const int lane, co
         ++spin;
         if ((spin & 255) == 0 && __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             return false;
-        if (spin > SPIN_LIMIT) {
+        if (spin > a.spin_limit) {
             __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
         }
@@ -651,7 +655,7 @@ __device__ __forceinline__ bool run_chain_diag(const RouteArgs& a, const int lan
         ++spin;
         if ((spin & 255) == 0 && __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             return false;
-        if (spin > SPIN_LIMIT) {
+        if (spin > a.spin_limit) {
             __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
         }
@@ -723,7 +727,8 @@ __device__ __forceinline__ double reservoir_step(const Dev& d, const int ri, con
         double xfer;
         long spin = 0;
         while (!slot_take(d.res_xfer + ri, xfer)) {
-            if (++spin > SPIN_LIMIT) {
+            if (++spin > SPIN_LIMIT ||
+                ((spin & 255) == 0 && __hip_atomic_load(d.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                 __hip_atomic_store(d.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 return 0.0 / 0.0;
             }
@@ -3107,6 +3112,7 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         if (e != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipMalloc slots"); goto done; }
         A.counters = dalloc<unsigned long long>(h, 2, e);
         A.abort_flag = dalloc<int>(h, 2, e);
+        A.spin_limit = SPIN_LIMIT;
         h->d.abort_flag = A.abort_flag;
         h->d.nonfinite_flag = A.abort_flag + 1;
         if (e != hipSuccess) { rc = fail(h, WFH_ERR_HIP, "hipMalloc counters"); goto done; }
@@ -3629,6 +3635,38 @@ extern "C" int32_t wfh_get_schedule(wfh_handle* h, int32_t* route_blocks, int32_
     if (max_blocks) *max_blocks = h->route_grid_max;
     if (tuning) *tuning = h->tune_left > 0;
     if (graph_mode) *graph_mode = h->graph_mode;
+    return WFH_OK;
+}
+
+// Testing only: the routing kernel's spin limit, and optionally a wait that can never be met. The
+// first upstream edge of cell `node` (node order) is redirected to the outflow slot of a cell in
+// another chunk that publishes nothing, so the cell's wave idles until the limit, raises the abort
+// flag, and every other wave leaves the kernel; wfh_sync then reports WFH_ERR_DEVICE. The edge
+// stays redirected: the handle is for destroying afterwards.
+extern "C" int32_t wfh_test_route_stall(wfh_handle* h, int64_t spin_limit, int64_t node) {
+    if (!h) return WFH_ERR_ARG;
+    if (spin_limit < 256) return fail(h, WFH_ERR_ARG, "spin_limit must be >= 256");
+    if (node >= h->n) return fail(h, WFH_ERR_ARG, "node out of range");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    drop_graphs(h);
+    h->ra.spin_limit = spin_limit;
+    if (node < 0) return WFH_OK;
+    const int64_t n = h->n;
+    std::vector<int> up_ptr(n + 1), down(n);
+    std::vector<uint8_t> flags(n);
+    HIPCHK(h, hipMemcpy(up_ptr.data(), h->d.up_ptr, (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(down.data(), h->d.down, n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(flags.data(), h->d.flags, n * sizeof(uint8_t), hipMemcpyDeviceToHost));
+    const int e0 = up_ptr[node];
+    if (e0 == up_ptr[node + 1]) return fail(h, WFH_ERR_ARG, "the cell has no upstream cell");
+    int from = -1;  // a cell of another chunk whose land outflow goes to no slot
+    for (int64_t q = 0; q < n && from < 0; ++q) {
+        const bool cons_in = down[q] >= 0 && (down[q] / WAVE) == (int)(q / WAVE);
+        if (q / WAVE != node / WAVE && (!(flags[q] & 1) || cons_in)) from = (int)q;
+    }
+    if (from < 0) return fail(h, WFH_ERR_ARG, "no cell of another chunk leaves its slot empty");
+    HIPCHK(h, hipMemcpy(const_cast<int*>(h->d.up_idx) + e0, &from, sizeof(int), hipMemcpyHostToDevice));
     return WFH_OK;
 }
 

@@ -47,6 +47,8 @@ def load_library():
     L.wfh_autotune.restype = i32
     L.wfh_get_schedule.argtypes = [P, P, P, P, P]
     L.wfh_get_schedule.restype = i32
+    L.wfh_test_route_stall.argtypes = [P, i64, i64]
+    L.wfh_test_route_stall.restype = i32
     L.wfh_get.argtypes = [P, i32, P, i64, i32]
     L.wfh_get.restype = i32
     L.wfh_set.argtypes = [P, i32, P, i64, i32]
@@ -160,6 +162,11 @@ class Model:
         v = [ctypes.c_int32() for _ in range(4)]
         self._check(self.lib.wfh_get_schedule(self.h, *(ctypes.addressof(x) for x in v)))
         return dict(zip(("route_blocks", "max_blocks", "tuning", "graph_mode"), (x.value for x in v)))
+
+    def test_route_stall(self, spin_limit, node=-1):
+        """Testing only: the routing kernel's spin limit and, for node >= 0, a wait of that cell
+        (node order) that can never be met (wfh_test_route_stall)."""
+        self._check(self.lib.wfh_test_route_stall(self.h, int(spin_limit), int(node)))
 
     def run_timestep(self, precipitation, potential_evaporation, temperature):
         """run_timestep! with the step's forcing supplied by the caller."""
