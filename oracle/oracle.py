@@ -22,7 +22,8 @@ sys.path.insert(0, os.path.join(ROOT, "wflow.jl_amd"))
 
 from wflowhip import _abi  # noqa: E402
 
-LIB_PATH = os.path.join(HERE, "build", "libwflow_oracle.so")
+# WFO_LIB names another build of the same sources (the sanitizer build, oracle/Makefile)
+LIB_PATH = os.environ.get("WFO_LIB") or os.path.join(HERE, "build", "libwflow_oracle.so")
 _lib = None
 
 _c = ctypes.c_double
