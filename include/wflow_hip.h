@@ -336,6 +336,22 @@ int32_t wfh_autotune(wfh_handle* h, int32_t trials);
  * still running and whether graph mode is on; any pointer may be NULL. */
 int32_t wfh_get_schedule(wfh_handle* h, int32_t* route_blocks, int32_t* max_blocks, int32_t* tuning,
                          int32_t* graph_mode);
+/* nsteps steps of update_model! in one launch: the time loop of run! (Wflow/src/Wflow.jl:324-372)
+ * over the forcing frames P, E, T ([nsteps][n] device arrays in the library's node order,
+ * wfh_get_node_order; frame k is step k's forcing). The results equal nsteps calls of
+ * wfh_set_forcing + wfh_step bit for bit; on the device a chunk of cells starts its next step as
+ * soon as it and the chunks downstream of it have finished the previous one, so the steps
+ * overlap. river_q_avg, if not NULL, receives every step's river discharge average
+ * (WFH_RIVER_Q_AVERAGE, [nsteps][nriv] in river order). Asynchronous like wfh_step (errors,
+ * non-finite forcing included, at wfh_sync); the frames must stay valid until it has completed.
+ * Fixed sub-steps only, without reservoirs, lateral snow transport, the exponential-constant
+ * Ksat profile or a floodplain (else WFH_ERR_ARG).
+ * Currently switched off: returns WFH_ERR_STATE (its first device run did not finish and the
+ * cause is not established); call wfh_step per step instead.
+ * Replaces: the time loop of run!  Wflow/src/Wflow.jl:324-372 (run_timestep! per step) */
+int32_t wfh_run(wfh_handle* h, int32_t nsteps, const double* P, const double* E, const double* T,
+                double* river_q_avg);
+
 /* Testing only (no counterpart in the reference): sets the routing kernel's spin limit (idle polls
  * before a waiting wave raises the abort flag; 2000000 after wfh_create, at least 256) and, for
  * node >= 0, redirects the first upstream edge of that cell (node order, wfh_get_node_order) to

@@ -104,3 +104,11 @@ def test_struct_layout_matches_header(tmp_path):
         want.append(ctypes.sizeof(py))
         want += [getattr(py, f).offset for f, _ in py._fields_]
     assert got == want
+
+
+def test_run_rejects_a_null_handle():
+    """wfh_run's first argument check runs without a device: a null handle is WFH_ERR_ARG."""
+    from wflowhip.model import load_library
+
+    lib = load_library()
+    assert lib.wfh_run(None, 1, None, None, None, None) == -1

@@ -49,22 +49,28 @@ def f(fam, q_old, acc, lat):
     return 0.5 * q_old + 0.25 * acc + lat * (1.0 + fam)
 
 
-def sequential(net, S, fams=2):
+def sequential(net, S, fams=2, steps=1, lat_of_step=None):
+    lat_of_step = lat_of_step or (lambda step, v: net.lat[v])
     q = [[0.0] * net.n for _ in range(fams)]
-    for fam in range(fams):
-        for k in range(S):
-            for v in range(net.n):
-                acc = 0.0
-                for u in net.ups[v]:
-                    acc = acc + q[fam][u]
-                lat = net.lat[v] + (q[fam - 1][v] if fam else 0.0)
-                q[fam][v] = f(fam, q[fam][v], acc, lat)
+    for step in range(steps):
+        for fam in range(fams):
+            for k in range(S):
+                for v in range(net.n):
+                    acc = 0.0
+                    for u in net.ups[v]:
+                        acc = acc + q[fam][u]
+                    lat = lat_of_step(step, v) + (q[fam - 1][v] if fam else 0.0)
+                    q[fam][v] = f(fam, q[fam][v], acc, lat)
     return q
 
 
 class Emu:
-    def __init__(self, net, S, R, G, seed, spin_limit=4000, fams=2, reverse_tickets=False, delay=3, orphan=None):
+    def __init__(self, net, S, R, G, seed, spin_limit=4000, fams=2, reverse_tickets=False, delay=3, orphan=None,
+                 steps=1, lat_of_step=None):
         self.net, self.S, self.R, self.G, self.fams = net, S, R, G, fams
+        self.steps = steps
+        self.lat_of_step = lat_of_step or (lambda step, v: net.lat[v])
+        self.done = [0] * ((net.n + net.W - 1) // net.W)  # steps completed per chunk
         self.rng = random.Random(seed)
         self.spin_limit, self.delay = spin_limit, delay
         self.nchunks = (net.n + net.W - 1) // net.W
@@ -100,16 +106,32 @@ class Emu:
                 keep.append(item)
         self.pending = keep
 
-    def wave(self):
+    def wave(self, wid=0):
         net, W, S = self.net, self.net.W, self.S
         while True:
             t = self.counter
             self.counter += 1
-            if t >= self.nchunks:
+            if t >= self.nchunks * self.steps:
                 return
+            step, t = divmod(t, self.nchunks)
             c = self.nchunks - 1 - t if self.reverse else t
+            self.cur[wid] = c
             lanes = [p for p in range(c * W, min((c + 1) * W, net.n))]
             lane_of = {p: i for i, p in enumerate(lanes)}
+            # multi-step run: the chunk's previous step, and the previous step of every chunk it
+            # hands outflow to, must be complete (bounded wait, like every other)
+            need = {c}
+            if getattr(self, "done_check_downstream", True):
+                need |= {net.down[p] // W for p in lanes if net.down[p] >= 0}
+            spin = 0
+            while any(self.done[x] < step for x in need):
+                spin += 1
+                if spin % 256 == 0 and self.abort:
+                    return
+                if spin > self.spin_limit:
+                    self.abort = True
+                    return
+                yield
             for fam in range(self.fams):
                 kk = [0] * len(lanes)
                 qv = [self.q[fam][p] for p in lanes]
@@ -150,7 +172,7 @@ class Emu:
                         done_any = True
                         for i, p in enumerate(lanes):
                             if can[i]:
-                                lat = net.lat[p] + (self.q[fam - 1][p] if fam else 0.0)
+                                lat = self.lat_of_step(step, p) + (self.q[fam - 1][p] if fam else 0.0)
                                 qv[i] = f(fam, qv[i], acc[i], lat)
                                 self.computed += 1
                                 if self.publishes(p):
@@ -188,19 +210,23 @@ class Emu:
                     yield
                 for i, p in enumerate(lanes):
                     self.q[fam][p] = qv[i]
+            self.done[c] += 1
 
-    def run(self, max_rounds=5_000_000):
+    def run(self, max_rounds=5_000_000, starve=()):
+        """starve: chunks whose waves are scheduled rarely (one round in a hundred)."""
         waiting = list(range(self.G))
         active = []
+        self.cur = {}
         while waiting or active:
             while waiting and len(active) < self.R:
-                waiting.pop(0)
-                active.append(self.wave())
-            g = self.rng.choice(active)
+                wid = waiting.pop(0)
+                active.append((wid, self.wave(wid)))
+            weights = [0.01 if self.cur.get(wid) in starve else 1.0 for wid, _ in active]
+            item = self.rng.choices(active, weights)[0]
             try:
-                next(g)
+                next(item[1])
             except StopIteration:
-                active.remove(g)
+                active.remove(item)
             self.round += 1
             self.flush()
             assert self.round < max_rounds, "the emulated kernel did not end"
@@ -256,3 +282,40 @@ def test_out_of_order_tickets_deadlock_and_the_limit_ends_them():
     e = Emu(net, 1, 1, 1, seed=0, spin_limit=2000, reverse_tickets=True)
     assert not e.run()
     assert e.abort
+
+
+@pytest.mark.parametrize("R,G,S,seed", [(1, 1, 1, 0), (2, 5, 2, 1), (6, 6, 3, 2), (16, 16, 2, 3)])
+def test_multi_step_run_drains_and_matches_the_steps(R, G, S, seed):
+    """The multi-step run (tickets step after step; a chunk waits for its own previous step and
+    for the previous step of every chunk it hands outflow to) drains for any interleaving and
+    equals the steps run one after another; the slots (one set, reused every step) are empty at
+    the end and are never written while full."""
+    net = Net(300, 20 + seed, 8, chain_from=150)
+    lat = lambda step, v: net.lat[v] * (1.0 + 0.1 * step)  # noqa: E731
+    ref = sequential(net, S, steps=5, lat_of_step=lat)
+    for sched in range(2):
+        e = Emu(net, S, R, G, seed=100 * seed + sched, steps=5, lat_of_step=lat)
+        assert e.run(), "aborted"
+        assert e.q == ref
+        assert e.computed == 2 * S * net.n * 5
+        assert all(v is EMPTY for fam in e.slots for row in fam for v in row)
+        assert e.done == [5] * e.nchunks
+
+
+def test_multi_step_run_without_the_downstream_wait_overwrites_a_full_slot():
+    """Pins the downstream wait: without it a chunk can run a step ahead of its consumer and
+    write a slot that still holds the previous step's outflow (the emulation asserts on that)."""
+    net = Net(300, 31, 8, chain_from=150)
+    # a consumer chunk of a cross-chunk edge, starved by the scheduler
+    W = net.W
+    starve = sorted({net.down[p] // W for p in range(net.n) if net.down[p] >= 0 and net.down[p] // W != p // W})[:6]
+    for check in (True, False):
+        hits = 0
+        for seed in range(10):
+            e = Emu(net, 2, 8, 8, seed=seed, steps=6, delay=0)
+            e.done_check_downstream = check
+            try:
+                e.run(starve=set(starve))
+            except AssertionError:
+                hits += 1
+        assert (hits == 0) if check else (hits > 0), (check, hits)

@@ -71,6 +71,24 @@ struct PrmL {  // layered parameter map [N][n]
 __device__ __forceinline__ double ld(const Prm& a, int i) { return a.p[i & a.mask]; }
 __device__ __forceinline__ double ld(const PrmL& a, int k, int i) { return a.p[k * a.kstride + (i & a.mask)]; }
 
+// Accesses of the arrays a step writes. COH (the multi-step run, where a cell's next step may run
+// on another wave, CU or XCD while the kernel is still running) makes them agent-scope relaxed
+// atomics, which bypass the non-coherent caches, as the outflow slots do; otherwise plain.
+template <bool COH>
+__device__ __forceinline__ double ldv(const double* p) {
+    if constexpr (COH)
+        return wfh_from_bits(__hip_atomic_load((uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    else
+        return *p;
+}
+template <bool COH>
+__device__ __forceinline__ void stv(double* p, const double v) {
+    if constexpr (COH)
+        __hip_atomic_store((uint64_t*)p, wfh_bits(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        *p = v;
+}
+
 struct Dev {
     int n, nriv;
     // forcing (node order)
@@ -166,6 +184,13 @@ struct RouteArgs {
     long spin_limit;  // SPIN_LIMIT (wfh_test_route_stall lowers it)
     int parity;
     int nchunks;
+    // multi-step run (k_route<N, false, true>): steps of the launch, per chunk the number of its
+    // steps completed, the forcing frames of the steps ([nsteps][n], node order) and optionally
+    // each step's river discharge average ([nsteps][nriv], river order)
+    int nsteps;
+    int* chunk_done;
+    const double *run_P, *run_E, *run_T;
+    double* run_q_avg;
     int S_s, S_l, S_r;
     double dts_s[2], dts_l[2], dts_r[2];  // sub-step lengths: first (S-1) steps and the last one
 };
@@ -212,7 +237,7 @@ struct VertOut {
 
 // The vertical update of cell i (node order): reads the step's forcing and the cell's states,
 // writes every state and flux of the vertical part back, and returns what routing reads.
-template <int N>
+template <int N, bool COH = false>
 __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut<N>& o) {
     const int n = d.n;
     // Every input of the cell is loaded here, in one block, unconditionally (parameters of
@@ -227,21 +252,21 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
     const double P_in = bad ? 0.0 : P_raw, E_in = bad ? 0.0 : E_raw, T = bad ? 283.15 : T_raw;
     const uint8_t fl = d.nres > 0 ? d.flags[i] : 0;
     ColState<N> s;
-    s.canopy = d.canopy[i];
-    s.ss = d.snow_s[i];
-    s.sw = d.snow_w[i];
-    s.tsws = d.tsws[i];
+    s.canopy = ldv<COH>(d.canopy + i);
+    s.ss = ldv<COH>(d.snow_s + i);
+    s.sw = ldv<COH>(d.snow_w + i);
+    s.tsws = ldv<COH>(d.tsws + i);
     ColParams p;
     p.gap = ld(d.gap, i), p.smax = ld(d.smax, i), p.kc = ld(d.kc, i), p.er = ld(d.er, i);
     p.tti = ld(d.tti, i), p.tt = ld(d.tt, i), p.ttm = ld(d.ttm, i), p.cfmax = ld(d.cfmax, i);
     p.whc = ld(d.whc, i), p.wsoil = ld(d.wsoil, i), p.cfsoil = ld(d.cfsoil, i);
     const int r = d.river_of[i];
-    const double wd_land = d.ol_h[i];
-    const double r_h0 = d.r_h[r >= 0 ? r : 0];
+    const double wd_land = ldv<COH>(d.ol_h + i);
+    const double r_h0 = ldv<COH>(d.r_h + (r >= 0 ? r : 0));
     p.rf = ld(d.rf, i), p.wf = ld(d.wf, i), p.sfrac = ld(d.sfrac, i);
     p.ts = ld(d.ts, i), p.tr = ld(d.tr, i), p.tfc = ld(d.tfc, i), p.dsoil = ld(d.d, i);
     p.kv0 = ld(d.kv0, i), p.fk = ld(d.f, i), p.zexp = ld(d.zexp, i);
-    s.swd = d.swd[i], s.tsoil = d.tsoil[i];
+    s.swd = ldv<COH>(d.swd + i), s.tsoil = ldv<COH>(d.tsoil + i);
     p.compf = ld(d.compf, i), p.icap_s = ld(d.icap_s, i), p.icap_c = ld(d.icap_c, i);
     p.rd = ld(d.rd, i), p.h3h = ld(d.h3h, i), p.h3l = ld(d.h3l, i);
     p.h1 = ld(d.h1, i), p.h2 = ld(d.h2, i), p.h4 = ld(d.h4, i), p.ah1 = ld(d.ah1, i), p.hb = ld(d.hb, i);
@@ -252,7 +277,7 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
     ColLayers<N> L;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        s.us[k] = d.ustore[k * n + i];
+        s.us[k] = ldv<COH>(d.ustore + (k * n + i));
         L.kvf[k] = ld(d.kvfrac, k, i);
         L.cbc[k] = ld(d.c, k, i);
         L.rfr[k] = ld(d.rootfrac, k, i);
@@ -262,31 +287,31 @@ __device__ __forceinline__ void vertical_cell(const Dev& d, const int i, VertOut
     sbm_column<N>(P, E, T, wd_land, r >= 0 ? r_h0 : 0.0, p, L, cfg, s, f);
 
     // ---- write back ----
-    if (d.balance) d.store_prev[i] = f.store_prev;
-    d.canopy[i] = s.canopy;
+    if (d.balance) stv<COH>(d.store_prev + (i), f.store_prev);
+    stv<COH>(d.canopy + (i), s.canopy);
     if (d.snow) {
-        d.swe[i] = f.swe;
-        d.snow_s[i] = s.ss;
-        d.snow_w[i] = s.sw;
+        stv<COH>(d.swe + (i), f.swe);
+        stv<COH>(d.snow_s + (i), s.ss);
+        stv<COH>(d.snow_w + (i), s.sw);
     }
-    d.tsoil[i] = s.tsoil;
+    stv<COH>(d.tsoil + (i), s.tsoil);
 #pragma unroll
-    for (int k = 0; k < N; ++k) d.ustore[k * n + i] = s.us[k];
-    d.zi[i] = f.zi;
-    d.interception[i] = f.interception;
-    d.transfer[i] = f.transfer;
-    d.recharge[i] = f.recharge;
-    d.act_inf[i] = f.act_inf;
-    d.inf_excess[i] = f.inf_exc;
-    d.sat_excess[i] = f.sat_exc;
-    d.transpiration[i] = f.transp;
-    d.soil_evap[i] = f.sevap;
-    d.capflux[i] = f.capflux;
-    d.leakage[i] = f.leak;
-    d.aet[i] = f.aet;
-    d.net_runoff_river[i] = f.nrr;
-    d.runoff_land[i] = f.rl;
-    d.aeow_l[i] = f.aol;
+    for (int k = 0; k < N; ++k) stv<COH>(d.ustore + (k * n + i), s.us[k]);
+    stv<COH>(d.zi + (i), f.zi);
+    stv<COH>(d.interception + (i), f.interception);
+    stv<COH>(d.transfer + (i), f.transfer);
+    stv<COH>(d.recharge + (i), f.recharge);
+    stv<COH>(d.act_inf + (i), f.act_inf);
+    stv<COH>(d.inf_excess + (i), f.inf_exc);
+    stv<COH>(d.sat_excess + (i), f.sat_exc);
+    stv<COH>(d.transpiration + (i), f.transp);
+    stv<COH>(d.soil_evap + (i), f.sevap);
+    stv<COH>(d.capflux + (i), f.capflux);
+    stv<COH>(d.leakage + (i), f.leak);
+    stv<COH>(d.aet + (i), f.aet);
+    stv<COH>(d.net_runoff_river + (i), f.nrr);
+    stv<COH>(d.runoff_land + (i), f.rl);
+    stv<COH>(d.aeow_l + (i), f.aol);
 #pragma unroll
     for (int k = 0; k < N; ++k) o.us[k] = s.us[k];
     o.zi = f.zi;
@@ -535,6 +560,28 @@ __device__ __forceinline__ bool run_family(const RouteArgs& a, const int lane, c
         }
     }
     __builtin_amdgcn_wave_barrier();
+    return true;
+}
+
+// Multi-step run: waits until chunk `ca` and chunk `cb` (per lane) have completed `s` steps. Both
+// hold lower tickets than the waiting wave (earlier steps), so the wait ends; it is bounded like
+// every other wait, and the loads after it are issued only once it has ended.
+__device__ __forceinline__ bool wait_chunks(const RouteArgs& a, const int ca, const int cb, const int s) {
+    long spin = 0;
+    for (;;) {
+        const int va = __hip_atomic_load(a.chunk_done + ca, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int vb = __hip_atomic_load(a.chunk_done + cb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!__any(va < s || vb < s)) break;
+        __builtin_amdgcn_s_sleep(1);
+        ++spin;
+        if ((spin & 255) == 0 && __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return false;
+        if (spin > a.spin_limit) {
+            __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+    }
+    asm volatile("" ::: "memory");
     return true;
 }
 
@@ -906,7 +953,7 @@ struct SsfEnd {
 // average_flux_vars! (groundwater.jl:630-636), update_soil_water_storage! (soil.jl:1294-1392),
 // the subsurface and land balances (mass_balance.jl:215-253, 470-490) and the overland lateral
 // inflow (surface_kinwave.jl:740-766) of cell p after its last subsurface sub-step.
-template <int N>
+template <int N, bool COH = false>
 __device__ __forceinline__ SsfEnd ssf_finish(const Dev& d, const int p, const Column<N>& col, const SsfCellIn& c,
                                              const SsfCum& cum, const double sq, const double szi) {
     const int n = d.n;
@@ -919,11 +966,11 @@ __device__ __forceinline__ SsfEnd ssf_finish(const Dev& d, const int p, const Co
     const double s_qnet_avg = cum.qnet / dt;
     const double rech_avg = cum.rech / dt;
     o.s_tor_avg = cum.tor / dt;
-    d.ssf_q[p] = sq;
-    d.ssf_qin_avg[p] = s_qin_avg;
-    d.ssf_q_avg[p] = s_q_avg;
-    d.ssf_tor_avg[p] = o.s_tor_avg;
-    d.ssf_storage[p] = c.sy * (dsoil - szi) * area;
+    stv<COH>(d.ssf_q + (p), sq);
+    stv<COH>(d.ssf_qin_avg + (p), s_qin_avg);
+    stv<COH>(d.ssf_q_avg + (p), s_q_avg);
+    stv<COH>(d.ssf_tor_avg + (p), o.s_tor_avg);
+    stv<COH>(d.ssf_storage + (p), c.sy * (dsoil - szi) * area);
     double usd = 0.0;
     {
         bool first = true;
@@ -947,7 +994,7 @@ __device__ __forceinline__ SsfEnd ssf_finish(const Dev& d, const int p, const Co
             else
                 v = ts;
         }
-        d.vwc[k * n + p] = v;
+        stv<COH>(d.vwc + (k * n + p), v);
     }
     double rs_unsat = 0.0;
 #pragma unroll
@@ -956,25 +1003,25 @@ __device__ __forceinline__ SsfEnd ssf_finish(const Dev& d, const int p, const Co
     const double rs_sat = wfh_max(0.0, rd - col.zi) * (ts - tr);
     const double rzs = rs_sat + rs_unsat;
     const double swd = (dsoil - col.zi) * (ts - tr);
-    d.ustoredepth[p] = usd;
-    d.swd[p] = swd;
-    d.tsws[p] = swd + usd;
-    d.exfilt[p] = s_exf_avg;
-    d.runoff[p] = sbm_runoff;
-    d.rzs[p] = rzs;
-    d.vwc_root[p] = rzs / rd + tr;
+    stv<COH>(d.ustoredepth + (p), usd);
+    stv<COH>(d.swd + (p), swd);
+    stv<COH>(d.tsws + (p), swd + usd);
+    stv<COH>(d.exfilt + (p), s_exf_avg);
+    stv<COH>(d.runoff + (p), sbm_runoff);
+    stv<COH>(d.rzs + (p), rzs);
+    stv<COH>(d.vwc_root + (p), rzs / rd + tr);
     const double net_runoff = sbm_runoff - c.aeow;
-    d.net_runoff[p] = net_runoff;
-    d.zi[p] = col.zi;
+    stv<COH>(d.net_runoff + (p), net_runoff);
+    stv<COH>(d.zi + (p), col.zi);
 #pragma unroll
-    for (int k = 0; k < N; ++k) d.ustore[k * n + p] = col.ustore[k];
+    for (int k = 0; k < N; ++k) stv<COH>(d.ustore + (k * n + p), col.ustore[k]);
     if (d.balance) {
         double fin = 0.0, fout = 0.0;  // mass_balance.jl:470-490
         if (rech_avg > 0.0)
             fin += rech_avg;
         else
             fout -= rech_avg;
-        d.bal_ssf[p] = (s_qin_avg + fin) - (s_q_avg + fout) - s_qnet_avg;
+        stv<COH>(d.bal_ssf + (p), (s_qin_avg + fin) - (s_q_avg + fout) - s_qnet_avg);
         // land balance  mass_balance.jl:215-253
         const double t_in = s_qin_avg / area + (c.P_in + 0.0) + c.snow_in + 0.0 / area;
         const double vout = net_runoff + c.aet + c.nrr + c.leak + 0.0;
@@ -984,12 +1031,12 @@ __device__ __forceinline__ SsfEnd ssf_finish(const Dev& d, const int p, const Co
         const double rate = (storage - c.store_prev) / dt;
         const double err = t_in - t_out - rate;
         const double avg = (t_in + t_out) / 2.0;
-        d.bal_land[p] = err;
-        d.bal_land_rel[p] = avg == 0.0 ? 0.0 : err / avg;
+        stv<COH>(d.bal_land + (p), err);
+        stv<COH>(d.bal_land_rel + (p), avg == 0.0 ? 0.0 : err / avg);
     }
     // overland lateral inflow  surface_kinwave.jl:740-766
     o.o_inwater = (net_runoff + 0.0) * area + 0.0;
-    d.ol_inwater[p] = o.o_inwater;
+    stv<COH>(d.ol_inwater + (p), o.o_inwater);
     o.usd = usd;
     o.swd = swd;
     return o;
@@ -998,6 +1045,7 @@ __device__ __forceinline__ SsfEnd ssf_finish(const Dev& d, const int p, const Co
 // average_flow_vars! of the overland model, its balance (mass_balance.jl:382-399) and the river
 // lateral inflow of a river cell (surface_kinwave.jl:710-734) after cell p's last overland
 // sub-step; returns the river lateral inflow (0 for a cell without river).
+template <bool COH = false>
 __device__ __forceinline__ double ol_finish(const Dev& d, const int p, const int r, const double oq, const double oh,
                                             const double o_storage, const double oqin, const double o_q_cum,
                                             const double o_tor_cum, const double o_qin_cum, const double o_inwater,
@@ -1007,40 +1055,42 @@ __device__ __forceinline__ double ol_finish(const Dev& d, const int p, const int
     const double q_avg = o_q_cum / dt;
     const double o_tor_avg = o_tor_cum / dt;
     const double qin_avg = o_qin_cum / dt;
-    d.ol_q[p] = oq;
-    d.ol_h[p] = oh;
-    d.ol_storage[p] = o_storage;
-    d.ol_qin[p] = oqin;
-    d.ol_q_avg[p] = q_avg;
-    d.ol_tor_avg[p] = o_tor_avg;
-    d.ol_qin_avg[p] = qin_avg;
-    if (d.balance) d.bal_ol[p] = (o_inwater + qin_avg) - q_avg - (o_storage - o_store_prev) / dt;
+    stv<COH>(d.ol_q + (p), oq);
+    stv<COH>(d.ol_h + (p), oh);
+    stv<COH>(d.ol_storage + (p), o_storage);
+    stv<COH>(d.ol_qin + (p), oqin);
+    stv<COH>(d.ol_q_avg + (p), q_avg);
+    stv<COH>(d.ol_tor_avg + (p), o_tor_avg);
+    stv<COH>(d.ol_qin_avg + (p), qin_avg);
+    if (d.balance) stv<COH>(d.bal_ol + (p), (o_inwater + qin_avg) - q_avg - (o_storage - o_store_prev) / dt);
     double r_inwater = 0.0;
     if (r >= 0) {
         r_inwater = (s_tor_avg + o_tor_avg + nrr * area) + 0.0 * rarea;
-        d.r_inwater[r] = r_inwater;
+        stv<COH>(d.r_inwater + (r), r_inwater);
     }
     return r_inwater;
 }
 
 // average_flow_vars! of the river model and its balance (mass_balance.jl:290-313) after river
 // cell r's last sub-step.
+template <bool COH = false>
 __device__ __forceinline__ void riv_finish(const Dev& d, const int r, const double rq, const double rh,
                                            const double r_storage, const double rqin, const double r_q_cum,
                                            const double r_qin_cum, const double r_inwater, const double r_store_prev) {
     const double dt = d.dt;
     const double q_avg = r_q_cum / dt, qin_avg = r_qin_cum / dt;
-    d.r_q[r] = rq;
-    d.r_h[r] = rh;
-    d.r_storage[r] = r_storage;
-    d.r_qin[r] = rqin;
-    d.r_q_avg[r] = q_avg;
-    d.r_qin_avg[r] = qin_avg;
+    stv<COH>(d.r_q + (r), rq);
+    stv<COH>(d.r_h + (r), rh);
+    stv<COH>(d.r_storage + (r), r_storage);
+    stv<COH>(d.r_qin + (r), rqin);
+    stv<COH>(d.r_q_avg + (r), q_avg);
+    stv<COH>(d.r_qin_avg + (r), qin_avg);
     if (d.balance)
-        d.bal_r[r] = (r_inwater + qin_avg + wfh_max(0.0, 0.0)) - (q_avg + 0.0 + 0.0) - (r_storage - r_store_prev) / dt;
+        stv<COH>(d.bal_r + (r), (r_inwater + qin_avg + wfh_max(0.0, 0.0)) - (q_avg + 0.0 + 0.0) - (r_storage - r_store_prev) / dt);
 }
 
 // update_total_water_storage!  sbm.jl:143-182
+template <bool COH = false>
 __device__ __forceinline__ void total_storage_finish(const Dev& d, const int p, const bool riv, const double rh,
                                                      const double rfw, const double rfl, const double area,
                                                      const double snow_s, const double snow_w, const double canopy,
@@ -1053,12 +1103,18 @@ __device__ __forceinline__ void total_storage_finish(const Dev& d, const int p, 
     const double sub = usd + swd;
     const double lat = oh * (1.0 - rfrac);
     tot += sub + lat;
-    d.total_storage[p] = tot;
+    stv<COH>(d.total_storage + (p), tot);
 }
 
 // EXT: the model has reservoirs or lateral snow transport (a separate instantiation keeps their
 // code, and its registers, out of the kernel that runs without them).
-template <int N, bool EXT>
+// MULTI: a run of a.nsteps steps in one launch (no EXT). Ticket t is step t / nchunks of chunk
+// t % nchunks, so tickets go out step after step; a chunk starts a step once it has completed the
+// previous one and every chunk it hands outflow to has completed the previous one too (which
+// emptied the slots it fills). The wave then does the cell's vertical update of that step as
+// well, from the step's forcing frame, and the arrays the steps hand on are accessed coherently.
+// A chunk's step can thus run while the long chains of the previous step are still draining.
+template <int N, bool EXT, bool MULTI = false>
 __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
     __shared__ int x_k[ROUTE_BLOCK];     // per-wave exchange tables of the routing families
@@ -1068,12 +1124,14 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
     if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[a.parity ^ 1] = 0ull;  // next launch's counter
     const int n = d.n;
     const double dt = d.dt;
+    const unsigned long long tickets = (unsigned long long)a.nchunks * (MULTI ? (unsigned long long)a.nsteps : 1ull);
     for (;;) {
         unsigned long long t = 0;
         if (lane == 0) t = atomicAdd(&a.counters[a.parity], 1ull);
         t = __shfl(t, 0);
-        if (t >= (unsigned long long)a.nchunks) break;
-        const int c = (int)t;
+        if (t >= tickets) break;
+        const int c = MULTI ? (int)(t % (unsigned long long)a.nchunks) : (int)t;
+        const int step = MULTI ? (int)(t / (unsigned long long)a.nchunks) : 0;
         const int p = c * WAVE + lane;
         const bool valid = p < n;
         const int q_ = valid ? p : 0;  // clamped index for loads of inactive lanes
@@ -1089,6 +1147,9 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         const int e0 = e0_, e1 = valid ? e1_ : e0_;
         const uint8_t fl = valid ? fl_ : 0;
         const int r = valid ? r_ : -1;
+        if constexpr (MULTI) {
+            if (!wait_chunks(a, c, dn >= 0 ? (dn >> 6) : c, step)) return;
+        }
 
         // ---------------- lateral subsurface flow ----------------
         const double area = ld(d.area, q_);
@@ -1099,9 +1160,17 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         // The cell's vertical update (the first half of the step), either done here at the start
         // of the chunk or read back from the separate vertical kernel's outputs.
         VertOut<N> vo;
-        if (FUSE_VERTICAL) {
+        if (FUSE_VERTICAL || MULTI) {
             if (valid) {
-                vertical_cell<N>(d, p, vo);
+                if constexpr (MULTI) {
+                    Dev dv = d;
+                    dv.P = a.run_P + (size_t)step * n;
+                    dv.E = a.run_E + (size_t)step * n;
+                    dv.T = a.run_T + (size_t)step * n;
+                    vertical_cell<N, true>(dv, p, vo);
+                } else {
+                    vertical_cell<N>(d, p, vo);
+                }
             } else {
 #pragma unroll
                 for (int k = 0; k < N; ++k) vo.us[k] = 0.0;
@@ -1141,14 +1210,16 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         const double leak = vo.leak, store_prev = vo.store_prev;
         const double snow_s = vo.snow_s, snow_w = vo.snow_w;
         const double alpha = ld(d.alpha, q_), sfw = ld(d.sfw, q_), rfrac = ld(d.rf, q_);
-        const double oq0 = d.ol_q[q_], oh0 = d.ol_h[q_], o_store_prev = d.ol_storage[q_];
+        const double oq0 = ldv<MULTI>(d.ol_q + q_), oh0 = ldv<MULTI>(d.ol_h + q_),
+                     o_store_prev = ldv<MULTI>(d.ol_storage + q_);
         const bool riv = r >= 0;
         const int rr = riv ? r : 0;
         const int re0 = d.rup_ptr[rr], re1_ = d.rup_ptr[riv ? rr + 1 : rr], re1 = riv ? re1_ : re0;
         const double rfl = d.r_fl[rr], rfw = d.r_fw[rr], ralpha = d.r_alpha[rr], rarea = d.r_area[rr];
         const int rres_ = EXT ? d.res_of_r[rr] : -1;
         const int rres = riv ? rres_ : -1;
-        const double rq0 = d.r_q[rr], rh0 = d.r_h[rr], r_store_prev = d.r_storage[rr];
+        const double rq0 = ldv<MULTI>(d.r_q + rr), rh0 = ldv<MULTI>(d.r_h + rr),
+                     r_store_prev = ldv<MULTI>(d.r_storage + rr);
         Column<N> col;
         soil_layers<N>(dsoil, d.cfg_cum, d.cfg_thick, col.act, col.cum);
 #pragma unroll
@@ -1157,7 +1228,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         col.n_unsat = set_layerthickness<N>(col.zi, col.cum, col.act, col.thick);
         col.theta_e = ts - tr;
         col.theta_fc_r = tfc - tr;
-        double sq = d.ssf_q[q_], szi = col.zi;  // sbm_model.jl:80 hands the soil water table over
+        double sq = ldv<MULTI>(d.ssf_q + q_), szi = col.zi;  // sbm_model.jl:80 hands the soil water table over
         SsfCum cum{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         // Inflow-independent part of the first sub-step's solve, evaluated before the chain
         // reaches this cell; the closing soil-column update of the last sub-step is applied
@@ -1202,7 +1273,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
         // ---------------- average_flux_vars!, update_soil_water_storage! ----------------
         SsfEnd se{0.0, 0.0, 0.0, 0.0};
         if (valid)
-            se = ssf_finish<N>(d, p, col,
+            se = ssf_finish<N, MULTI>(d, p, col,
                                SsfCellIn{ts, tr, rd, dsoil, area, sy, sat_exc, runoff_l, inf_exc, aeow, nrr, P_in, aet,
                                          leak, canopy, snow_s, snow_w, store_prev, snow_in, snow_out},
                                cum, sq, szi);
@@ -1238,8 +1309,8 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             return;
         double r_inwater = 0.0;
         if (valid)
-            r_inwater = ol_finish(d, p, r, oq, oh, o_storage, oqin, o_q_cum, o_tor_cum, o_qin_cum, o_inwater,
-                                  o_store_prev, s_tor_avg, nrr, area, rarea);
+            r_inwater = ol_finish<MULTI>(d, p, r, oq, oh, o_storage, oqin, o_q_cum, o_tor_cum, o_qin_cum, o_inwater,
+                                         o_store_prev, s_tor_avg, nrr, area, rarea);
 
         // ---------------- river kinematic wave ----------------
         const bool rcons_in = riv && (fl & 4) && cons_in;  // a river cell drains to the land downstream
@@ -1274,10 +1345,19 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
                                                              riv && (fl & 4) && !rcons_in,
                                                              rcons_in ? (dn & 63) : -1, lds_k, lds_q, riv_compute))
             return;
-        if (riv) riv_finish(d, r, rq, rh, r_storage, rqin, r_q_cum, r_qin_cum, r_inwater, r_store_prev);
+        if (riv) riv_finish<MULTI>(d, r, rq, rh, r_storage, rqin, r_q_cum, r_qin_cum, r_inwater, r_store_prev);
+        if (MULTI && riv && a.run_q_avg) a.run_q_avg[(size_t)step * d.nriv + r] = r_q_cum / dt;
 
         // ---------------- update_total_water_storage!  sbm.jl:143-182 ----------------
-        if (valid) total_storage_finish(d, p, riv, rh, rfw, rfl, area, snow_s, snow_w, canopy, usd, swd, oh, rfrac);
+        if (valid)
+            total_storage_finish<MULTI>(d, p, riv, rh, rfw, rfl, area, snow_s, snow_w, canopy, usd, swd, oh, rfrac);
+        if constexpr (MULTI) {
+            // every store of the chunk's step has completed before the step is marked done
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+            asm volatile("" ::: "memory");
+            if (lane == 0) __hip_atomic_store(a.chunk_done + c, step + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 // ============================================================================================
@@ -1946,6 +2026,7 @@ struct wfh_handle {
     bool timing = false;
     bool forcing_set = false, broken = false;
     bool broken_nonfinite = false;  // broken by non-finite forcing only: a loaded state repairs it
+    int* run_chunk_done = nullptr;  // [nchunks] steps completed per chunk (wfh_run)
     int parity = 0;
     int route_grid = 0;
     int route_grid_max = 0;  // resident blocks of the routing kernel on this device
@@ -3635,6 +3716,76 @@ extern "C" int32_t wfh_get_schedule(wfh_handle* h, int32_t* route_blocks, int32_
     if (max_blocks) *max_blocks = h->route_grid_max;
     if (tuning) *tuning = h->tune_left > 0;
     if (graph_mode) *graph_mode = h->graph_mode;
+    return WFH_OK;
+}
+
+// ---- multi-step run (wfh_run) ----
+// __global__ fill of the per-chunk step counters (a kernel, not a memset, so that the run can be
+// recorded into a graph by a caller)
+__global__ void k_fill_i32(int* p, int v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <int N>
+static void launch_run(wfh_handle* h, RouteArgs a) {
+    a.parity = h->parity;
+    h->parity ^= 1;
+    hipLaunchKernelGGL((k_route<N, false, true>), dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, h->d, a);
+}
+
+// nsteps steps of update_model! in one launch (the time loop of run!, Wflow.jl:324-372, for the
+// forcing frames given): bitwise what nsteps calls of wfh_set_forcing + wfh_step give, with the
+// steps overlapping on the device. Fixed sub-steps, no reservoirs, lateral snow transport,
+// exponential-constant profile, floodplain or adaptive sub-steps.
+// Switched off: the first device run of this kernel (a 48 x 48 catchment, 12 steps) did not finish
+// within minutes, and its cause is not established (the ticket and wait protocol is deadlock-free
+// in its CPU emulation, and every wait is bounded). wfh_run reports WFH_ERR_STATE until the cause
+// is found; the kernel is kept for that work.
+constexpr bool RUN_ENABLED = false;
+
+extern "C" int32_t wfh_run(wfh_handle* h, int32_t nsteps, const double* P, const double* E, const double* T,
+                           double* river_q_avg) {
+    if (!h) return WFH_ERR_ARG;
+    if (!RUN_ENABLED)
+        return fail(h, WFH_ERR_STATE, "wfh_run is switched off: its device run did not finish and the cause is "
+                                      "not established; use wfh_step per step");
+    if (h->broken) return fail(h, WFH_ERR_STATE, "handle is in an error state (a previous step failed)");
+    if (nsteps < 1 || !P || !E || !T) return fail(h, WFH_ERR_ARG, "nsteps >= 1 and forcing frames required");
+    if (h->cfg.kinematic_wave_adaptive || h->d.fp || h->d.res_linked || h->ext)
+        return fail(h, WFH_ERR_ARG,
+                    "wfh_run covers fixed sub-steps without reservoirs, lateral snow transport, the "
+                    "exponential-constant profile or a floodplain");
+    if ((unsigned long long)nsteps * (unsigned long long)h->nchunks > (1ull << 40))
+        return fail(h, WFH_ERR_ARG, "too many steps for one run");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    if (!h->run_chunk_done) {
+        void* p = nullptr;
+        HIPCHK(h, hipMalloc(&p, (size_t)h->nchunks * sizeof(int)));
+        h->allocs.push_back(p);
+        h->run_chunk_done = (int*)p;
+    }
+    hipLaunchKernelGGL(k_fill_i32, dim3((h->nchunks + 255) / 256), dim3(256), 0, h->stream, h->run_chunk_done, 0,
+                       h->nchunks);
+    RouteArgs a = h->ra;
+    a.nsteps = nsteps;
+    a.chunk_done = h->run_chunk_done;
+    a.run_P = P;
+    a.run_E = E;
+    a.run_T = T;
+    a.run_q_avg = river_q_avg;
+    hipEvent_t* ev = timing_events(h);
+    if (ev) HIPCHK(h, hipEventRecord(ev[0], h->stream));
+    if (ev) HIPCHK(h, hipEventRecord(ev[1], h->stream));
+    switch (h->N) {
+        case 2: launch_run<2>(h, a); break;
+        case 3: launch_run<3>(h, a); break;
+        case 4: launch_run<4>(h, a); break;
+        case 5: launch_run<5>(h, a); break;
+        default: launch_run<6>(h, a); break;
+    }
+    HIPCHK(h, hipGetLastError());
+    if (ev) HIPCHK(h, hipEventRecord(ev[2], h->stream));
     return WFH_OK;
 }
 

@@ -184,7 +184,7 @@ EXPORTS = ["wfh_create", "wfh_set_forcing", "wfh_step", "wfh_get", "wfh_set", "w
            "wfh_state_size", "wfh_save_state", "wfh_load_state", "wfh_sync", "wfh_set_timing",
            "wfh_get_timing", "wfh_last_error", "wfh_destroy", "wfh_get_node_order",
            "wfh_kernel_bytes", "wfh_last_substeps", "wfh_set_julian_day", "wfh_step_noalloc", "wfh_set_graph", "wfh_autotune",
-           "wfh_get_schedule", "wfh_test_route_stall",
+           "wfh_get_schedule", "wfh_test_route_stall", "wfh_run",
            "wfh_ens_create", "wfh_ens_set_forcing", "wfh_ens_step", "wfh_ens_get", "wfh_ens_balance",
            "wfh_ens_set_schedule", "wfh_ens_set_autotune", "wfh_ens_get_schedule", "wfh_ens_sync",
            "wfh_ens_set_timing", "wfh_ens_get_timing", "wfh_ens_step_bytes", "wfh_ens_last_error",

@@ -49,6 +49,8 @@ def load_library():
     L.wfh_get_schedule.restype = i32
     L.wfh_test_route_stall.argtypes = [P, i64, i64]
     L.wfh_test_route_stall.restype = i32
+    L.wfh_run.argtypes = [P, i32, P, P, P, P]
+    L.wfh_run.restype = i32
     L.wfh_get.argtypes = [P, i32, P, i64, i32]
     L.wfh_get.restype = i32
     L.wfh_set.argtypes = [P, i32, P, i64, i32]
@@ -162,6 +164,24 @@ class Model:
         v = [ctypes.c_int32() for _ in range(4)]
         self._check(self.lib.wfh_get_schedule(self.h, *(ctypes.addressof(x) for x in v)))
         return dict(zip(("route_blocks", "max_blocks", "tuning", "graph_mode"), (x.value for x in v)))
+
+    def run(self, precipitation, potential_evaporation, temperature, river_q_avg=None):
+        """`nsteps` steps in one launch (wfh_run): forcing frames as device tensors [nsteps, n] in
+        node order (node_order()); river_q_avg, a device tensor [nsteps, nriv], receives each
+        step's river discharge average. Equal to nsteps run_timestep calls bit for bit."""
+        frames = [precipitation, potential_evaporation, temperature]
+        nsteps = int(frames[0].shape[0])
+        for f in frames:
+            if not (f.is_cuda and f.is_contiguous() and tuple(f.shape) == (nsteps, self.n)):
+                raise ValueError("forcing frames must be contiguous device tensors [nsteps, n] in node order")
+        out = 0
+        if river_q_avg is not None:
+            if not (river_q_avg.is_cuda and river_q_avg.is_contiguous() and
+                    tuple(river_q_avg.shape) == (nsteps, self.nriv)):
+                raise ValueError("river_q_avg must be a contiguous device tensor [nsteps, nriv]")
+            out = river_q_avg.data_ptr()
+        self._keep_run = frames + ([river_q_avg] if river_q_avg is not None else [])
+        self._check(self.lib.wfh_run(self.h, nsteps, *(f.data_ptr() for f in frames), out))
 
     def test_route_stall(self, spin_limit, node=-1):
         """Testing only: the routing kernel's spin limit and, for node >= 0, a wait of that cell
