@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--no-variants", action="store_true",
                     help="grid workload: skip the batched (ensemble) and batched+learned lines attached to it")
     ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--tune-trials", type=int, default=1,
+                    help="batch workload: routing-grid autotuner trials per candidate during the warm-up")
     ap.add_argument("--nx", type=int, default=4096)
     ap.add_argument("--frames", type=int, default=24, help="distinct forcing frames kept in HBM")
     ap.add_argument("--spinup", type=int, default=-1,
@@ -74,6 +76,8 @@ def parse():
                     help="adaptive kinematic-wave sub-steps (kinematic_wave__adaptive_time_step_flag) instead of "
                          "the fixed ones; not the headline line: no roofline, every sub-step waits for the last")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_routing_4096.json"))
+    ap.add_argument("--ens-traffic-file", default=os.path.join(ROOT, "profiles", "traffic_ensemble256_2048.json"),
+                    help="HBM bytes per ensemble step from counter runs (roofline.traffic of the ensemble)")
     ap.add_argument("--fp64-file", default=os.path.join(ROOT, "profiles", "fp64_4096.json"),
                     help="FP64 VALU FLOPs per launch from a counter run (SQ_INSTS_VALU_*_F64) of the same workload")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -296,6 +300,10 @@ def ensemble_run(args, t0, learned=False):
         "roofline": {"bound": "hbm", "kernel": "k_ens_stage (all stages of a step)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
     }
+    if os.path.exists(args.ens_traffic_file):
+        tf = json.load(open(args.ens_traffic_file))
+        if tf.get("members") == M and tf.get("n") == c.n:
+            result["roofline"]["traffic"] = tf["bytes_per_step"]
     ens.close()
     if learned:
         # plain and learned side by side over the same model hours: two ensembles of half the
@@ -349,6 +357,22 @@ def ensemble_run(args, t0, learned=False):
     return result, c, frames, node_t
 
 
+def calibration_kernel(args):
+    """One elementwise kernel of known traffic for scaling the HBM counters (--calibration-copy-mib)."""
+    if args.calibration_copy_mib <= 0:
+        return
+    import torch
+
+    nb = args.calibration_copy_mib << 20
+    src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    torch.neg(src, out=dst)  # one elementwise kernel: reads nb bytes, writes nb bytes
+    torch.cuda.synchronize()
+    print(f"# calibration kernel (neg): {nb} bytes read, {nb} bytes written", file=sys.stderr, flush=True)
+    del src, dst
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -364,6 +388,7 @@ def main():
     t0 = time.time()
     if args.workload == "ensemble":
         result, c, frames, node_t = ensemble_run(args, t0, learned=args.learned)
+        calibration_kernel(args)
         if not args.no_cpu_baseline:
             node = node_t.cpu().numpy()
             inv = np.empty_like(node)
@@ -438,8 +463,18 @@ def main():
             m.set_forcing(*frames[k % nframes], memkind=_abi.WFH_MEM_DEVICE_NODE_ORDER)
         m.update_model()
 
-    for k in range(args.warmup):
+    tuned = None
+    if args.workload == "batch" and plain_cfg and args.tune_trials > 0:
+        # the routing-grid autotuner runs on the first warm-up steps (real steps of the model)
+        m.autotune(args.tune_trials)
+    k = 0
+    while k < args.warmup or (tuned is None and m.schedule()["tuning"]):
         step(k)
+        k += 1
+    if args.workload == "batch" and plain_cfg and args.tune_trials > 0:
+        tuned = m.schedule()
+        print(f"# autotuned routing grid: {tuned} after {k} warm-up steps", file=sys.stderr, flush=True)
+    args.warmup = k
     m.sync()
     torch.cuda.synchronize()
     states = None
@@ -460,6 +495,21 @@ def main():
     elapsed = time.perf_counter() - t1
     k1_ms, k2_ms = m.timing()
     k1_bytes, k2_bytes = m.kernel_bytes()
+    m.set_timing(False)
+    graph_replay = None
+    if args.workload == "batch" and plain_cfg:
+        # the same steps replayed from the HIP graph of the step (graph mode), after the timed region
+        m.set_graph(True)
+        step(args.warmup + args.steps)  # records the graphs
+        step(args.warmup + args.steps + 1)
+        m.sync()
+        tg = time.perf_counter()
+        for k in range(args.warmup + args.steps + 2, args.warmup + 2 * args.steps + 2):
+            step(k)
+        m.sync()
+        graph_replay = {"ms_per_step": (time.perf_counter() - tg) / args.steps * 1e3, "schedule": m.schedule()}
+        m.set_graph(False)
+        print(f"# graph replay: {graph_replay['ms_per_step']:.3f} ms per step", file=sys.stderr, flush=True)
     bal = float(np.abs(m.get(Var.BALANCE_LAND_ERROR)).max())
     print(f"# {args.steps} steps in {elapsed:.3f}s: vertical {k1_ms:.3f} ms, routing {k2_ms:.3f} ms per step; "
           f"max |land balance error| {bal:.2e}", file=sys.stderr, flush=True)
@@ -502,6 +552,7 @@ def main():
                    "cells": c.n, "river_cells": c.nriv, "longest_flow_path": c.max_depth,
                    "kernel_ms": {"k_vertical": k1_ms, "k_route": k2_ms},
                    "kernel_bytes": {"k_vertical": k1_bytes, "k_route": k2_bytes}},
+        **({"autotuned_schedule": tuned, "graph_replay": graph_replay} if graph_replay else {}),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
     }
@@ -524,15 +575,7 @@ def main():
         del result["config"]["kernel_bytes"]
         result["config"]["substeps_mean"] = [float(np.mean([s[i] for s in subs])) for i in range(3)]
         result["config"]["substeps_max"] = [int(max(s[i] for s in subs)) for i in range(3)]
-    if args.calibration_copy_mib > 0:
-        nb = args.calibration_copy_mib << 20
-        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        torch.neg(src, out=dst)  # one elementwise kernel: reads nb bytes, writes nb bytes
-        torch.cuda.synchronize()
-        print(f"# calibration kernel (neg): {nb} bytes read, {nb} bytes written", file=sys.stderr, flush=True)
-        del src, dst
+    calibration_kernel(args)
     if not args.no_cpu_baseline:
         frames_host = []
         inv = np.empty_like(node)
