@@ -551,8 +551,8 @@ def main():
         "config": {"workload": workload,
                    "cells": c.n, "river_cells": c.nriv, "longest_flow_path": c.max_depth,
                    "kernel_ms": {"k_vertical": k1_ms, "k_route": k2_ms},
-                   "kernel_bytes": {"k_vertical": k1_bytes, "k_route": k2_bytes}},
-        **({"autotuned_schedule": tuned, "graph_replay": graph_replay} if graph_replay else {}),
+                   "kernel_bytes": {"k_vertical": k1_bytes, "k_route": k2_bytes},
+                   **({"autotuned_schedule": tuned, "graph_replay": graph_replay} if graph_replay else {})},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
     }

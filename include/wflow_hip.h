@@ -324,8 +324,9 @@ int32_t wfh_step_noalloc(wfh_handle* h);
 /* Graph mode (enable = 1): the fixed-sub-step step's kernel launches are recorded into a HIP
  * graph once (per starting ticket parity) and every following wfh_step replays it. No
  * counterpart in the reference. Forcing bound in node order (WFH_MEM_DEVICE_NODE_ORDER) is then
- * copied into the library's buffers. Results are wfh_step's. Adaptive, floodplain and
- * linked-weir models (one launch per sub-step) return WFH_ERR_ARG. */
+ * copied into the library's buffers; forcing bound that way before graph mode was enabled must
+ * be set again (wfh_step returns WFH_ERR_STATE until it is). Results are wfh_step's. Adaptive,
+ * floodplain and linked-weir models (one launch per sub-step) return WFH_ERR_ARG. */
 int32_t wfh_set_graph(wfh_handle* h, int32_t enable);
 /* Autotuner of the routing schedule (no counterpart in the reference): the next trials x 4 steps
  * run the candidate persistent routing grids (all resident blocks, 3/4, 1/2, 1/4 of them) in

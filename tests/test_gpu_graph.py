@@ -81,3 +81,36 @@ def test_graph_with_reservoirs_snow_transport_and_device_forcing():
             assert _same(gm.get(v), ref.get(v)), (k, v.name)
     for m in (ref, gm):
         m.close()
+
+
+def test_enabling_graph_mode_unbinds_node_order_forcing():
+    """Forcing bound in node order before graph mode is enabled is not what the graph reads: the
+    next step without new forcing is refused, and setting the forcing again continues the run,
+    equal to a run without graph mode."""
+    import torch
+
+    from wflowhip.model import WflowError
+
+    c = synth.make_catchment(40, cell=250.0, dt=3600.0, river_threshold=30, seed=606)
+    a, b = Model(c), Model(c)
+    node = torch.as_tensor(a.node_order(), device="cuda")
+    elev = torch.as_tensor(c.arrays["elevation"], device="cuda")[node].contiguous()
+    fr = [synth.forcing_torch(c, k, elev) for k in range(3)]
+    NODE = _abi.WFH_MEM_DEVICE_NODE_ORDER
+    for m in (a, b):
+        m.set_forcing(*fr[0], memkind=NODE)
+        m.update_model()
+    a.set_forcing(*fr[1], memkind=NODE)
+    a.set_graph(True)
+    with pytest.raises(WflowError) as e:
+        a.update_model()
+    assert e.value.code == -6
+    for k in (1, 2):
+        a.set_forcing(*fr[k], memkind=NODE)
+        a.update_model()
+        b.set_forcing(*fr[k], memkind=NODE)
+        b.update_model()
+    for v in STATE_VARS:
+        assert np.array_equal(a.get(v).view(np.uint64), b.get(v).view(np.uint64)), v.name
+    a.close()
+    b.close()

@@ -3677,8 +3677,9 @@ extern "C" int32_t wfh_set_graph(wfh_handle* h, int32_t enable) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
     drop_graphs(h);
     h->graph_mode = enable != 0;
+    // forcing bound in node order is not what a graph reads (its own buffers): it must be set again
     if (h->graph_mode && h->forcing_set && (h->d.P != h->fP || h->d.E != h->fE || h->d.T != h->fT))
-        return fail(h, WFH_ERR_STATE, "set the forcing again after enabling graph mode");
+        h->forcing_set = false;
     return WFH_OK;
 }
 
