@@ -235,13 +235,19 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
         const ColConfig cfg{dt, e.gash != 0, e.snow != 0, e.infred != 0, e.balance != 0, e.cfg_cum, e.cfg_thick};
         sbm_column<N>(P, E, T, oh0, riv ? rh0 : 0.0, p, L, cfg, s, cf);
     }
-    if (MODE == ENS_COLUMN) {
+    // the column's states are written now, so that they do not stay live through the routing
+    // (the balance below needs only their sum)
+    if (MODE != ENS_ROUTE) {
         e.canopy[i] = s.canopy;
         if (e.snow) {
             e.snow_s[i] = s.ss;
             e.snow_w[i] = s.sw;
         }
         e.tsoil[i] = s.tsoil;
+    }
+    const double snow_bal = e.snow ? s.ss + s.sw : 0.0 + 0.0;
+    const double canopy_bal = s.canopy;
+    if (MODE == ENS_COLUMN) {
 #pragma unroll
         for (int k = 0; k < N; ++k) e.ustore[k * nM + i] = s.us[k];
         double* q = e.pre + (size_t)e.pre_slot[v] * M + m;
@@ -263,10 +269,12 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
     const double rq0 = e.r_q[ri], r_store_prev = e.r_storage[ri], ralpha = e.r_alpha[ri];
     const double area = ens_ld(e.area, v), flen = ens_ld(e.fl, v), fw = ens_ld(e.fw, v), slope = ens_ld(e.slope, v);
     const double sy = ens_ld(e.sy, v), sfw = ens_ld(e.sfw, v);
-    const double kh0 = ens_ld(e.khfrac, v) * p.kv0;
+    // the column's parameters are read again (the same values) rather than kept live through it
+    const double kv0 = ens_ld(e.kv0, v) * e.fa[m], fk = ens_ld(e.f, v) * e.fb[m];
+    const double kh0 = ens_ld(e.khfrac, v) * kv0;
 
     // ---- lateral subsurface flow (one sub-step of length dt) ----
-    const double ts = p.ts, tr = p.tr, dsoil = p.dsoil;
+    const double ts = ens_ld(e.ts, v), tr = ens_ld(e.tr, v), dsoil = ens_ld(e.d, v), tfc = ens_ld(e.tfc, v);
     Column<N> col;
     soil_layers<N>(dsoil, e.cfg_cum, e.cfg_thick, col.act, col.cum);
 #pragma unroll
@@ -274,7 +282,7 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
     col.zi = cf.zi;
     col.n_unsat = set_layerthickness<N>(col.zi, col.cum, col.act, col.thick);
     col.theta_e = ts - tr;
-    col.theta_fc_r = p.tfc - tr;
+    col.theta_fc_r = tfc - tr;
     const int e0 = e.up_ptr[v], e1 = e.up_ptr[v + 1];
     double sq, szi;
     SsfCum cum{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -288,7 +296,7 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
             tor = tor + q * fu;
         }
         const double dts = dt;
-        const SsfPrep pp = ssf_prepare(sq0, col.zi, slope, sy, dts, flen, kh0, p.fk, INFINITY);
+        const SsfPrep pp = ssf_prepare(sq0, col.zi, slope, sy, dts, flen, kh0, fk, INFINITY);
         double flx = cf.recharge * area;  // flux!(::RechargeModel) with check_flux
         if (col.zi >= dsoil) flx = wfh_max(0.0, flx);
         cum.rech += flx * dts;
@@ -296,7 +304,7 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
         cum.tor += tor * dts;
         double qn, zin, exf, net;
         const double z0 = col.zi;
-        if (kinematic_wave_ssf<N>(qin, sq0, z0, qnb, slope, sy, dsoil, dts, flen, fw, qmax, kh0, p.fk, INFINITY, pp,
+        if (kinematic_wave_ssf<N>(qin, sq0, z0, qnb, slope, sy, dsoil, dts, flen, fw, qmax, kh0, fk, INFINITY, pp,
                                   col, true, qn, zin, exf, net))
             update_ustorelayerdepth<N>(col, z0, zin);
         sq = qn;
@@ -328,21 +336,12 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
     for (int k = 0; k < N; ++k) e.ustore[k * nM + i] = col.ustore[k];
     e.swd[i] = swd;
     e.tsws[i] = swd + usd;
-    if (MODE == ENS_FUSED) {
-        e.canopy[i] = s.canopy;
-        if (e.snow) {
-            e.snow_s[i] = s.ss;
-            e.snow_w[i] = s.sw;
-        }
-        e.tsoil[i] = s.tsoil;
-    }
     double bal = 0.0;
     if (e.balance) {  // land balance  mass_balance.jl:215-253
         const double t_in = s_qin_avg / area + (cf.P + 0.0) + 0.0 + 0.0 / area;
         const double vout = net_runoff + cf.aet + cf.nrr + cf.leak + 0.0;
         const double t_out = s_q_avg / area + vout + 0.0 + 0.0 / area;
-        const double snow = e.snow ? s.ss + s.sw : 0.0 + 0.0;
-        const double storage = (swd + usd) + s.canopy + snow + 0.0 * 0.0 + 0.0;
+        const double storage = (swd + usd) + canopy_bal + snow_bal + 0.0 * 0.0 + 0.0;
         const double rate = (storage - cf.store_prev) / dt;
         bal = wfh_fabs(t_in - t_out - rate);
     }
@@ -368,8 +367,8 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
         double qn, ar;
         ens_kw<VAR>(e, W, 0, (unsigned)v * 2654435761u ^ (unsigned)m * 40503u, qin, op, alpha, dts, flen, slope, qn,
                       ar, hist);
-        double oh = oh0;
-        if (sfw > 0.0) oh = ar / sfw;
+        // (without a surface flow width the depth is unchanged: read again rather than kept live)
+        const double oh = sfw > 0.0 ? ar / sfw : e.ol_h[i];
         e.ol_q[i] = qn;
         e.ol_h[i] = oh;
         o_tor_avg = o_tor_cum / dt;
@@ -385,7 +384,8 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
         const double rqlat_all = rqlat + inflow;
         const int re0 = e.rup_ptr[r], re1 = e.rup_ptr[r + 1];
         const size_t rM = (size_t)e.nriv * M;
-        double rq = rq0, rh = rh0, r_storage = r_store_prev, r_q_cum = 0.0;
+        // every step has at least one river sub-step, which sets h and the storage
+        double rq = rq0, rh = 0.0, r_storage = 0.0, r_q_cum = 0.0;
         for (int k = 0; k < e.S_r; ++k) {
             const double dts = (k == e.S_r - 1) ? e.dts_r[1] : e.dts_r[0];
             const double* qk = (k == e.S_r - 1) ? e.r_q : e.rq_sub + (size_t)k * rM;
