@@ -77,6 +77,15 @@ def test_ensemble_hourly_matches_oracle_per_member():
     ens.close()
 
 
+def test_ensemble_six_river_substeps():
+    """Six river sub-steps per step (river_dt 600 s): the chain stages read the inflow of the first
+    four sub-steps ahead and of the others in the sub-step loop."""
+    _say("ensemble 40x40 hourly, river_dt 600 s (6 river sub-steps), 64 members, 4 steps")
+    c = synth.make_catchment(40, cell=250.0, dt=3600.0, river_threshold=30, seed=403)
+    ens, _ = run_against_oracle(c, 64, 4, [0, 45], river_dt=600.0)
+    ens.close()
+
+
 def test_ensemble_daily_two_groups():
     _say("ensemble 40x40 daily (land_dt = dt), 128 members, 5 steps")
     c = synth.make_catchment(40, cell=1000.0, dt=86400.0, river_threshold=30, seed=402)

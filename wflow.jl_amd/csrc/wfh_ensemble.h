@@ -284,16 +284,49 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
     col.theta_e = ts - tr;
     col.theta_fc_r = tfc - tr;
     const int e0 = e.up_ptr[v], e1 = e.up_ptr[v + 1];
+    // Upstream inflow of every family. The upstream cells are complete (earlier stages or earlier
+    // in this strand), so on a chain (MODE ROUTE) all of it is read here, in one round trip,
+    // instead of one round trip before each solve; each sum is the same left fold in ascending
+    // upstream id either way.
+    double up_s_qin = 0.0, up_s_tor = 0.0, up_o_tor = 0.0, up_o_qin = 0.0;
+    constexpr int PRE_SR = 4;  // river sub-steps whose inflow is read ahead (registers)
+    double up_r[PRE_SR];
+    const int r_e0 = riv ? e.rup_ptr[r] : 0, r_e1 = riv ? e.rup_ptr[r + 1] : 0;
+    const size_t rM = (size_t)e.nriv * M;
+    if (MODE == ENS_ROUTE) {
+        for (int j = e0; j < e1; ++j) {
+            const int u = e.up_idx[j];
+            const double fu = e.up_f[j];
+            const double qs = e.ssf_q[(size_t)u * M + m], qo = e.ol_q[(size_t)u * M + m];
+            up_s_qin = up_s_qin + qs * (1.0 - fu);  // lateral_subsurface_flow.jl:228-240
+            up_s_tor = up_s_tor + qs * fu;
+            up_o_tor = up_o_tor + qo * fu;          // surface_kinwave.jl:317-326
+            up_o_qin = up_o_qin + qo * (1.0 - fu);
+        }
+#pragma unroll
+        for (int k = 0; k < PRE_SR; ++k) up_r[k] = 0.0;
+        for (int j = r_e0; j < r_e1; ++j) {
+            const size_t ui = (size_t)e.rup_idx[j] * M + m;
+#pragma unroll
+            for (int k = 0; k < PRE_SR; ++k)
+                if (k < e.S_r) up_r[k] = up_r[k] + (k == e.S_r - 1 ? e.r_q : e.rq_sub + (size_t)k * rM)[ui];
+        }
+    }
     double sq, szi;
     SsfCum cum{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     {
         double qin = 0.0, tor = 0.0;  // fold in ascending upstream id  lateral_subsurface_flow.jl:228-240
-        for (int j = e0; j < e1; ++j) {
-            const int u = e.up_idx[j];
-            const double fu = e.up_f[j];
-            const double q = e.ssf_q[(size_t)u * M + m];
-            qin = qin + q * (1.0 - fu);
-            tor = tor + q * fu;
+        if (MODE == ENS_ROUTE) {
+            qin = up_s_qin;
+            tor = up_s_tor;
+        } else {
+            for (int j = e0; j < e1; ++j) {
+                const int u = e.up_idx[j];
+                const double fu = e.up_f[j];
+                const double q = e.ssf_q[(size_t)u * M + m];
+                qin = qin + q * (1.0 - fu);
+                tor = tor + q * fu;
+            }
         }
         const double dts = dt;
         const SsfPrep pp = ssf_prepare(sq0, col.zi, slope, sy, dts, flen, kh0, fk, INFINITY);
@@ -351,12 +384,17 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
     double o_tor_avg;
     {
         double tor = 0.0, qin_all = 0.0;  // surface_kinwave.jl:317-326
-        for (int j = e0; j < e1; ++j) {
-            const int u = e.up_idx[j];
-            const double fu = e.up_f[j];
-            const double q = e.ol_q[(size_t)u * M + m];
-            tor = tor + q * fu;
-            qin_all = qin_all + q * (1.0 - fu);
+        if (MODE == ENS_ROUTE) {
+            tor = up_o_tor;
+            qin_all = up_o_qin;
+        } else {
+            for (int j = e0; j < e1; ++j) {
+                const int u = e.up_idx[j];
+                const double fu = e.up_f[j];
+                const double q = e.ol_q[(size_t)u * M + m];
+                tor = tor + q * fu;
+                qin_all = qin_all + q * (1.0 - fu);
+            }
         }
         const double dts = dt;
         const double oqlat = o_inwater / flen;
@@ -382,15 +420,19 @@ __device__ __forceinline__ double ens_cell(const EnsDev& e, const int v, const i
         double inflow = 0.0 / rfl;
         inflow -= 0.0 / rfl;
         const double rqlat_all = rqlat + inflow;
-        const int re0 = e.rup_ptr[r], re1 = e.rup_ptr[r + 1];
-        const size_t rM = (size_t)e.nriv * M;
         // every step has at least one river sub-step, which sets h and the storage
         double rq = rq0, rh = 0.0, r_storage = 0.0, r_q_cum = 0.0;
         for (int k = 0; k < e.S_r; ++k) {
             const double dts = (k == e.S_r - 1) ? e.dts_r[1] : e.dts_r[0];
-            const double* qk = (k == e.S_r - 1) ? e.r_q : e.rq_sub + (size_t)k * rM;
             double sumq = 0.0;  // surface_kinwave.jl:520
-            for (int j = re0; j < re1; ++j) sumq = sumq + qk[(size_t)e.rup_idx[j] * M + m];
+            if (MODE == ENS_ROUTE && k < PRE_SR) {
+#pragma unroll
+                for (int kk = 0; kk < PRE_SR; ++kk)
+                    if (kk == k) sumq = up_r[kk];
+            } else {
+                const double* qk = (k == e.S_r - 1) ? e.r_q : e.rq_sub + (size_t)k * rM;
+                for (int j = r_e0; j < r_e1; ++j) sumq = sumq + qk[(size_t)e.rup_idx[j] * M + m];
+            }
             const KwPrep rp = kw_prepare(rq, rqlat_all, ralpha, dts, rfl);
             const double qin = 0.0 + sumq;
             double qn, ar;
