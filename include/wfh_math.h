@@ -68,6 +68,20 @@ WFH_FN double wfh_fabs(double x) { return wfh_from_bits(wfh_bits(x) & 0x7fffffff
 #define WFH_CEIL(x) ceil(x)
 #endif
 
+/* A constant of the series and reductions below. On the GPU it is materialised in scalar
+ * registers where it is used (an empty volatile asm takes it as an operand), instead of being
+ * hoisted out of a kernel's long loops and kept in vector registers, or spilled, for the whole
+ * loop. The value, and so every result, is the same. */
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ static inline double wfh_k(double c) {
+    asm volatile("" : "+s"(c));
+    return c;
+}
+#define WFH_K(c) wfh_k(c)
+#else
+#define WFH_K(c) (c)
+#endif
+
 /* 1 when any of a, b, c is NaN or infinite (all exponent bits set) */
 WFH_FN int wfh_nonfinite3(double a, double b, double c) {
     const uint64_t e = 0x7ff0000000000000ULL;
@@ -81,24 +95,24 @@ WFH_FN double wfh_exp(double x) {
     if (wfh_isnan(x)) return x;
     if (x > 709.782712893384) return wfh_from_bits(0x7ff0000000000000ULL);
     if (x < -745.1332191019412) return 0.0;
-    const double inv_ln2 = 0x1.71547652b82fep+0;
-    const double ln2_hi = 0x1.62e42ff000000p-1;
-    const double ln2_lo = -0x1.718432a1b0e26p-35;
+    const double inv_ln2 = WFH_K(0x1.71547652b82fep+0);
+    const double ln2_hi = WFH_K(0x1.62e42ff000000p-1);
+    const double ln2_lo = WFH_K(-0x1.718432a1b0e26p-35);
     const double k = WFH_RINT(x * inv_ln2);
     double r = WFH_FMA(-k, ln2_hi, x);
     r = WFH_FMA(-k, ln2_lo, r);
-    double p = 1.1470745597729725e-11;         /* 1/14! */
-    p = WFH_FMA(p, r, 1.6059043836821613e-10); /* 1/13! */
-    p = WFH_FMA(p, r, 2.08767569878681e-09);   /* 1/12! */
-    p = WFH_FMA(p, r, 2.505210838544172e-08);  /* 1/11! */
-    p = WFH_FMA(p, r, 2.755731922398589e-07);  /* 1/10! */
-    p = WFH_FMA(p, r, 2.7557319223985893e-06); /* 1/9! */
-    p = WFH_FMA(p, r, 2.48015873015873e-05);   /* 1/8! */
-    p = WFH_FMA(p, r, 0.0001984126984126984);  /* 1/7! */
-    p = WFH_FMA(p, r, 0.001388888888888889);   /* 1/6! */
-    p = WFH_FMA(p, r, 0.008333333333333333);   /* 1/5! */
-    p = WFH_FMA(p, r, 0.041666666666666664);   /* 1/4! */
-    p = WFH_FMA(p, r, 0.16666666666666666);    /* 1/3! */
+    double p = WFH_K(1.1470745597729725e-11);         /* 1/14! */
+    p = WFH_FMA(p, r, WFH_K(1.6059043836821613e-10)); /* 1/13! */
+    p = WFH_FMA(p, r, WFH_K(2.08767569878681e-09));   /* 1/12! */
+    p = WFH_FMA(p, r, WFH_K(2.505210838544172e-08));  /* 1/11! */
+    p = WFH_FMA(p, r, WFH_K(2.755731922398589e-07));  /* 1/10! */
+    p = WFH_FMA(p, r, WFH_K(2.7557319223985893e-06)); /* 1/9! */
+    p = WFH_FMA(p, r, WFH_K(2.48015873015873e-05));   /* 1/8! */
+    p = WFH_FMA(p, r, WFH_K(0.0001984126984126984));  /* 1/7! */
+    p = WFH_FMA(p, r, WFH_K(0.001388888888888889));   /* 1/6! */
+    p = WFH_FMA(p, r, WFH_K(0.008333333333333333));   /* 1/5! */
+    p = WFH_FMA(p, r, WFH_K(0.041666666666666664));   /* 1/4! */
+    p = WFH_FMA(p, r, WFH_K(0.16666666666666666));    /* 1/3! */
     p = WFH_FMA(p, r, 0.5);
     p = WFH_FMA(p, r, 1.0);
     p = WFH_FMA(p, r, 1.0);
@@ -129,23 +143,23 @@ WFH_FN double wfh_log(double x) {
     const double f = m - 1.0;
     const double s = f / (2.0 + f);
     const double z = s * s;
-    double R = 0.08;                           /* 2/25 */
-    R = WFH_FMA(R, z, 0.08695652173913043);     /* 2/23 */
-    R = WFH_FMA(R, z, 0.09523809523809523);     /* 2/21 */
-    R = WFH_FMA(R, z, 0.10526315789473684);     /* 2/19 */
-    R = WFH_FMA(R, z, 0.11764705882352941);     /* 2/17 */
-    R = WFH_FMA(R, z, 0.13333333333333333);     /* 2/15 */
-    R = WFH_FMA(R, z, 0.15384615384615385);     /* 2/13 */
-    R = WFH_FMA(R, z, 0.18181818181818182);     /* 2/11 */
-    R = WFH_FMA(R, z, 0.2222222222222222);      /* 2/9 */
-    R = WFH_FMA(R, z, 0.2857142857142857);      /* 2/7 */
-    R = WFH_FMA(R, z, 0.4);                     /* 2/5 */
-    R = WFH_FMA(R, z, 0.6666666666666666);      /* 2/3 */
+    double R = WFH_K(0.08);                           /* 2/25 */
+    R = WFH_FMA(R, z, WFH_K(0.08695652173913043));     /* 2/23 */
+    R = WFH_FMA(R, z, WFH_K(0.09523809523809523));     /* 2/21 */
+    R = WFH_FMA(R, z, WFH_K(0.10526315789473684));     /* 2/19 */
+    R = WFH_FMA(R, z, WFH_K(0.11764705882352941));     /* 2/17 */
+    R = WFH_FMA(R, z, WFH_K(0.13333333333333333));     /* 2/15 */
+    R = WFH_FMA(R, z, WFH_K(0.15384615384615385));     /* 2/13 */
+    R = WFH_FMA(R, z, WFH_K(0.18181818181818182));     /* 2/11 */
+    R = WFH_FMA(R, z, WFH_K(0.2222222222222222));      /* 2/9 */
+    R = WFH_FMA(R, z, WFH_K(0.2857142857142857));      /* 2/7 */
+    R = WFH_FMA(R, z, WFH_K(0.4));                     /* 2/5 */
+    R = WFH_FMA(R, z, WFH_K(0.6666666666666666));      /* 2/3 */
     R = R * z;
     const double hfsq = 0.5 * f * f;
     const double dk = (double)k;
-    const double ln2_hi = 0x1.62e42ff000000p-1;
-    const double ln2_lo = -0x1.718432a1b0e26p-35;
+    const double ln2_hi = WFH_K(0x1.62e42ff000000p-1);
+    const double ln2_lo = WFH_K(-0x1.718432a1b0e26p-35);
     return dk * ln2_hi + (f - (hfsq - (s * (hfsq + R) + dk * ln2_lo)));
 }
 

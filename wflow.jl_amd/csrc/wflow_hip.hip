@@ -48,6 +48,9 @@ namespace {
 constexpr uint64_t SLOT_EMPTY = 0xFFF40000DEADBEEFULL;  // signalling NaN, never produced by arithmetic
 constexpr int WAVE = 64;
 constexpr int ROUTE_BLOCK = 256;
+// Waves per SIMD the routing kernel is compiled for: two (at most 256 registers per lane), so
+// that one wave's memory round trips and slot polls overlap the other's arithmetic.
+constexpr int ROUTE_WAVES = 2;
 // Polls without progress (each with a sleep) before a wave gives up (RouteArgs::spin_limit). A
 // wave idle this long raises the abort flag; every other wave reads the flag after at most 256
 // idle polls of its own and leaves the kernel.
@@ -1115,7 +1118,8 @@ __device__ __forceinline__ void total_storage_finish(const Dev& d, const int p, 
 // well, from the step's forcing frame, and the arrays the steps hand on are accessed coherently.
 // A chunk's step can thus run while the long chains of the previous step are still draining.
 template <int N, bool EXT, bool MULTI = false>
-__global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
+__global__ void __launch_bounds__(ROUTE_BLOCK) __attribute__((amdgpu_waves_per_eu(ROUTE_WAVES))) k_route(Dev d,
+                                                                                                    RouteArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
     __shared__ int x_k[ROUTE_BLOCK];     // per-wave exchange tables of the routing families
     __shared__ double x_q[ROUTE_BLOCK];
@@ -1182,44 +1186,8 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             for (int k = 0; k < N; ++k) vo.us[k] = d.ustore[k * n + q_];
             vo.zi = d.zi[q_];
             vo.recharge = d.recharge[q_];
-            vo.sat_exc = d.sat_excess[q_];
-            vo.runoff_l = d.runoff_land[q_];
-            vo.inf_exc = d.inf_excess[q_];
-            vo.aeow_l = d.aeow_l[q_];
-            vo.nrr = d.net_runoff_river[q_];
-            vo.canopy = d.canopy[q_];
-            const double P_ = d.P[q_], aet_ = d.aet[q_], leak_ = d.leakage[q_], sp_ = d.store_prev[q_];
-            const double ss_ = d.snow_s[q_], sw_ = d.snow_w[q_];
-            vo.P = d.balance ? ((EXT && (fl_ & 8)) ? 0.0 : P_) : 0.0;
-            vo.aet = d.balance ? aet_ : 0.0;
-            vo.leak = d.balance ? leak_ : 0.0;
-            vo.store_prev = d.balance ? sp_ : 0.0;
-            vo.snow_s = d.snow ? ss_ : 0.0;
-            vo.snow_w = d.snow ? sw_ : 0.0;
         }
         const double rech = vo.recharge;
-        // Inputs of the later phases (soil storage, balances, overland, river, total storage),
-        // loaded with the chunk's first loads instead of one dependent round trip per phase.
-        // Every one of these arrays is written, in this kernel, only by the lane that reads it,
-        // after the read.
-        const double rd = ld(d.rd, q_);
-        const double snow_in = EXT ? d.snow_in[q_] : 0.0, snow_out = EXT ? d.snow_out[q_] : 0.0;
-        const double sat_exc = vo.sat_exc, runoff_l = vo.runoff_l, inf_exc = vo.inf_exc;
-        const double aeow = vo.aeow_l, nrr = vo.nrr, canopy = vo.canopy;
-        const double P_in = vo.P, aet = vo.aet;
-        const double leak = vo.leak, store_prev = vo.store_prev;
-        const double snow_s = vo.snow_s, snow_w = vo.snow_w;
-        const double alpha = ld(d.alpha, q_), sfw = ld(d.sfw, q_), rfrac = ld(d.rf, q_);
-        const double oq0 = ldv<MULTI>(d.ol_q + q_), oh0 = ldv<MULTI>(d.ol_h + q_),
-                     o_store_prev = ldv<MULTI>(d.ol_storage + q_);
-        const bool riv = r >= 0;
-        const int rr = riv ? r : 0;
-        const int re0 = d.rup_ptr[rr], re1_ = d.rup_ptr[riv ? rr + 1 : rr], re1 = riv ? re1_ : re0;
-        const double rfl = d.r_fl[rr], rfw = d.r_fw[rr], ralpha = d.r_alpha[rr], rarea = d.r_area[rr];
-        const int rres_ = EXT ? d.res_of_r[rr] : -1;
-        const int rres = riv ? rres_ : -1;
-        const double rq0 = ldv<MULTI>(d.r_q + rr), rh0 = ldv<MULTI>(d.r_h + rr),
-                     r_store_prev = ldv<MULTI>(d.r_storage + rr);
         Column<N> col;
         soil_layers<N>(dsoil, d.cfg_cum, d.cfg_thick, col.act, col.cum);
 #pragma unroll
@@ -1270,11 +1238,41 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
             return;
         if (col_upd) update_ustorelayerdepth<N>(col, col_z0, col_z1);
 
+        // Inputs of the later phases (soil storage, balances, overland flow), loaded once the
+        // subsurface family is done, so that they take no registers while it runs. Every one of
+        // these arrays is written, in this kernel, only by the lane that reads it, after the read.
+        if constexpr (!(FUSE_VERTICAL || MULTI)) {
+            vo.sat_exc = d.sat_excess[q_];
+            vo.runoff_l = d.runoff_land[q_];
+            vo.inf_exc = d.inf_excess[q_];
+            vo.aeow_l = d.aeow_l[q_];
+            vo.nrr = d.net_runoff_river[q_];
+            vo.canopy = d.canopy[q_];
+            const double P_ = d.P[q_], aet_ = d.aet[q_], leak_ = d.leakage[q_], sp_ = d.store_prev[q_];
+            const double ss_ = d.snow_s[q_], sw_ = d.snow_w[q_];
+            vo.P = d.balance ? ((EXT && (fl & 8)) ? 0.0 : P_) : 0.0;
+            vo.aet = d.balance ? aet_ : 0.0;
+            vo.leak = d.balance ? leak_ : 0.0;
+            vo.store_prev = d.balance ? sp_ : 0.0;
+            vo.snow_s = d.snow ? ss_ : 0.0;
+            vo.snow_w = d.snow ? sw_ : 0.0;
+        }
+        const double rd = ld(d.rd, q_), ts_ = ld(d.ts, q_), tr_ = ld(d.tr, q_);
+        const double snow_in = EXT ? d.snow_in[q_] : 0.0, snow_out = EXT ? d.snow_out[q_] : 0.0;
+        const double sat_exc = vo.sat_exc, runoff_l = vo.runoff_l, inf_exc = vo.inf_exc;
+        const double aeow = vo.aeow_l, nrr = vo.nrr, canopy = vo.canopy;
+        const double P_in = vo.P, aet = vo.aet;
+        const double leak = vo.leak, store_prev = vo.store_prev;
+        const double snow_s = vo.snow_s, snow_w = vo.snow_w;
+        const double alpha = ld(d.alpha, q_), sfw = ld(d.sfw, q_), rfrac = ld(d.rf, q_);
+        const double oq0 = ldv<MULTI>(d.ol_q + q_), oh0 = ldv<MULTI>(d.ol_h + q_),
+                     o_store_prev = ldv<MULTI>(d.ol_storage + q_);
+
         // ---------------- average_flux_vars!, update_soil_water_storage! ----------------
         SsfEnd se{0.0, 0.0, 0.0, 0.0};
         if (valid)
             se = ssf_finish<N, MULTI>(d, p, col,
-                               SsfCellIn{ts, tr, rd, dsoil, area, sy, sat_exc, runoff_l, inf_exc, aeow, nrr, P_in, aet,
+                               SsfCellIn{ts_, tr_, rd, dsoil, area, sy, sat_exc, runoff_l, inf_exc, aeow, nrr, P_in, aet,
                                          leak, canopy, snow_s, snow_w, store_prev, snow_in, snow_out},
                                cum, sq, szi);
         const double o_inwater = se.o_inwater, s_tor_avg = se.s_tor_avg, usd = se.usd, swd = se.swd;
@@ -1307,6 +1305,15 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) k_route(Dev d, RouteArgs a) {
                                        (size_t)n, p, (fl & 1) && !lcons_in, lcons_in ? (dn & 63) : -1, lds_k, lds_q,
                                        ol_compute))
             return;
+        // inputs of the river phase, loaded once the overland family is done
+        const bool riv = r >= 0;
+        const int rr = riv ? r : 0;
+        const int re0 = d.rup_ptr[rr], re1_ = d.rup_ptr[riv ? rr + 1 : rr], re1 = riv ? re1_ : re0;
+        const double rfl = d.r_fl[rr], rfw = d.r_fw[rr], ralpha = d.r_alpha[rr], rarea = d.r_area[rr];
+        const int rres_ = EXT ? d.res_of_r[rr] : -1;
+        const int rres = riv ? rres_ : -1;
+        const double rq0 = ldv<MULTI>(d.r_q + rr), rh0 = ldv<MULTI>(d.r_h + rr),
+                     r_store_prev = ldv<MULTI>(d.r_storage + rr);
         double r_inwater = 0.0;
         if (valid)
             r_inwater = ol_finish<MULTI>(d, p, r, oq, oh, o_storage, oqin, o_q_cum, o_tor_cum, o_qin_cum, o_inwater,
