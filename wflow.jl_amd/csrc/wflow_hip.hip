@@ -48,9 +48,10 @@ namespace {
 constexpr uint64_t SLOT_EMPTY = 0xFFF40000DEADBEEFULL;  // signalling NaN, never produced by arithmetic
 constexpr int WAVE = 64;
 constexpr int ROUTE_BLOCK = 256;
-// Waves per SIMD the routing kernel is compiled for: two (at most 256 registers per lane), so
-// that one wave's memory round trips and slot polls overlap the other's arithmetic.
-constexpr int ROUTE_WAVES = 2;
+// Waves per SIMD the routing kernel is compiled for: two (at most 256 registers per lane) up to
+// four soil layers, so that one wave's memory round trips and slot polls overlap the other's
+// arithmetic; one for five and six layers, whose column would spill at two.
+constexpr int route_waves(const int nlayers) { return nlayers <= 4 ? 2 : 1; }
 // Polls without progress (each with a sleep) before a wave gives up (RouteArgs::spin_limit). A
 // wave idle this long raises the abort flag; every other wave reads the flag after at most 256
 // idle polls of its own and leaves the kernel.
@@ -1118,7 +1119,7 @@ __device__ __forceinline__ void total_storage_finish(const Dev& d, const int p, 
 // well, from the step's forcing frame, and the arrays the steps hand on are accessed coherently.
 // A chunk's step can thus run while the long chains of the previous step are still draining.
 template <int N, bool EXT, bool MULTI = false>
-__global__ void __launch_bounds__(ROUTE_BLOCK) __attribute__((amdgpu_waves_per_eu(ROUTE_WAVES))) k_route(Dev d,
+__global__ void __launch_bounds__(ROUTE_BLOCK) __attribute__((amdgpu_waves_per_eu(route_waves(N)))) k_route(Dev d,
                                                                                                     RouteArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
     __shared__ int x_k[ROUTE_BLOCK];     // per-wave exchange tables of the routing families
