@@ -1124,8 +1124,10 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) __attribute__((amdgpu_waves_per_e
     const int lane = threadIdx.x & (WAVE - 1);
     __shared__ int x_k[ROUTE_BLOCK];     // per-wave exchange tables of the routing families
     __shared__ double x_q[ROUTE_BLOCK];
-    int* const lds_k = x_k + (threadIdx.x & ~(WAVE - 1));
-    double* const lds_q = x_q + (threadIdx.x & ~(WAVE - 1));
+    // the wave's table base, wave-uniform (a scalar register rather than a vector one per lane)
+    const int wbase = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~(WAVE - 1)));
+    int* const lds_k = x_k + wbase;
+    double* const lds_q = x_q + wbase;
     if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[a.parity ^ 1] = 0ull;  // next launch's counter
     const int n = d.n;
     const double dt = d.dt;
@@ -1143,7 +1145,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) __attribute__((amdgpu_waves_per_e
         // Loads are issued for every lane (inactive lanes read cell 0) and selected afterwards:
         // a load under a per-lane condition becomes a branch, and the compiler then waits for
         // it before the join instead of overlapping it with the others.
-        const int dn_ = d.down[q_], e0_ = d.up_ptr[q_], e1_ = d.up_ptr[q_ + 1], r_ = d.river_of[q_];
+        const int dn_ = d.down[q_], e0_ = d.up_ptr[q_], e1_ = d.up_ptr[q_ + 1];
         const uint8_t fl_ = d.flags[q_];
         const int dn = valid ? dn_ : -1;
         const bool cons_in = dn >= 0 && (dn >> 6) == c;
@@ -1151,7 +1153,6 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) __attribute__((amdgpu_waves_per_e
         const bool lcons_in = EXT ? (cons_in && (fl_ & 1)) : cons_in;
         const int e0 = e0_, e1 = valid ? e1_ : e0_;
         const uint8_t fl = valid ? fl_ : 0;
-        const int r = valid ? r_ : -1;
         if constexpr (MULTI) {
             if (!wait_chunks(a, c, dn >= 0 ? (dn >> 6) : c, step)) return;
         }
@@ -1258,7 +1259,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) __attribute__((amdgpu_waves_per_e
             vo.snow_s = d.snow ? ss_ : 0.0;
             vo.snow_w = d.snow ? sw_ : 0.0;
         }
-        const double rd = ld(d.rd, q_), ts_ = ld(d.ts, q_), tr_ = ld(d.tr, q_);
+        const double rd = ld(d.rd, q_);
         const double snow_in = EXT ? d.snow_in[q_] : 0.0, snow_out = EXT ? d.snow_out[q_] : 0.0;
         const double sat_exc = vo.sat_exc, runoff_l = vo.runoff_l, inf_exc = vo.inf_exc;
         const double aeow = vo.aeow_l, nrr = vo.nrr, canopy = vo.canopy;
@@ -1273,7 +1274,7 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) __attribute__((amdgpu_waves_per_e
         SsfEnd se{0.0, 0.0, 0.0, 0.0};
         if (valid)
             se = ssf_finish<N, MULTI>(d, p, col,
-                               SsfCellIn{ts_, tr_, rd, dsoil, area, sy, sat_exc, runoff_l, inf_exc, aeow, nrr, P_in, aet,
+                               SsfCellIn{ts, tr, rd, dsoil, area, sy, sat_exc, runoff_l, inf_exc, aeow, nrr, P_in, aet,
                                          leak, canopy, snow_s, snow_w, store_prev, snow_in, snow_out},
                                cum, sq, szi);
         const double o_inwater = se.o_inwater, s_tor_avg = se.s_tor_avg, usd = se.usd, swd = se.swd;
@@ -1307,6 +1308,8 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) __attribute__((amdgpu_waves_per_e
                                        ol_compute))
             return;
         // inputs of the river phase, loaded once the overland family is done
+        const int r_ = d.river_of[q_];
+        const int r = valid ? r_ : -1;
         const bool riv = r >= 0;
         const int rr = riv ? r : 0;
         const int re0 = d.rup_ptr[rr], re1_ = d.rup_ptr[riv ? rr + 1 : rr], re1 = riv ? re1_ : re0;
