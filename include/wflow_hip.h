@@ -403,15 +403,20 @@ int32_t wfh_load_state(wfh_handle* h, const void* buf, int64_t bytes);
 int32_t wfh_sync(wfh_handle* h);
 /* Kernel timing: with wfh_set_timing(h, 1) every following step records HIP events on the
  * handle's stream around each kernel; wfh_get_timing() returns the mean duration (ms) of the
- * vertical and the routing kernel over the steps recorded since timing was enabled. */
+ * vertical and the routing kernel over the steps recorded since timing was enabled. A plain
+ * model (no reservoirs, lateral snow transport, exponential-constant profile, floodplain or
+ * adaptive sub-steps; at most four layers) does the vertical update inside the routing launch:
+ * its vertical time is then about zero and the routing time covers the whole step. */
 int32_t wfh_set_timing(wfh_handle* h, int32_t enable);
 int32_t wfh_get_timing(wfh_handle* h, double* vertical_ms, double* routing_ms);
 /* Algorithmic HBM bytes of one launch of each kernel for this model: every input array read
  * once and every output array written once (parameter maps stored once when uniform count
  * 8 bytes), upstream discharge read once per edge and sub-step. Counts the fixed sub-steps'
  * single routing launch (with adaptive sub-steps the routing is one launch per sub-step and
- * this count does not apply). No counterpart in the reference; used for the roofline of the
- * measurement harness. */
+ * this count does not apply). When the routing launch does the vertical update (see
+ * wfh_get_timing), vertical_bytes is 0 and routing_bytes counts both halves, without the
+ * column state that stays in registers between them. No counterpart in the reference; used for
+ * the roofline of the measurement harness. */
 int32_t wfh_kernel_bytes(wfh_handle* h, int64_t* vertical_bytes, int64_t* routing_bytes);
 const char* wfh_last_error(const wfh_handle* h);
 void wfh_destroy(wfh_handle* h);

@@ -56,12 +56,6 @@ constexpr int route_waves(const int nlayers) { return nlayers <= 4 ? 2 : 1; }
 // wave idle this long raises the abort flag; every other wave reads the flag after at most 256
 // idle polls of its own and leaves the kernel.
 constexpr long SPIN_LIMIT = 2000000;
-// true: the routing kernel performs each cell's vertical update itself at the start of the
-// cell's chunk (one launch per step) instead of a separate vertical kernel before it. Measured
-// on MI355X (4096^2 hourly and the 64-catchment batch) the separate kernel is faster: it runs
-// at five waves per SIMD, while the routing kernel's one wave per SIMD leaves the vertical
-// update's latency exposed.
-constexpr bool FUSE_VERTICAL = false;
 
 struct Prm {  // parameter map, broadcast when uniform
     const double* p;
@@ -1112,13 +1106,16 @@ __device__ __forceinline__ void total_storage_finish(const Dev& d, const int p, 
 
 // EXT: the model has reservoirs or lateral snow transport (a separate instantiation keeps their
 // code, and its registers, out of the kernel that runs without them).
+// FUSE: the wave performs each cell's vertical update itself at the start of the chunk (one
+// launch per step instead of the vertical kernel before this one), so that the vertical work runs
+// beside the long chains of the network instead of before them.
 // MULTI: a run of a.nsteps steps in one launch (no EXT). Ticket t is step t / nchunks of chunk
 // t % nchunks, so tickets go out step after step; a chunk starts a step once it has completed the
 // previous one and every chunk it hands outflow to has completed the previous one too (which
 // emptied the slots it fills). The wave then does the cell's vertical update of that step as
 // well, from the step's forcing frame, and the arrays the steps hand on are accessed coherently.
 // A chunk's step can thus run while the long chains of the previous step are still draining.
-template <int N, bool EXT, bool MULTI = false>
+template <int N, bool EXT, bool MULTI = false, bool FUSE = false>
 __global__ void __launch_bounds__(ROUTE_BLOCK) __attribute__((amdgpu_waves_per_eu(route_waves(N)))) k_route(Dev d,
                                                                                                     RouteArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -1157,16 +1154,10 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) __attribute__((amdgpu_waves_per_e
             if (!wait_chunks(a, c, dn >= 0 ? (dn >> 6) : c, step)) return;
         }
 
-        // ---------------- lateral subsurface flow ----------------
-        const double area = ld(d.area, q_);
-        const double dsoil = ld(d.d, q_), ts = ld(d.ts, q_), tr = ld(d.tr, q_), tfc = ld(d.tfc, q_);
-        const double sy = ld(d.sy, q_), slope = ld(d.slope, q_), flen = ld(d.fl, q_), fw = ld(d.fw, q_);
-        const double qmax = ld(d.qmax, q_), kh0 = ld(d.kh0, q_), fk = ld(d.f, q_);
-        const double zexp = EXT ? ld(d.zexp, q_) : INFINITY;
-        // The cell's vertical update (the first half of the step), either done here at the start
-        // of the chunk or read back from the separate vertical kernel's outputs.
+        // The cell's vertical update (the first half of the step): done here at the start of the
+        // chunk (FUSE, MULTI), or its state outputs read back from the vertical kernel's.
         VertOut<N> vo;
-        if (FUSE_VERTICAL || MULTI) {
+        if constexpr (FUSE || MULTI) {
             if (valid) {
                 if constexpr (MULTI) {
                     Dev dv = d;
@@ -1189,6 +1180,12 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) __attribute__((amdgpu_waves_per_e
             vo.zi = d.zi[q_];
             vo.recharge = d.recharge[q_];
         }
+        // ---------------- lateral subsurface flow ----------------
+        const double area = ld(d.area, q_);
+        const double dsoil = ld(d.d, q_), ts = ld(d.ts, q_), tr = ld(d.tr, q_), tfc = ld(d.tfc, q_);
+        const double sy = ld(d.sy, q_), slope = ld(d.slope, q_), flen = ld(d.fl, q_), fw = ld(d.fw, q_);
+        const double qmax = ld(d.qmax, q_), kh0 = ld(d.kh0, q_), fk = ld(d.f, q_);
+        const double zexp = EXT ? ld(d.zexp, q_) : INFINITY;
         const double rech = vo.recharge;
         Column<N> col;
         soil_layers<N>(dsoil, d.cfg_cum, d.cfg_thick, col.act, col.cum);
@@ -1241,9 +1238,10 @@ __global__ void __launch_bounds__(ROUTE_BLOCK) __attribute__((amdgpu_waves_per_e
         if (col_upd) update_ustorelayerdepth<N>(col, col_z0, col_z1);
 
         // Inputs of the later phases (soil storage, balances, overland flow), loaded once the
-        // subsurface family is done, so that they take no registers while it runs. Every one of
-        // these arrays is written, in this kernel, only by the lane that reads it, after the read.
-        if constexpr (!(FUSE_VERTICAL || MULTI)) {
+        // subsurface family is done, so that they take no registers while it runs (with FUSE,
+        // the vertical fluxes this lane stored at the start of the chunk). Every one of these
+        // arrays is written, in this kernel, only by the lane that reads it, after the read.
+        if constexpr (!MULTI) {
             vo.sat_exc = d.sat_excess[q_];
             vo.runoff_l = d.runoff_land[q_];
             vo.inf_exc = d.inf_excess[q_];
@@ -2079,6 +2077,7 @@ struct wfh_handle {
     int64_t nres = 0;
     std::vector<double> fftr_land;  // flow fraction to river, land index order
     bool ext = false;               // reservoirs, lateral snow transport or an exponential-constant profile
+    bool fuse = false;              // the routing kernel does the vertical update (launch_step)
 };
 
 namespace {
@@ -3244,6 +3243,10 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         const int want = (h->nchunks + (ROUTE_BLOCK / WAVE) - 1) / (ROUTE_BLOCK / WAVE);
         h->route_grid = std::max(1, std::min(want, per_cu * cus));
         h->route_grid_max = h->route_grid;
+        // One launch per step (the vertical update inside the routing kernel) for the plain
+        // fixed-sub-step model at two routing waves per SIMD; the others keep the vertical kernel
+        // (lateral snow transport and reservoir forcing run between the two).
+        h->fuse = !h->ext && !adaptive && !fp && !linked && route_waves(N) == 2;
         if (adaptive || fp || linked) {
             // river positions: river cells sorted by node, a topological order of the river graph
             // because every river edge is a land edge and node order is topological
@@ -3371,16 +3374,26 @@ static void launch_step(wfh_handle* h, const Dev& d, const RouteArgs& base, int&
     if (ev) hipEventRecord(ev[0], h->stream);
     if (h->nres > 0)
         hipLaunchKernelGGL(k_res_forcing, dim3((unsigned)((h->nres + 63) / 64)), dim3(64), 0, h->stream, d);
-    if (!FUSE_VERTICAL) hipLaunchKernelGGL(k_vertical<N>, dim3((n + 255) / 256), dim3(256), 0, h->stream, d);
+    if (!h->fuse) hipLaunchKernelGGL(k_vertical<N>, dim3((n + 255) / 256), dim3(256), 0, h->stream, d);
     if (ev) hipEventRecord(ev[1], h->stream);
     if (d.snow_lat) launch_snow_transport(h, d, base, parity);
     RouteArgs a = base;
     a.parity = parity;
     parity ^= 1;
-    if (h->ext)
+    bool fused = false;
+    if constexpr (route_waves(N) == 2) {
+        if (h->fuse) {
+            hipLaunchKernelGGL((k_route<N, false, false, true>), dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream,
+                               d, a);
+            fused = true;
+        }
+    }
+    if (fused) {
+    } else if (h->ext) {
         hipLaunchKernelGGL((k_route<N, true>), dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, d, a);
-    else
+    } else {
         hipLaunchKernelGGL((k_route<N, false>), dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream, d, a);
+    }
     if (ev) hipEventRecord(ev[2], h->stream);
 }
 
@@ -4369,10 +4382,10 @@ extern "C" int32_t wfh_kernel_bytes(wfh_handle* h, int64_t* vertical_bytes, int6
     k2 += A * 9;                                          // overland outputs + total storage
     k2 += 8 * nr * 7;                                     // river outputs
     if (D.balance) k2 += 4 * A + 8 * nr;                  // balance errors
-    if (FUSE_VERTICAL) {
-        // one kernel: the routing part takes the vertical results from registers instead of
-        // reading back ustore[N], zi, recharge, 5 fluxes, canopy, snow and the balance inputs
-        k2 += k1 - (A * (N + 1) + A + A * 5 + A * 2 + (D.balance ? 4 * A : 0));
+    if (h->fuse) {
+        // one kernel: the routing part takes ustore[N], zi and recharge from registers instead of
+        // reading them back (the fluxes, canopy, snow and balance inputs are read back)
+        k2 += k1 - (A * (N + 1) + A);
         k1 = 0;
     }
     if (vertical_bytes) *vertical_bytes = k1;
