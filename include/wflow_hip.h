@@ -403,10 +403,10 @@ int32_t wfh_load_state(wfh_handle* h, const void* buf, int64_t bytes);
 int32_t wfh_sync(wfh_handle* h);
 /* Kernel timing: with wfh_set_timing(h, 1) every following step records HIP events on the
  * handle's stream around each kernel; wfh_get_timing() returns the mean duration (ms) of the
- * vertical and the routing kernel over the steps recorded since timing was enabled. A plain
- * model (no reservoirs, lateral snow transport, exponential-constant profile, floodplain or
- * adaptive sub-steps; at most four layers) does the vertical update inside the routing launch:
- * its vertical time is then about zero and the routing time covers the whole step. */
+ * vertical and the routing kernel over the steps recorded since timing was enabled. A model
+ * with fixed sub-steps, at most four layers and no lateral snow transport, floodplain or linked
+ * free weirs does the vertical update inside the routing launch: its vertical time is then
+ * about zero and the routing time covers the whole step. */
 int32_t wfh_set_timing(wfh_handle* h, int32_t enable);
 int32_t wfh_get_timing(wfh_handle* h, double* vertical_ms, double* routing_ms);
 /* Algorithmic HBM bytes of one launch of each kernel for this model: every input array read

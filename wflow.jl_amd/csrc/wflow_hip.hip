@@ -3243,10 +3243,10 @@ extern "C" int32_t wfh_create(const wfh_config* cfg, const wfh_static* st, wfh_h
         const int want = (h->nchunks + (ROUTE_BLOCK / WAVE) - 1) / (ROUTE_BLOCK / WAVE);
         h->route_grid = std::max(1, std::min(want, per_cu * cus));
         h->route_grid_max = h->route_grid;
-        // One launch per step (the vertical update inside the routing kernel) for the plain
-        // fixed-sub-step model at two routing waves per SIMD; the others keep the vertical kernel
-        // (lateral snow transport and reservoir forcing run between the two).
-        h->fuse = !h->ext && !adaptive && !fp && !linked && route_waves(N) == 2;
+        // One launch per step (the vertical update inside the routing kernel) for fixed
+        // sub-steps at two routing waves per SIMD; lateral snow transport keeps the vertical
+        // kernel, because the transport runs between the two halves of the step.
+        h->fuse = !D.snow_lat && !adaptive && !fp && !linked && route_waves(N) == 2;
         if (adaptive || fp || linked) {
             // river positions: river cells sorted by node, a topological order of the river graph
             // because every river edge is a land edge and node order is topological
@@ -3382,11 +3382,13 @@ static void launch_step(wfh_handle* h, const Dev& d, const RouteArgs& base, int&
     parity ^= 1;
     bool fused = false;
     if constexpr (route_waves(N) == 2) {
-        if (h->fuse) {
+        if (h->fuse && h->ext)
+            hipLaunchKernelGGL((k_route<N, true, false, true>), dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream,
+                               d, a);
+        else if (h->fuse)
             hipLaunchKernelGGL((k_route<N, false, false, true>), dim3(h->route_grid), dim3(ROUTE_BLOCK), 0, h->stream,
                                d, a);
-            fused = true;
-        }
+        fused = h->fuse;
     }
     if (fused) {
     } else if (h->ext) {
