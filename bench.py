@@ -527,7 +527,9 @@ def main():
     if os.path.exists(args.traffic_file):
         try:
             tf = json.load(open(args.traffic_file))
-            if tf.get("kernel") == dom and tf.get("n") == c.n and bool(tf.get("vertical_fused")) == fused:
+            # the counters were taken on the plain configuration of this grid
+            if (plain_cfg and tf.get("kernel") == dom and tf.get("n") == c.n
+                    and bool(tf.get("vertical_fused")) == fused):
                 traffic = tf.get("bytes_per_launch")
         except Exception:
             traffic = None
@@ -563,7 +565,7 @@ def main():
     }
     # FP64 VALU rate: FLOPs per launch from the counter run of the same workload over this run's
     # live kernel times
-    if os.path.exists(args.fp64_file) and not args.adaptive and not args.floodplain:
+    if os.path.exists(args.fp64_file) and plain_cfg:
         try:
             ff = json.load(open(args.fp64_file))
             if ff.get("n") == c.n and bool(ff.get("vertical_fused")) == fused:

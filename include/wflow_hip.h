@@ -414,8 +414,9 @@ int32_t wfh_get_timing(wfh_handle* h, double* vertical_ms, double* routing_ms);
  * 8 bytes), upstream discharge read once per edge and sub-step. Counts the fixed sub-steps'
  * single routing launch (with adaptive sub-steps the routing is one launch per sub-step and
  * this count does not apply). When the routing launch does the vertical update (see
- * wfh_get_timing), vertical_bytes is 0 and routing_bytes counts both halves, without the
- * column state that stays in registers between them. No counterpart in the reference; used for
+ * wfh_get_timing), vertical_bytes is 0 and routing_bytes counts both halves without any
+ * read-back of what the vertical half wrote (the launch does re-read the vertical fluxes, which
+ * shows as traffic above this count). No counterpart in the reference; used for
  * the roofline of the measurement harness. */
 int32_t wfh_kernel_bytes(wfh_handle* h, int64_t* vertical_bytes, int64_t* routing_bytes);
 const char* wfh_last_error(const wfh_handle* h);

@@ -4385,9 +4385,10 @@ extern "C" int32_t wfh_kernel_bytes(wfh_handle* h, int64_t* vertical_bytes, int6
     k2 += 8 * nr * 7;                                     // river outputs
     if (D.balance) k2 += 4 * A + 8 * nr;                  // balance errors
     if (h->fuse) {
-        // one kernel: the routing part takes ustore[N], zi and recharge from registers instead of
-        // reading them back (the fluxes, canopy, snow and balance inputs are read back)
-        k2 += k1 - (A * (N + 1) + A);
+        // one kernel: nothing the vertical update writes needs reading back, so none of those
+        // reads is algorithmic (the launch keeps ustore[N], zi and recharge in registers and
+        // re-reads the 5 fluxes, canopy, snow and balance inputs: traffic above this count)
+        k2 += k1 - (A * (N + 1) + A + A * 5 + A * 2 + (D.balance ? 4 * A : 0));
         k1 = 0;
     }
     if (vertical_bytes) *vertical_bytes = k1;
